@@ -1,0 +1,232 @@
+/*
+ * mrsim_gpu.h -- C ABI of the MI355X exact k-space DFT forward model (and its adjoint).
+ *
+ * The reference (PennLINC/mrsim-acq, Rust) has no FFI layer: its seam is the crate-internal
+ *   fn build_coil_kspace_timed(inp: &SliceInput, acq: &Acquisition, coil: usize, ncoils: usize,
+ *                              timing: &LineTiming, log_amp: Option<&[f64]>) -> Vec<C>
+ * (src/kspace.rs:607-609).  This header is the plain-C mirror of that call and of the POD types it
+ * reads, plus batched plan/execute entry points that run many (slice, volume, echo) units in one
+ * launch on one GPU.  Everything here is plain C: pointers and sizes, no C++ or torch types.
+ *
+ * Conventions (identical to the reference):
+ *   images on the simulation grid, index x + snx*y;  k-space on the acquired matrix, kx + nx*ky;
+ *   forward sign e^{+i2pi(...)} (src/kspace.rs:5-9);  unacquired rows are exactly 0.
+ *
+ * Every entry point returns MRS_OK (0) or a negative MRS_E* status and never aborts the process.
+ * When `err` is non-NULL the failure message (the reference's panic text where one exists) is
+ * written there, truncated to `errlen` bytes.
+ */
+#ifndef MRSIM_GPU_H
+#define MRSIM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRS_OK 0
+#define MRS_E_INVALID (-1)   /* bad shape / value: the reference's assert! or validator Err */
+#define MRS_E_DEVICE (-2)    /* a HIP call failed */
+#define MRS_E_NOMEM (-3)     /* device or host allocation failed */
+#define MRS_E_UNSUPPORTED (-4)
+
+#define MRS_MAX_COMP 4       /* compartments per unit the kernels are instantiated for */
+#define MRS_MAX_COILS 64
+
+/* Complex f64, the reference's `C { re, im }` (src/kspace.rs:29-33). */
+typedef struct { double re, im; } mrs_c64;
+/* Complex f32: the device output format (numpy/torch complex64 layout). */
+typedef struct { float re, im; } mrs_c32;
+
+/* PartialFourierMode (src/kspace.rs:84-106). */
+enum { MRS_PF_FIBERFOX = 0, MRS_PF_CONTIGUOUS = 1, MRS_PF_SCANNER = 2 };
+/* EchoFormation (src/kspace.rs:205-209). */
+enum { MRS_ECHO_SPIN = 0, MRS_ECHO_GRADIENT = 1 };
+/* KspaceWindow (src/kspace.rs:126-131): Tukey uses window_a = alpha; Fermi window_a = radius,
+ * window_b = width. */
+enum { MRS_WINDOW_NONE = 0, MRS_WINDOW_TUKEY = 1, MRS_WINDOW_HANN = 2, MRS_WINDOW_FERMI = 3 };
+
+/* Acquisition (src/kspace.rs:166-197), every field in the reference's units (ms, Hz, rad).
+ * mrs_acquisition_default() gives `Acquisition::default()` (src/kspace.rs:239-267). */
+typedef struct {
+    double t_line;          /* ms per PE line */
+    double t_echo;          /* ms */
+    double t_inhom;         /* ms, T2' used when a slice gives no t_inhom */
+    double signal_scale;
+    int32_t reverse_phase;
+    int32_t do_distortions;
+    int32_t do_relaxation;
+    int32_t pf_mode;        /* MRS_PF_* */
+    double noise_variance;
+    double partial_fourier;
+    double ghost_offset;
+    double eddy_strength;
+    double eddy_quad;
+    double eddy_phase;
+    double eddy_tau;        /* ms */
+    double spike_amplitude;
+    uint64_t n_spikes;
+    int32_t window;         /* MRS_WINDOW_* */
+    int32_t echo;           /* MRS_ECHO_* */
+    double window_a, window_b;
+    uint64_t n_coils;
+    uint64_t accel;
+    uint64_t acs_lines;
+    uint64_t seed;
+} mrs_acquisition;
+
+/* T2Slice (src/kspace.rs:380-384): `map` NULL means Uniform(value), else Map(map[snx*sny]). */
+typedef struct {
+    const float* map;
+    float value;
+} mrs_t2_slice;
+
+/* SliceInput (src/kspace.rs:458-489).  Host pointers.  `t_inhom` and `phase0` may be NULL
+ * (the reference's None).  `compartments[q]` and the maps are snx*sny, layout x + snx*y. */
+typedef struct {
+    const float* const* compartments;
+    size_t n_compartments;
+    const mrs_t2_slice* t2;        /* n_compartments entries */
+    const mrs_t2_slice* t_inhom;   /* n_compartments entries, or NULL */
+    const float* fmap;             /* Hz */
+    const double* phase0;          /* rad, or NULL */
+    size_t sim[2];                 /* [snx, sny] */
+    size_t acq_matrix[2];          /* [nx, ny] */
+    size_t z, nz;
+    int32_t has_eddy_drive;        /* eddy_drive: Option<[f64;3]> */
+    double eddy_drive[3];
+    int32_t has_prep_drive;        /* informational at slice level, as in the reference */
+    double prep_magnitude;
+    double prep_direction[3];
+    uint64_t slice_seed;
+} mrs_slice_input;
+
+/* LineTiming (src/kspace.rs:291-296): ny entries each. */
+typedef struct {
+    const double* t_ms;
+    const double* trf_ms;
+    const double* tread_ms;
+    const int8_t* polarity;
+    size_t ny;
+} mrs_line_timing;
+
+/* ---------------------------------------------------------------- host-side protocol tables */
+
+/* Acquisition::default() (src/kspace.rs:239-267). */
+int mrs_acquisition_default(mrs_acquisition* acq);
+
+/* LineTiming::for_acquisition (src/kspace.rs:310-326) built from SingleShotEpi
+ * (src/readout.rs:45-69) and line_times (src/kspace.rs:271-284).  Each output holds ny values. */
+int mrs_line_timing_for_acquisition(const mrs_acquisition* acq, size_t nx, size_t ny, double* t_ms,
+                                    double* trf_ms, double* tread_ms, int8_t* polarity);
+
+/* sampling_mask (src/kspace.rs:546-588): nx*ny flags (1 = acquired), layout kx + nx*ky. */
+int mrs_sampling_mask(size_t nx, size_t ny, const mrs_acquisition* acq, uint8_t* mask);
+
+/* coil_sensitivity (src/kspace.rs:226-237) evaluated on the simulation grid with the forward
+ * model's half-cell registration (src/kspace.rs:701-702), times `scale`, for coils
+ * [coil0, coil0 + n) of `ncoils`: out[c*snx*sny + x + snx*y], imaginary parts 0. */
+int mrs_ring_coil_maps(size_t coil0, size_t n, size_t ncoils, size_t snx, size_t sny, size_t nx, size_t ny,
+                       double scale, mrs_c32* out);
+
+/* ---------------------------------------------------------------- drop-in per-slice forward */
+
+/* build_coil_kspace_timed (src/kspace.rs:607-1042): one coil's acquired k-space, nx*ny complex
+ * f64 in `out` (layout kx + nx*ky), normalised by 1/(snx*sny), with spikes and thermal noise on
+ * acquired samples.  `timing` NULL uses LineTiming::for_acquisition; `log_amp` may be NULL.
+ * Host buffers; the sums run on the current HIP device in FP32 (split-BF16 MFMA) with FP64
+ * range-reduced phases. */
+int mrs_build_coil_kspace(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil, size_t ncoils,
+                          const mrs_line_timing* timing, const double* log_amp, mrs_c64* out, char* err,
+                          size_t errlen);
+
+/* build_coil_kspaces (src/kspace.rs:1248-1251): all acq->n_coils coils of one slice in one launch,
+ * out[coil*nx*ny + kx + nx*ky].  Same semantics as calling mrs_build_coil_kspace per coil. */
+int mrs_build_coil_kspaces(const mrs_slice_input* inp, const mrs_acquisition* acq, const mrs_line_timing* timing,
+                           const double* log_amp, mrs_c64* out, char* err, size_t errlen);
+
+/* ---------------------------------------------------------------- batched plan (device buffers) */
+
+/* Arithmetic of the contractions.  Phases are formed and range-reduced in FP64 in both. */
+enum {
+    MRS_PREC_SPLIT_BF16 = 0, /* three-term split-BF16 MFMA products, FP32 accumulation */
+    MRS_PREC_FP32 = 1        /* plain FP32 VALU contractions, one launch per unit ("plain") */
+};
+
+/* Geometry and protocol shared by a group of units. */
+typedef struct {
+    size_t snx, sny, nx, ny;
+    size_t n_coils;                  /* coils per unit (rows of the output) */
+    size_t n_compartments;           /* 1..MRS_MAX_COMP */
+    int32_t complex_images;          /* 0: real compartments + phase0; 1: complex64 images */
+    mrs_acquisition acq;
+    const mrs_line_timing* timing;   /* host table, NULL = mrs_line_timing_for_acquisition */
+} mrs_group_desc;
+
+/* Per-slice static inputs (device pointers), shared by every unit that names the slice. */
+typedef struct {
+    int32_t group;
+    const float* fmap;               /* P = snx*sny, Hz */
+    const double* phase0;            /* P rad, or NULL */
+    const float* t2;                 /* Q*P maps (ms) or NULL: t2_uniform[q] */
+    const float* t_inhom;            /* Q*P maps (ms) or NULL: t_inhom_uniform[q] if has_t_inhom,
+                                        else acq.t_inhom */
+    const mrs_c32* coil_maps;        /* n_coils*P complex sensitivities (signal_scale applied by
+                                        the plan) */
+    const double* log_amp;           /* P, or NULL */
+    float t2_uniform[MRS_MAX_COMP];
+    float t_inhom_uniform[MRS_MAX_COMP];
+    int32_t has_t_inhom;
+    size_t z, nz;
+    int32_t has_eddy_drive;
+    double eddy_drive[3];
+} mrs_slice_desc;
+
+/* One unit = one (slice, volume, echo) forward: n_coils x ny x nx complex64 out (unacquired rows
+ * zero).  `images`: Q planes of P float (real) or P complex64 (complex_images).  Device pointers. */
+typedef struct {
+    int32_t slice;
+    const void* images;
+    mrs_c32* out;
+    uint64_t slice_seed;             /* spikes and noise streams (src/kspace.rs:1013, 1031) */
+    uint64_t coil0;                  /* global index of the unit's first coil, for noise seeds */
+} mrs_unit_desc;
+
+typedef struct mrs_plan mrs_plan;
+
+/* Validate, precompute tables (line tables, twiddles) and allocate device scratch.  Pointers in
+ * `slices` and `units` are read on every execute; update them with mrs_plan_set_pointers. */
+int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, const mrs_slice_desc* slices, size_t nslices,
+                    const mrs_unit_desc* units, size_t nunits, int32_t precision, mrs_plan** plan, char* err,
+                    size_t errlen);
+int mrs_plan_set_pointers(mrs_plan* plan, const mrs_slice_desc* slices, const mrs_unit_desc* units, char* err,
+                          size_t errlen);
+/* Forward on `stream` (a hipStream_t; NULL = the null stream).  Asynchronous. */
+int mrs_plan_forward(mrs_plan* plan, void* stream, char* err, size_t errlen);
+/* Record the forward into a HIP graph once and replay it on later calls (same pointers). */
+int mrs_plan_forward_graph(mrs_plan* plan, void* stream, char* err, size_t errlen);
+
+/* Adjoint (vector-Jacobian product) of the noiseless forward.  Per unit u:
+ *   grad_out[u]     n_coils*ny*nx complex64 cotangent (PyTorch's conjugate-Wirtinger gradient)
+ *   grad_images[u]  Q planes of P complex64 (complex_images) or P float (real images), or NULL
+ *   grad_fmap[u]    P float (Hz), or NULL
+ *   grad_t_inhom[u] Q*P float (ms), or NULL
+ * Gradients are written (not accumulated). */
+int mrs_plan_vjp(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* grad_images, float* const* grad_fmap,
+                 float* const* grad_t_inhom, void* stream, char* err, size_t errlen);
+/* Terms of the literal sum per batch (sum of n_coils*nx*Ly*snx*sny) and acquired lines. */
+int mrs_plan_stats(const mrs_plan* plan, double* terms, double* lines, double* units);
+int mrs_plan_destroy(mrs_plan* plan);
+
+/* Kernel tile parameters chosen by the autotuner (see mrs_plan_tune). */
+int mrs_plan_tune(mrs_plan* plan, void* stream, int32_t trials, char* report, size_t reportlen);
+
+/* Library version string. */
+const char* mrs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
