@@ -1,0 +1,206 @@
+"""Benchmark of the batched exact k-space forward on one MI355X.
+
+Workload (default): 30 slices x 20 volumes x 8 coils x 3 compartments, 128^2 sim grid -> 64^2
+acquired, T2/T2' maps per compartment, spin-echo EPI (t_line 0.5 ms, TE 30 ms), synthetic inputs.
+A step is one forward of the whole batch (600 slice units) replayed from a HIP graph, inputs
+resident in HBM.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "mrsim-acq_amd"))
+sys.path.insert(0, ROOT)
+
+# MI355X published peaks
+PEAK_HBM_GBS = 8000.0
+PEAK_TRANS_TS = 256 * 16 * 2.4e9 / 1e12   # quarter-rate transcendental lanes: 9.83 T/s
+PEAK_FP32_VALU_TF = 157.3
+PEAK_BF16_MFMA_TF = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(budget_s: float = 20.0):
+    """The oracle's restatement of the reference's default exact-DFT path (rotor y-stage, twiddle
+    x-stage), one process per unit on the host cores as the reference's `par` runs volumes, on a
+    bounded sample of units of the same workload shape."""
+    import multiprocessing as mp
+
+    cores = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    one = _cpu_unit(0)  # single unit timing on one core decides the sample size
+    t1 = time.perf_counter() - t0
+    n = max(cores, int(budget_s / max(t1, 1e-3)) // cores * cores)
+    n = min(n, 4 * cores)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        pool.map(_cpu_unit, range(n))
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "slices/s", "cores": cores, "kind": "port",
+            "sample": f"{n} slice units (128^2->64^2, 8 coils, 3 compartments, T2/T2' maps) of the oracle's "
+                      f"restatement of build_coil_kspace_timed x 8 coils, {cores} processes; "
+                      f"1 unit on 1 core {t1:.2f} s"}
+
+
+def _cpu_unit(i):
+    import numpy as np
+
+    from mrsim_amd import abi
+    from oracle import oracle as O
+    from mrsim_amd import workloads as W
+    st = W.Stream(0xC0DE + i)
+    snx = sny = 128
+    P = snx * sny
+    comps = [st.unit(P).astype(np.float32) for _ in range(3)]
+    t2 = [(40 + 80 * st.unit(P)).astype(np.float32) for _ in range(3)]
+    ti = [(20 + 60 * st.unit(P)).astype(np.float32) for _ in range(3)]
+    fmap = (100 * (st.unit(P) - 0.5)).astype(np.float32)
+    b = abi.SliceInputBuilder(comps, t2, fmap, (snx, sny), (64, 64), t_inhom=ti, phase0=st.unit(P) * 6.28)
+    acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, n_coils=8)
+    for c in range(8):
+        O.coil_kspace(b, acq, c, 8)
+    return i
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--slices", type=int, default=30)
+    ap.add_argument("--volumes", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--check-units", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the forward has no cross-GPU step, and multi-GPU runs are not "
+              "part of this build (each GPU would take its own share of units).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from mrsim_amd import abi, workloads
+    torch.cuda.set_device(0)
+    t_gen = time.perf_counter()
+    w = workloads.make("batch_128", n_slices=args.slices, n_volumes=args.volumes)
+    log(f"workload generated in {time.perf_counter() - t_gen:.1f} s: {len(w.units)} units")
+    plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
+    st = torch.cuda.current_stream()
+    tune_report = plan.tune(trials=2)
+    log(tune_report.strip())
+    nunits = len(w.units)
+
+    # warm-up, then K graph replays bracketed by synchronisation
+    for _ in range(args.warmup):
+        plan.forward_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.forward_graph()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_step = 1e3 * dt / args.steps
+    sps = nunits * args.steps / dt
+    log(f"split-bf16 batched: {ms_step:.3f} ms/step, {sps:.0f} slices/s")
+
+    # per-stage kernel durations with HIP events on the launch stream
+    stage_ms = plan.stage_times(iters=max(3, args.steps // 2))
+    log(f"stage times (ms): {stage_ms}")
+
+    # parity of sampled units against the FP64 oracle
+    errs = []
+    if args.check_units > 0:
+        from oracle import oracle as O
+        idx = np.linspace(0, nunits - 1, args.check_units).astype(int)
+        for u in idx:
+            want = O.forward_workload_unit(w, int(u))
+            got = outs[int(u)].cpu().numpy()
+            errs.append(float(np.linalg.norm(got - want) / np.linalg.norm(want)))
+        log(f"rel-L2 vs FP64 oracle on units {list(idx)}: {errs}")
+
+    # the plain FP32 per-slice path (one launch pair per unit) on the same batch
+    plain = None
+    if not args.no_plain:
+        pplan, pouts, pkeep = workloads.to_plan(w, abi.PREC_FP32)
+        pplan.forward()
+        torch.cuda.synchronize()
+        ksteps = max(1, args.steps // 4)
+        t0 = time.perf_counter()
+        for _ in range(ksteps):
+            pplan.forward()
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - t0
+        plain = nunits * ksteps / pdt
+        log(f"plain fp32 per-slice: {1e3 * pdt / ksteps:.3f} ms/step, {plain:.0f} slices/s")
+        pplan.close()
+
+    stats = plan.stats()
+    terms = stats["terms"]
+    lines = stats["lines"]
+    P = w.snx * w.sny
+    Q, C = w.n_comp, w.n_coils
+    # algorithmic work of the y-stage per launch (whole batch): transcendentals of the literal sum
+    trans = lines * P * (Q + 2)
+    y_ms = stage_ms["ystage"]
+    achieved = trans / (y_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "trans",
+        "achieved": round(achieved, 4),
+        "peak": round(PEAK_TRANS_TS, 3),
+        "unit": "Ttrans/s",
+        "frac": round(achieved / PEAK_TRANS_TS, 4),
+        "traffic": None,
+        "kernel": "ystage_mfma",
+        "kernel_ms": round(y_ms, 4),
+        "per_unit": f"Ly*P*(Q+2) = {int(lines / nunits)}*{P}*{Q + 2} transcendentals",
+        "secondary": {
+            "valu_fp32_tflops": round(8.0 * C * P * lines / (y_ms * 1e-3) / 1e12, 2),
+            "hbm_gbs": round((nunits * (Q * P * 4 + C * w.nx * lines / nunits * 8) +
+                              len(w.slices) * P * (4 + 8 + 8 * Q + 8 * C)) / (y_ms * 1e-3) / 1e9, 1),
+            "xstage_mfma_tflops": round(3 * 8.0 * C * lines * w.nx * w.snx / (stage_ms["xstage"] * 1e-3) / 1e12, 2),
+        },
+    }
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    out = {
+        "metric": "slices/s (128^2->64^2, 8 coils) batched split-BF16 forward",
+        "value": round(sps, 1),
+        "unit": "slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "split-bf16 mfma / fp32 acc / fp64 phase",
+        "data": "synthetic",
+        "config": {"workload": "batch_128: 30 slices x 20 volumes x 8 coils x 3 compartments, 128^2->64^2, "
+                               "T2/T2' maps, one hipGraph per step",
+                   "units_per_step": nunits, "terms_per_step": terms},
+        "terms_per_s": round(terms * args.steps / dt, 1),
+        "plain_fp32_slices_per_s": None if plain is None else round(plain, 1),
+        "batched_over_plain": None if plain is None else round(sps / plain, 2),
+        "rel_l2_vs_fp64": errs,
+        "stage_ms": stage_ms,
+        "tiles": plan.tiles(),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

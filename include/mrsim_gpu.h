@@ -216,12 +216,27 @@ int mrs_plan_forward_graph(mrs_plan* plan, void* stream, char* err, size_t errle
  * Gradients are written (not accumulated). */
 int mrs_plan_vjp(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* grad_images, float* const* grad_fmap,
                  float* const* grad_t_inhom, void* stream, char* err, size_t errlen);
+/* Average duration (ms) of each forward stage over `iters` eager launches, timed with HIP events
+ * on `stream`: ms[0] slice prep, ms[1] y-stage, ms[2] x-stage, ms[3] zero rows + spikes/noise. */
+int mrs_plan_stage_times(mrs_plan* plan, void* stream, int32_t iters, double* ms, char* err, size_t errlen);
 /* Terms of the literal sum per batch (sum of n_coils*nx*Ly*snx*sny) and acquired lines. */
 int mrs_plan_stats(const mrs_plan* plan, double* terms, double* lines, double* units);
 int mrs_plan_destroy(mrs_plan* plan);
 
 /* Kernel tile parameters chosen by the autotuner (see mrs_plan_tune). */
 int mrs_plan_tune(mrs_plan* plan, void* stream, int32_t trials, char* report, size_t reportlen);
+
+/* Set / read the y-stage tiles directly: x columns per block (1..64) and 16-line tiles per pass
+ * (1, 2 or 4). */
+int mrs_plan_set_tiles(mrs_plan* plan, int32_t x_tile, int32_t line_tiles, char* err, size_t errlen);
+int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tiles);
+
+/* Size of a public struct as compiled into the library (ABI check for bindings). */
+size_t mrs_sizeof(const char* name);
+
+/* Device evaluation of sin/cos(2 pi cyc[i]) exactly as the kernels evaluate their phases (host
+ * buffers); lets callers measure the phase arithmetic's error. */
+int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, char* err, size_t errlen);
 
 /* Library version string. */
 const char* mrs_version(void);

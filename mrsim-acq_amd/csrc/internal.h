@@ -1,0 +1,102 @@
+// internal.h -- device-side records shared by the HIP kernels (kernels.hip) and the host plan
+// (plan.cpp).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/mrsim_gpu.h"
+
+namespace mrs {
+
+// One acquired phase-encode line of a timing group, in the order the kernels walk them: sorted by
+// x-stage table (the Nyquist-ghost polarity), then ky; each table segment padded to a multiple of
+// 16 lines with ky = -1 records.
+struct LineRec {
+    double t_s;    // fieldmap phase time t_ms/1000 (s), src/kspace.rs:844
+    double kyn;    // (ky - ny/2) / sny, src/kspace.rs:729
+    double d;      // eddy clock exp(-tread/tau_eddy) * t (s), src/kspace.rs:910; 0 without eddy
+    float trf;     // ms from the RF (T2 decay), src/kspace.rs:845
+    float tau;     // ms of T2' decay: |t|*1000 (spin echo) or trf (gradient echo), :896-897
+    float tb0;     // s of fieldmap phase accrued (t, or t + TE/1000 for gradient echo): d phase/d fmap
+    int32_t ky;    // output row, -1 for padding
+    int32_t table; // x-stage table 0 (+ghost) or 1 (-ghost), src/kspace.rs:987, 1109
+    int32_t pad;
+};
+static_assert(sizeof(LineRec) == 48, "LineRec layout");
+
+// Static per-slice planes, written by the prep kernel, all in [x][y] (y fastest) order so a lane's
+// four consecutive y values are one 16-byte load.
+struct DevSlice {
+    double* base;      // P: object phase in cycles (phase0 + eddy ramp)/2pi [+ fmap*TE/1000 for GE]
+    float* rate;       // P: fieldmap (Hz) if do_distortions else 0
+    double* poly;      // P: eddy polynomial (cycles per second of eddy clock), 0 without eddy
+    float* nr2;        // Q*P: -log2(e)/T2 (1/ms), 0 without relaxation
+    float* nri;        // Q*P: -log2(e)/T2' (1/ms)
+    uint32_t* s_hi;    // [x][c][y] packed bf16 (re | im<<16) of signal_scale*exp(log_amp)*S_c
+    uint32_t* s_lo;    // residual of the same after s_hi, bf16
+    float2* s32;       // [c][y][x] the same in FP32 (plain path)
+    // user inputs (device)
+    const float* fmap;
+    const double* phase0;
+    const float* t2;
+    const float* t_inhom;
+    const mrs_c32* coil_maps;
+    const double* log_amp;
+    float t2_uniform[MRS_MAX_COMP];
+    float ti_uniform[MRS_MAX_COMP];
+    int32_t has_t_inhom;
+    int32_t group;
+    int32_t z, nz;
+    int32_t has_eddy;
+    double eddy[3];
+};
+
+struct DevGroup {
+    int32_t snx, sny, nx, ny;
+    int32_t C, Q, RB;          // coils, compartments, 16-row blocks of 2C
+    int32_t Ly;                // acquired lines
+    int32_t LyP;               // padded lines (multiple of 16)
+    int32_t cplx;              // complex images
+    int32_t echo, do_relax, do_dist;
+    double sys_yoff;           // yy = y - (sys + yoff)
+    double sxs_xoff;
+    int32_t ox, oy;
+    double t_echo, t_inhom, signal_scale;
+    double eddy_strength, eddy_quad, eddy_phase;
+    const LineRec* lines;      // LyP records
+    const uint32_t* t_hi[2];   // [kx][x] packed bf16 twiddles / (snx*sny)
+    const uint32_t* t_lo[2];
+    const float2* t32[2];      // [kx][x] FP32 twiddles / (snx*sny)
+    const int32_t* zero_rows;  // unacquired ky rows
+    int32_t n_zero_rows;
+    int32_t noise;             // noise_variance > 0
+    int32_t n_spikes;
+    double noise_sigma;
+    double spike_amplitude;
+};
+
+struct DevUnit {
+    const void* images;   // Q planes of P float or complex64 (user layout x + snx*y)
+    float2* out;          // C x ny x nx
+    float2* gbuf;         // y-stage result [x][LyP][C] complex f32 (plan scratch)
+    int32_t slice;
+    int32_t group;
+    uint64_t seed;
+    uint64_t coil0;
+};
+
+// Work item of the batched kernels.
+struct WorkItem {
+    int32_t unit;
+    int32_t tile;   // x tile (y-stage) or column tile (x-stage)
+};
+
+struct VjpUnit {
+    const float2* gout;   // cotangent C x ny x nx
+    float2* gimg;         // Q planes of P (complex) or P float (real: Q planes float)
+    float* gfmap;         // P
+    float* gti;           // Q*P
+    float2* gbar;         // x-adjoint result [x][LyP][C] (plan scratch)
+};
+
+}  // namespace mrs

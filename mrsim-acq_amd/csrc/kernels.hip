@@ -1,0 +1,682 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the exact k-space DFT forward model.
+//
+// Per slice unit, coil c, acquired line l (ky), readout sample kx (src/kspace.rs:607-1042):
+//   K_c[kx,ky] = sum_x T_tab(l)[kx,x] * sum_y S_c(x,y) * E(x,y,l)
+//   E(x,y,l)   = W(x,y,l) * cis(2 pi cyc(x,y,l))
+//   W          = sum_q comp_q(x,y) * exp2(trf_l * nr2_q(x,y) + tau_l * nri_q(x,y))     (relaxation)
+//   cyc        = base(x,y) + rate(x,y)*t_l + kyn_l*(y - sys - yoff) + poly(x,y)*d_l  (FP64)
+// where T_tab is the x-stage twiddle table of the line's ghost polarity divided by snx*sny,
+// S_c = signal_scale * exp(log_amp) * sensitivity, and the eddy polynomial's x, y and z factors
+// (src/kspace.rs:908-925, 971, 977-984) are one phase term poly*d.  The phase is formed and
+// range-reduced in FP64, then evaluated by an FP32 sincos of the reduced argument.
+//
+// Two arithmetic variants:
+//   split-BF16  y- and x-contractions on MFMA (v_mfma_f32_16x16x32_bf16), every operand split
+//               into bf16 hi + lo and multiplied as hi*hi + hi*lo + lo*hi with FP32 accumulation;
+//               E is generated in registers directly in the MFMA B-operand layout (no LDS copy).
+//   plain FP32  one thread per (x, line) summing over y on the VALU, then per (row, kx).
+#include "internal.h"
+
+namespace mrs {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+
+static constexpr double kTau = 6.283185307179586476925286766559;
+static constexpr float kLog2e = 1.4426950408889634f;
+
+// ----------------------------------------------------------------------------- small helpers
+
+__device__ __forceinline__ uint32_t bf16_bits(float v) {
+    __bf16 h = (__bf16)v;  // round to nearest even (v_cvt_pk_bf16_f32)
+    return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ float bf16_float(uint32_t b) { return __builtin_bit_cast(float, b << 16); }
+
+// (a, b) -> packed hi = bf16(a) | bf16(b) << 16, lo = bf16 of the residuals.
+__device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+    uint32_t ha = bf16_bits(a), hb = bf16_bits(b);
+    float ra = a - bf16_float(ha), rb = b - bf16_float(hb);
+    hi = ha | (hb << 16);
+    lo = bf16_bits(ra) | (bf16_bits(rb) << 16);
+}
+
+// A-operand rows of a complex contraction out = sum S * v, split into (re row, im row):
+//   re row: [S_re, -S_im] per complex k;  im row: [S_im, S_re].
+// `p` holds four packed (re | im<<16) complex values.
+__device__ __forceinline__ u32x4 rows_of(u32x4 p, int part) {
+    u32x4 r;
+    if (part == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = p[i] ^ 0x80000000u;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = (p[i] >> 16) | (p[i] << 16);
+    }
+    return r;
+}
+
+__device__ __forceinline__ f32x4 mfma3(u32x4 ahi, u32x4 alo, u32x4 bhi, u32x4 blo, f32x4 acc) {
+    bf16x8 Ah = __builtin_bit_cast(bf16x8, ahi), Al = __builtin_bit_cast(bf16x8, alo);
+    bf16x8 Bh = __builtin_bit_cast(bf16x8, bhi), Bl = __builtin_bit_cast(bf16x8, blo);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, acc, 0, 0, 0);
+    return acc;
+}
+
+// sin and cos of 2*pi*cyc.  The argument is reduced in FP64 to the nearest quarter turn, and the
+// FP32 residual f in [-1/8, 1/8] turn is evaluated by Taylor polynomials of sin and cos about 0
+// (degree 9 and 10 in theta = 2 pi f; truncation below 2e-9 at |theta| = pi/4).
+__device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
+#if MRS_HW_SINCOS
+    double rq = cyc - __builtin_rint(cyc);
+    float r = (float)rq;
+    s = __builtin_amdgcn_sinf(r);
+    c = __builtin_amdgcn_cosf(r);
+#else
+    double q4 = __builtin_rint(cyc * 4.0);
+    float f = (float)__builtin_fma(q4, -0.25, cyc);
+    int k = (int)q4 & 3;
+    float th = f * 6.2831853071795864769f;
+    float t2 = th * th;
+    float ps = __builtin_fmaf(t2, 2.7557319223985893e-06f, -1.9841269841269841e-04f);
+    ps = __builtin_fmaf(t2, ps, 8.3333333333333333e-03f);
+    ps = __builtin_fmaf(t2, ps, -1.6666666666666667e-01f);
+    ps = t2 * ps;
+    float sn = __builtin_fmaf(th, ps, th);
+    float pc = __builtin_fmaf(t2, -2.7557319223985891e-07f, 2.4801587301587302e-05f);
+    pc = __builtin_fmaf(t2, pc, -1.3888888888888889e-03f);
+    pc = __builtin_fmaf(t2, pc, 4.1666666666666667e-02f);
+    pc = __builtin_fmaf(t2, pc, -0.5f);
+    float cs = __builtin_fmaf(t2, pc, 1.0f);
+    // rotate by k quarter turns
+    float s1 = (k & 1) ? cs : sn;
+    float c1 = (k & 1) ? sn : cs;
+    s = (k & 2) ? -s1 : s1;
+    c = ((k + 1) & 2) ? -c1 : c1;
+#endif
+}
+
+// ----------------------------------------------------------------------------- slice prep
+// Static per-slice planes (src/kspace.rs:678-725, 908-925): one thread per padded voxel.
+__global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices, const DevGroup* __restrict__ groups,
+                                                   int nslices) {
+    const int s = blockIdx.y;
+    if (s >= nslices) return;
+    DevSlice& sl = slices[s];
+    const DevGroup& gr = groups[sl.group];
+    const int snx = gr.snx, sny = gr.sny;
+    const int snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
+    const int C = gr.C, Q = gr.Q;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= snxP * snyP) return;
+    const int y = idx % snyP, x = idx / snyP;
+    const size_t o = (size_t)x * snyP + y;
+    const size_t PP = (size_t)snxP * snyP;
+    const bool valid = x < snx && y < sny;
+    const int i = x + snx * y;
+    const int nx = gr.nx, ny = gr.ny;
+    double base = 0.0, poly = 0.0;
+    float rate = 0.0f;
+    double amp = gr.signal_scale;
+    if (valid) {
+        const double xc = ((double)x - gr.sxs_xoff) / (double)gr.ox;
+        const double yc = ((double)y - gr.sys_yoff) / (double)gr.oy;
+        const double zc = (double)sl.z - (double)(sl.nz / 2);
+        const double zc_c = (double)sl.z - ((double)sl.nz - 1.0) / 2.0;
+        const double g0 = sl.eddy[0], g1 = sl.eddy[1], g2 = sl.eddy[2];
+        double p0 = sl.phase0 ? sl.phase0[i] : 0.0;
+        if (gr.eddy_phase != 0.0 && sl.has_eddy) p0 += gr.eddy_phase * (g0 * xc + g1 * yc + g2 * zc_c);
+        base = p0 / kTau;
+        if (gr.echo == MRS_ECHO_GRADIENT && gr.do_dist) base += (double)sl.fmap[i] * (gr.t_echo / 1000.0);
+        rate = gr.do_dist ? sl.fmap[i] : 0.0f;
+        if (gr.eddy_strength != 0.0 && sl.has_eddy) {
+            poly = gr.eddy_strength * (g0 * xc + g1 * yc + g2 * zc) +
+                   gr.eddy_quad * (g0 * xc * xc + g1 * yc * yc + g2 * zc * zc);
+        }
+        if (sl.log_amp) amp *= exp(sl.log_amp[i]);
+    }
+    sl.base[o] = base;
+    sl.rate[o] = rate;
+    sl.poly[o] = poly;
+    for (int q = 0; q < Q; q++) {
+        float a = 0.0f, b = 0.0f;
+        if (valid && gr.do_relax) {
+            float t2 = sl.t2 ? sl.t2[(size_t)q * snx * sny + i] : sl.t2_uniform[q];
+            float ti = sl.t_inhom ? sl.t_inhom[(size_t)q * snx * sny + i]
+                                  : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
+            a = (float)(-1.4426950408889634 / (double)t2);
+            b = (float)(-1.4426950408889634 / (double)ti);
+        }
+        sl.nr2[q * PP + o] = a;
+        sl.nri[q * PP + o] = b;
+    }
+    for (int c = 0; c < C; c++) {
+        float re = 0.0f, im = 0.0f;
+        if (valid) {
+            mrs_c32 v = sl.coil_maps[(size_t)c * snx * sny + i];
+            re = (float)((double)v.re * amp);
+            im = (float)((double)v.im * amp);
+        }
+        uint32_t hi, lo;
+        split2(re, im, hi, lo);
+        const size_t so = ((size_t)x * C + c) * snyP + y;
+        sl.s_hi[so] = hi;
+        sl.s_lo[so] = lo;
+        sl.s32[((size_t)c * snyP + y) * snxP + x] = make_float2(re, im);
+    }
+    (void)nx;
+    (void)ny;
+}
+
+// ----------------------------------------------------------------------------- E generation
+// The four voxels (x, y0..y0+3) a lane evaluates for every line it owns.
+template <int Q, bool CPLX, bool EDDY>
+struct Vox4 {
+    double base[4];
+    double poly[EDDY ? 4 : 1];
+    float rate[4];
+    float nr2[Q][4], nri[Q][4];
+    float cr[Q][4];
+    float ci[CPLX ? Q : 1][4];
+};
+
+template <int Q, bool CPLX, bool EDDY>
+__device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice& sl, const void* images, int snx,
+                                          int sny, int x, int y0, int snyP, size_t PP) {
+    const size_t o = (size_t)x * snyP + y0;
+    const double2* b2 = reinterpret_cast<const double2*>(sl.base + o);
+    double2 b01 = b2[0], b23 = b2[1];
+    v.base[0] = b01.x; v.base[1] = b01.y; v.base[2] = b23.x; v.base[3] = b23.y;
+    if constexpr (EDDY) {
+        const double2* p2 = reinterpret_cast<const double2*>(sl.poly + o);
+        double2 p01 = p2[0], p23 = p2[1];
+        v.poly[0] = p01.x; v.poly[1] = p01.y; v.poly[2] = p23.x; v.poly[3] = p23.y;
+    }
+    float4 r4 = *reinterpret_cast<const float4*>(sl.rate + o);
+    v.rate[0] = r4.x; v.rate[1] = r4.y; v.rate[2] = r4.z; v.rate[3] = r4.w;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float4 a = *reinterpret_cast<const float4*>(sl.nr2 + q * PP + o);
+        float4 b = *reinterpret_cast<const float4*>(sl.nri + q * PP + o);
+        v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
+        v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
+    }
+    const size_t P = (size_t)snx * sny;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int y = y0 + j;
+        const bool ok = x < snx && y < sny;
+        const size_t i = (size_t)x + (size_t)snx * y;
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if constexpr (CPLX) {
+                float2 m = ok ? reinterpret_cast<const float2*>(images)[q * P + i] : make_float2(0.f, 0.f);
+                v.cr[q][j] = m.x;
+                v.ci[q][j] = m.y;
+            } else {
+                v.cr[q][j] = ok ? reinterpret_cast<const float*>(images)[q * P + i] : 0.0f;
+            }
+        }
+    }
+}
+
+// E(x, y0+j, line) for one j: yy = y0 + j - (sys + yoff)
+template <int Q, bool CPLX, bool EDDY>
+__device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, double yy, double t_s, double kyn,
+                                      double d, float trf, float tau, float& er, float& ei) {
+    double cyc = __builtin_fma(kyn, yy, v.base[j]);
+    cyc = __builtin_fma((double)v.rate[j], t_s, cyc);
+    if constexpr (EDDY) cyc = __builtin_fma(v.poly[j], d, cyc);
+    float sn, cs;
+    sincos_turns(cyc, sn, cs);
+    float wr = 0.0f, wi = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
+        wr = __builtin_fmaf(v.cr[q][j], dq, wr);
+        if constexpr (CPLX) wi = __builtin_fmaf(v.ci[q][j], dq, wi);
+    }
+    if constexpr (CPLX) {
+        er = wr * cs - wi * sn;
+        ei = wr * sn + wi * cs;
+    } else {
+        er = wr * cs;
+        ei = wr * sn;
+    }
+}
+
+// ----------------------------------------------------------------------------- y-stage, MFMA
+// Block = 4 waves = one (unit, x tile of `xt` columns).  Per column x a wave accumulates
+//   G[(coil, part), line] = sum_{y, part'} A[(coil,part), (y,part')] * B[(y,part'), line]
+// with A the split coil sensitivities and B = E generated in registers, 16 lines per MFMA tile,
+// GP tiles per pass and RB 16-row blocks (8 coils each).
+template <int Q, int RB, int GP, bool CPLX, bool EDDY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
+                 const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work, int xt) {
+    __shared__ LineRec lines_s[256];
+    const WorkItem wi = work[blockIdx.x];
+    const DevUnit& u = units[wi.unit];
+    const DevGroup& gr = groups[u.group];
+    const DevSlice& sl = slices[u.slice];
+    for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane >> 4, ln = lane & 15;
+    const int snx = gr.snx, sny = gr.sny;
+    const int snyP = (sny + 15) & ~15;
+    const int snxP = (snx + 15) & ~15;
+    const size_t PP = (size_t)snxP * snyP;
+    const int C = gr.C, LyP = gr.LyP, G = LyP >> 4;
+    const int part = ln >> 3;
+    const double sys_yoff = gr.sys_yoff;
+    for (int xi = wave; xi < xt; xi += 4) {
+        const int x = wi.tile * xt + xi;
+        if (x >= snx) break;
+        for (int g0 = 0; g0 < G; g0 += GP) {
+            f32x4 acc[RB][GP];
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int g = 0; g < GP; g++) acc[rb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int ch = 0; ch < snyP; ch += 16) {
+                const int y0 = ch + lq * 4;
+                Vox4<Q, CPLX, EDDY> v;
+                load_vox4<Q, CPLX, EDDY>(v, sl, u.images, snx, sny, x, y0, snyP, PP);
+                u32x4 ahi[RB], alo[RB];
+#pragma unroll
+                for (int rb = 0; rb < RB; rb++) {
+                    const int c = rb * 8 + (ln & 7);
+                    if (c < C) {
+                        const size_t so = ((size_t)x * C + c) * snyP + y0;
+                        u32x4 h = *reinterpret_cast<const u32x4*>(sl.s_hi + so);
+                        u32x4 l = *reinterpret_cast<const u32x4*>(sl.s_lo + so);
+                        ahi[rb] = rows_of(h, part);
+                        alo[rb] = rows_of(l, part);
+                    } else {
+                        ahi[rb] = u32x4{0u, 0u, 0u, 0u};
+                        alo[rb] = u32x4{0u, 0u, 0u, 0u};
+                    }
+                }
+                const double yy0 = (double)y0 - sys_yoff;
+#pragma unroll
+                for (int g = 0; g < GP; g++) {
+                    if (g0 + g < G) {
+                        const LineRec& L = lines_s[(g0 + g) * 16 + ln];
+                        const bool ok = L.ky >= 0;
+                        const double t_s = L.t_s, kyn = L.kyn, d = L.d;
+                        const float trf = L.trf, tau = L.tau;
+                        u32x4 bhi, blo;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            float er, ei;
+                            gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
+                            uint32_t h, l;
+                            split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
+                            bhi[j] = h;
+                            blo[j] = l;
+                        }
+#pragma unroll
+                        for (int rb = 0; rb < RB; rb++) acc[rb][g] = mfma3(ahi[rb], alo[rb], bhi, blo, acc[rb][g]);
+                    }
+                }
+            }
+            // acc[rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
+#pragma unroll
+            for (int g = 0; g < GP; g++) {
+                if (g0 + g < G) {
+                    const int lp = (g0 + g) * 16 + ln;
+#pragma unroll
+                    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int c = rb * 8 + 4 * (lq & 1) + r;
+                            if (c < C) {
+                                float* dst = reinterpret_cast<float*>(u.gbuf + ((size_t)x * LyP + lp) * C + c);
+                                dst[lq >> 1] = acc[rb][g][r];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- x-stage, MFMA
+// Block = 4 waves; each wave owns a tile of 16 output columns n = line*C + coil and up to 8
+// M-tiles (8 kx x re/im each) per pass, contracting over x in chunks of 16.
+__global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ units,
+                                                   const DevGroup* __restrict__ groups,
+                                                   const WorkItem* __restrict__ work) {
+    const WorkItem wi = work[blockIdx.x];
+    const DevUnit& u = units[wi.unit];
+    const DevGroup& gr = groups[u.group];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane >> 4, ln = lane & 15;
+    const int C = gr.C, LyP = gr.LyP;
+    const int ncol = LyP * C;
+    const int n0 = (wi.tile * 4 + wave) * 16;
+    if (n0 >= ncol) return;
+    const int n = n0 + ln;
+    const int lp = n / C, c = n % C;
+    const int tb = gr.lines[n0 / C].table;
+    const LineRec Lrow = gr.lines[lp];
+    const int snx = gr.snx, snxP = (snx + 15) & ~15;
+    const int nx = gr.nx, nxP = (nx + 7) & ~7;
+    const int mtiles = nxP >> 3;
+    const uint32_t* __restrict__ thi = gr.t_hi[tb];
+    const uint32_t* __restrict__ tlo = gr.t_lo[tb];
+    const int part = ln >> 3;
+    for (int m0 = 0; m0 < mtiles; m0 += 8) {
+        f32x4 acc[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int ch = 0; ch < snxP; ch += 16) {
+            const int x0 = ch + lq * 4;
+            u32x4 bhi, blo;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                float2 gv = u.gbuf[((size_t)(x0 + j) * LyP + lp) * C + c];
+                uint32_t h, l;
+                split2(gv.x, gv.y, h, l);
+                bhi[j] = h;
+                blo[j] = l;
+            }
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                if (m0 + m < mtiles) {
+                    const int kx = (m0 + m) * 8 + (ln & 7);
+                    const size_t to = (size_t)kx * snxP + x0;
+                    u32x4 h = *reinterpret_cast<const u32x4*>(thi + to);
+                    u32x4 l = *reinterpret_cast<const u32x4*>(tlo + to);
+                    acc[m] = mfma3(rows_of(h, part), rows_of(l, part), bhi, blo, acc[m]);
+                }
+            }
+        }
+        if (Lrow.ky >= 0) {
+            float* orow = reinterpret_cast<float*>(u.out + ((size_t)c * gr.ny + Lrow.ky) * nx);
+#pragma unroll
+            for (int m = 0; m < 8; m++) {
+                if (m0 + m < mtiles) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int kx = (m0 + m) * 8 + 4 * (lq & 1) + r;
+                        if (kx < nx) orow[2 * kx + (lq >> 1)] = acc[m][r];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- plain FP32
+// y-stage: one thread per (x, padded line) of one unit; FP32 complex MACs over y for all coils.
+template <int CM>
+__global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ units,
+                                                    const DevSlice* __restrict__ slices,
+                                                    const DevGroup* __restrict__ groups, int unit) {
+    const DevUnit& u = units[unit];
+    const DevGroup& gr = groups[u.group];
+    const DevSlice& sl = slices[u.slice];
+    const int LyP = gr.LyP, snx = gr.snx, sny = gr.sny;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lp = t % LyP, x = t / LyP;
+    if (x >= snx) return;
+    const LineRec L = gr.lines[lp];
+    const int C = gr.C, Q = gr.Q;
+    const int snyP = (sny + 15) & ~15, snxP = (snx + 15) & ~15;
+    const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
+    const bool cplx = gr.cplx != 0;
+    float2 acc[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) acc[c] = make_float2(0.f, 0.f);
+    if (L.ky >= 0) {
+        for (int y = 0; y < sny; y++) {
+            const size_t o = (size_t)x * snyP + y;
+            const size_t i = (size_t)x + (size_t)snx * y;
+            double cyc = __builtin_fma(L.kyn, (double)y - gr.sys_yoff, sl.base[o]);
+            cyc = __builtin_fma((double)sl.rate[o], L.t_s, cyc);
+            cyc = __builtin_fma(sl.poly[o], L.d, cyc);
+            float sn, cs;
+            sincos_turns(cyc, sn, cs);
+            float wr = 0.f, wi = 0.f;
+            for (int q = 0; q < Q; q++) {
+                float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, sl.nr2[q * PP + o], L.tau * sl.nri[q * PP + o]));
+                if (cplx) {
+                    float2 m = reinterpret_cast<const float2*>(u.images)[q * P + i];
+                    wr = __builtin_fmaf(m.x, dq, wr);
+                    wi = __builtin_fmaf(m.y, dq, wi);
+                } else {
+                    wr = __builtin_fmaf(reinterpret_cast<const float*>(u.images)[q * P + i], dq, wr);
+                }
+            }
+            const float er = wr * cs - wi * sn, ei = wr * sn + wi * cs;
+#pragma unroll
+            for (int c = 0; c < CM; c++) {
+                if (c < C) {
+                    const float2 s = sl.s32[((size_t)c * snyP + y) * snxP + x];
+                    acc[c].x = __builtin_fmaf(s.x, er, __builtin_fmaf(-s.y, ei, acc[c].x));
+                    acc[c].y = __builtin_fmaf(s.x, ei, __builtin_fmaf(s.y, er, acc[c].y));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CM; c++)
+        if (c < C) u.gbuf[((size_t)x * LyP + lp) * C + c] = acc[c];
+}
+
+// x-stage: one thread per (column n = line*C + coil, kx).
+__global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ units,
+                                                    const DevGroup* __restrict__ groups, int unit) {
+    const DevUnit& u = units[unit];
+    const DevGroup& gr = groups[u.group];
+    const int nx = gr.nx, C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int kx = t % nx, n = t / nx;
+    if (n >= LyP * C) return;
+    const int lp = n / C, c = n % C;
+    const LineRec L = gr.lines[lp];
+    if (L.ky < 0) return;
+    const float2* __restrict__ tw = gr.t32[L.table] + (size_t)kx * snxP;
+    float ar = 0.f, ai = 0.f;
+    for (int x = 0; x < snx; x++) {
+        const float2 g = u.gbuf[((size_t)x * LyP + lp) * C + c];
+        const float2 w = tw[x];
+        ar = __builtin_fmaf(g.x, w.x, __builtin_fmaf(-g.y, w.y, ar));
+        ai = __builtin_fmaf(g.x, w.y, __builtin_fmaf(g.y, w.x, ai));
+    }
+    u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = make_float2(ar, ai);
+}
+
+// ----------------------------------------------------------------------------- epilogues
+// Unacquired rows are exactly zero (src/kspace.rs:839-843).  grid.y = unit.
+__global__ void zero_rows(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups, int unit0) {
+    const DevUnit& u = units[unit0 + blockIdx.y];
+    const DevGroup& gr = groups[u.group];
+    const int nx = gr.nx;
+    const int total = gr.C * gr.n_zero_rows * nx;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int kx = t % nx, r = (t / nx) % gr.n_zero_rows, c = t / (nx * gr.n_zero_rows);
+        u.out[((size_t)c * gr.ny + gr.zero_rows[r]) * nx + kx] = make_float2(0.f, 0.f);
+    }
+}
+
+// SplitMix64 (src/kspace.rs:60-77); counter-based: draw n of state s is mix(s + n*gamma).
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t s, uint64_t n) {
+    uint64_t z = s + n * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit_of(uint64_t v) { return ((double)(v >> 11) + 0.5) / 9007199254740992.0; }
+
+// Spikes then thermal noise on acquired samples (src/kspace.rs:997-1039).  One block per
+// (unit, coil): blockIdx.x = coil, blockIdx.y = unit.  Acquired samples in index order are the
+// acquired rows (ascending ky) times nx.
+__global__ __launch_bounds__(256) void spikes_noise(const DevUnit* __restrict__ units,
+                                                    const DevGroup* __restrict__ groups, int unit0,
+                                                    const int32_t* __restrict__ rows_by_group) {
+    const DevUnit& u = units[unit0 + blockIdx.y];
+    const DevGroup& gr = groups[u.group];
+    const int c = blockIdx.x;
+    if (c >= gr.C) return;
+    const int nx = gr.nx, Ly = gr.Ly;
+    const int32_t* rows = rows_by_group + u.group * 256;  // ascending acquired ky
+    float2* K = u.out + (size_t)c * gr.ny * nx;
+    const int nacq = Ly * nx;
+    const uint64_t coil = u.coil0 + c;
+    if (gr.n_spikes > 0) {
+        __shared__ float best_m[256];
+        __shared__ int best_i[256];
+        float bm = -1.0f;
+        int bi = 0x7fffffff;
+        for (int t = threadIdx.x; t < gr.ny * nx; t += blockDim.x) {
+            float2 v = K[t];
+            float m = sqrtf(v.x * v.x + v.y * v.y);
+            if (m > bm) { bm = m; bi = t; }
+        }
+        best_m[threadIdx.x] = bm;
+        best_i[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = -1.0f;
+            int idx = 0;
+            for (int t = 0; t < (int)blockDim.x; t++)
+                if (best_m[t] > m || (best_m[t] == m && best_i[t] < idx)) { m = best_m[t]; idx = best_i[t]; }
+            float2 peak = m > 0.0f ? K[idx] : make_float2(0.f, 0.f);
+            float2 spike = make_float2((float)((double)peak.x * gr.spike_amplitude),
+                                       (float)((double)peak.y * gr.spike_amplitude));
+            if (nacq > 0) {
+                uint64_t s = u.seed ^ 0xA5A5123456789ABCULL;
+                for (int k = 0; k < gr.n_spikes; k++) {
+                    uint64_t v = splitmix_at(s, (uint64_t)k + 1);
+                    int j = (int)(v % (uint64_t)nacq);
+                    K[(size_t)rows[j / nx] * nx + (j % nx)] = spike;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (gr.noise) {
+        const uint64_t s0 = (u.seed ^ (coil * 0x9E3779B9ULL)) | 1ULL;
+        for (int j = threadIdx.x; j < nacq; j += blockDim.x) {
+            const uint64_t b = 4ull * (uint64_t)j;
+            double u1 = unit_of(splitmix_at(s0, b + 1)), u2 = unit_of(splitmix_at(s0, b + 2));
+            double u3 = unit_of(splitmix_at(s0, b + 3)), u4 = unit_of(splitmix_at(s0, b + 4));
+            double gre = sqrt(-2.0 * log(u1)) * cos(kTau * u2);
+            double gim = sqrt(-2.0 * log(u3)) * cos(kTau * u4);
+            float2& k = K[(size_t)rows[j / nx] * nx + (j % nx)];
+            k.x = (float)((double)k.x + gre * gr.noise_sigma);
+            k.y = (float)((double)k.y + gim * gr.noise_sigma);
+        }
+    }
+}
+
+// Accuracy probe of the phase evaluation (used by the GPU tests): sin/cos of 2*pi*cyc.
+__global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__ s, float* __restrict__ c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sincos_turns(cyc[i], s[i], c[i]);
+}
+
+}  // namespace mrs
+
+// ----------------------------------------------------------------------------- launch wrappers
+namespace mrs {
+
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, hipStream_t st) {
+    dim3 grid((maxvox + 255) / 256, nslices);
+    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, nslices);
+    return hipGetLastError();
+}
+
+template <int Q, int RB, bool CPLX, bool EDDY>
+static hipError_t ylaunch_gp(int gp, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                             const DevGroup* g, const WorkItem* w, int xt) {
+    switch (gp) {
+    case 1: hipLaunchKernelGGL((ystage_mfma<Q, RB, 1, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
+    case 2: hipLaunchKernelGGL((ystage_mfma<Q, RB, 2, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
+    case 4: hipLaunchKernelGGL((ystage_mfma<Q, RB, 4, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <int Q, bool CPLX, bool EDDY>
+static hipError_t ylaunch_rb(int rb, int gp, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                             const DevGroup* g, const WorkItem* w, int xt) {
+    switch (rb) {
+    case 1: return ylaunch_gp<Q, 1, CPLX, EDDY>(gp, grid, st, u, s, g, w, xt);
+    case 2: return ylaunch_gp<Q, 2, CPLX, EDDY>(gp > 2 ? 2 : gp, grid, st, u, s, g, w, xt);
+    case 4: return ylaunch_gp<Q, 4, CPLX, EDDY>(1, grid, st, u, s, g, w, xt);
+    case 8: return ylaunch_gp<Q, 8, CPLX, EDDY>(1, grid, st, u, s, g, w, xt);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool CPLX, bool EDDY>
+static hipError_t ylaunch_q(int Q, int RB, int GP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                            const DevGroup* g, const WorkItem* w, int xt) {
+    switch (Q) {
+    case 1: return ylaunch_rb<1, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
+    case 2: return ylaunch_rb<2, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
+    case 3: return ylaunch_rb<3, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
+    case 4: return ylaunch_rb<4, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
+                              const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    dim3 grid(nwork);
+    if (cplx) return eddy ? ylaunch_q<true, true>(Q, RB, GP, grid, st, u, s, g, w, xt)
+                          : ylaunch_q<true, false>(Q, RB, GP, grid, st, u, s, g, w, xt);
+    return eddy ? ylaunch_q<false, true>(Q, RB, GP, grid, st, u, s, g, w, xt)
+                : ylaunch_q<false, false>(Q, RB, GP, grid, st, u, s, g, w, xt);
+}
+
+hipError_t launch_xstage_mfma(int nwork, const DevUnit* u, const DevGroup* g, const WorkItem* w, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(xstage_mfma, dim3(nwork), dim3(256), 0, st, u, g, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
+                             const DevGroup* g, hipStream_t st) {
+    const int ny_threads = LyP * snx;
+    dim3 gy((ny_threads + 255) / 256);
+    if (C <= 8) hipLaunchKernelGGL((ystage_plain<8>), gy, dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 16) hipLaunchKernelGGL((ystage_plain<16>), gy, dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 32) hipLaunchKernelGGL((ystage_plain<32>), gy, dim3(256), 0, st, u, s, g, unit);
+    else hipLaunchKernelGGL((ystage_plain<64>), gy, dim3(256), 0, st, u, s, g, unit);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nxt = LyP * C * nx;
+    hipLaunchKernelGGL(xstage_plain, dim3((nxt + 255) / 256), dim3(256), 0, st, u, g, unit);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st) {
+    if (nunits <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_rows, dim3(64, nunits), dim3(256), 0, st, u, g, unit0);
+    return hipGetLastError();
+}
+
+hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u, const DevGroup* g,
+                               const int32_t* rows, hipStream_t st) {
+    if (nunits <= 0) return hipSuccess;
+    hipLaunchKernelGGL(spikes_noise, dim3(maxC, nunits), dim3(256), 0, st, u, g, unit0, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, hipStream_t st) {
+    hipLaunchKernelGGL(probe_sincos, dim3((n + 255) / 256), dim3(256), 0, st, cyc, s, c, n);
+    return hipGetLastError();
+}
+
+}  // namespace mrs

@@ -1,0 +1,1023 @@
+// plan.cpp -- host side of the C ABI (include/mrsim_gpu.h): protocol tables, validation, device
+// plans for batched forward/adjoint launches, and the drop-in per-slice calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "internal.h"
+
+namespace mrs {
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, hipStream_t st);
+hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
+                              const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st);
+hipError_t launch_xstage_mfma(int nwork, const DevUnit* u, const DevGroup* g, const WorkItem* w, hipStream_t st);
+hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
+                             const DevGroup* g, hipStream_t st);
+hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st);
+hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u, const DevGroup* g,
+                               const int32_t* rows, hipStream_t st);
+hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, hipStream_t st);
+}  // namespace mrs
+
+using namespace mrs;
+
+static constexpr double kTau = 6.283185307179586476925286766559;
+
+namespace {
+
+struct Status {
+    int code = MRS_OK;
+    std::string msg;
+};
+
+int fail(char* err, size_t errlen, int code, const char* fmt, ...) {
+    if (err && errlen) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(err, errlen, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------ protocol tables (host, FP64)
+
+// SingleShotEpi (src/readout.rs:21-69): tick -> (kx, ky) snake trajectory and its times.
+struct Epi {
+    size_t nx, ny;
+    double t_line, t_echo;
+    bool reverse;
+    double dt() const { return t_line / (double)nx; }
+    double half() const { return (double)(nx * ny) * dt() / 2.0; }
+    void index(size_t tick, size_t& kx, size_t& ky) const {
+        size_t x = tick % nx, y = tick / nx;
+        if (!reverse) {
+            y = ny - 1 - y;
+            if (y % 2 == 1) x = nx - x - 1;
+        } else if (y % 2 == 1) {
+            x = nx - x - 1;
+        }
+        kx = x;
+        ky = y;
+    }
+    double from_echo(size_t tick) const { return dt() * ((double)tick + 0.5) - half(); }
+    double from_prep(size_t tick) const { return (double)tick * dt() + dt() / 2.0; }
+    double from_rf(size_t tick) const { return t_echo + from_echo(tick); }
+};
+
+size_t pf_skipped_lines(size_t ny, const mrs_acquisition& a) {
+    if (a.partial_fourier < 1.0 && a.pf_mode == MRS_PF_SCANNER) {
+        size_t keep = std::min((size_t)std::round((double)ny * a.partial_fourier), ny);
+        return ny - keep;
+    }
+    return 0;
+}
+
+void timing_table(const mrs_acquisition& a, size_t nx, size_t ny, double* t, double* trf, double* tread, int8_t* pol) {
+    Epi e{nx, ny, a.t_line, a.t_echo, a.reverse_phase != 0};
+    std::fill(t, t + ny, 0.0);
+    std::fill(trf, trf + ny, 0.0);
+    std::fill(tread, tread + ny, 0.0);
+    const size_t centre = nx / 2;
+    for (size_t tick = 0; tick < nx * ny; tick++) {
+        size_t kx, ky;
+        e.index(tick, kx, ky);
+        if (kx != centre) continue;
+        t[ky] = e.from_echo(tick);
+        trf[ky] = e.from_rf(tick);
+        tread[ky] = e.from_prep(tick);
+    }
+    for (size_t k = 0; k < ny; k++) pol[k] = (k % 2 == 1) ? -1 : 1;
+    const double offset = (double)pf_skipped_lines(ny, a) * a.t_line;
+    if (offset > 0.0)
+        for (size_t k = 0; k < ny; k++) tread[k] -= offset;
+}
+
+// Acquired PE rows (sampling_mask rows, src/kspace.rs:546-588).
+std::vector<uint8_t> acquired_rows(size_t ny, const mrs_acquisition& a) {
+    std::vector<uint8_t> rows(ny, 0);
+    const size_t ys = ny / 2;
+    const size_t accel = std::max<uint64_t>(a.accel, 1);
+    const size_t keep_n = (size_t)std::round((double)ny * a.partial_fourier);
+    const bool pf = a.partial_fourier < 1.0;
+    for (size_t k = 0; k < ny; k++) {
+        bool skip = false;
+        if (pf && a.pf_mode == MRS_PF_SCANNER) {
+            skip = a.reverse_phase ? (k < ny - keep_n) : (k >= keep_n);
+        } else if (pf && a.pf_mode == MRS_PF_CONTIGUOUS) {
+            skip = a.reverse_phase ? (k >= keep_n) : (k < ny - keep_n);
+        } else if (pf) {
+            skip = a.reverse_phase ? ((double)k > std::ceil((double)ny * a.partial_fourier))
+                                   : (((double)k < std::floor((double)ny * (1.0 - a.partial_fourier))) &&
+                                      (k > 0 || ny % 2 == 1));
+        }
+        if (skip) continue;
+        if (accel <= 1 || k % accel == ys % accel) rows[k] = 1;
+    }
+    return rows;
+}
+
+double ring_sensitivity(size_t coil, size_t n_coils, double x, double y, size_t nx, size_t ny) {
+    if (n_coils <= 1) return 1.0;
+    const double cx = (double)nx / 2.0, cy = (double)ny / 2.0;
+    const double m = (double)std::max(nx, ny);
+    const double r = 0.6 * m;
+    const double ang = kTau * (double)coil / (double)n_coils;
+    const double px = cx + r * std::cos(ang), py = cy + r * std::sin(ang);
+    const double dx = x - px, dy = y - py;
+    const double sigma = 0.9 * m;
+    return std::exp(-(dx * dx + dy * dy) / (2.0 * sigma * sigma)) + 0.15;
+}
+
+uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+float bf16_to_float(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+// FP64 complex -> packed bf16 hi and residual lo, via FP32 (the device's operand precision).
+void split_c(double re, double im, uint32_t& hi, uint32_t& lo) {
+    float fr = (float)re, fi = (float)im;
+    uint16_t hr = bf16_rne(fr), hm = bf16_rne(fi);
+    float rr = (float)(re - (double)bf16_to_float(hr)), ri = (float)(im - (double)bf16_to_float(hm));
+    hi = (uint32_t)hr | ((uint32_t)hm << 16);
+    lo = (uint32_t)bf16_rne(rr) | ((uint32_t)bf16_rne(ri) << 16);
+}
+
+template <class T>
+int dev_alloc(T** p, size_t n, char* err, size_t errlen) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(err, errlen, MRS_E_NOMEM, "hipMalloc(%zu bytes): %s", n * sizeof(T),
+                                     hipGetErrorString(e));
+    return MRS_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ plan
+
+struct GroupHost {
+    mrs_group_desc desc;
+    DevGroup dev;
+    std::vector<LineRec> lines;
+    std::vector<int32_t> zero_rows;
+    std::vector<int32_t> acq_rows;
+    LineRec* d_lines = nullptr;
+    uint32_t* d_thi[2] = {nullptr, nullptr};
+    uint32_t* d_tlo[2] = {nullptr, nullptr};
+    float2* d_t32[2] = {nullptr, nullptr};
+    int32_t* d_zero = nullptr;
+};
+
+struct mrs_plan {
+    int precision = MRS_PREC_SPLIT_BF16;
+    std::vector<GroupHost> groups;
+    std::vector<DevSlice> slices;
+    std::vector<DevUnit> units;
+    std::vector<mrs_slice_desc> slice_desc;
+    std::vector<mrs_unit_desc> unit_desc;
+    DevGroup* d_groups = nullptr;
+    DevSlice* d_slices = nullptr;
+    DevUnit* d_units = nullptr;
+    int32_t* d_rows = nullptr;   // 256 acquired rows per group
+    std::vector<void*> allocations;
+    // y-stage launch classes keyed by (Q, RB, complex images, eddy)
+    struct YClass {
+        int Q, RB, cplx, eddy;
+        std::vector<WorkItem> items;
+        WorkItem* d_items = nullptr;
+    };
+    std::vector<YClass> yclasses;
+    std::vector<WorkItem> xitems;
+    WorkItem* d_xitems = nullptr;
+    int xt = 8, gp = 4;
+    int maxvox = 0;
+    int maxC = 1;
+    bool any_zero_rows = false, any_epilogue = false;
+    double terms = 0.0, nlines = 0.0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    // adjoint scratch
+    std::vector<VjpUnit> vjp_units;
+    VjpUnit* d_vjp_units = nullptr;
+};
+
+static void free_plan_memory(mrs_plan* p) {
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    for (void* a : p->allocations) hipFree(a);
+    p->allocations.clear();
+}
+
+template <class T>
+static int plan_alloc(mrs_plan* p, T** ptr, size_t n, char* err, size_t errlen) {
+    int rc = dev_alloc(ptr, n, err, errlen);
+    if (rc == MRS_OK) p->allocations.push_back((void*)*ptr);
+    return rc;
+}
+
+// Reorder work so that the hardware's round-robin dispatch of consecutive blocks over the 8 XCDs
+// gives each XCD a contiguous run of items (units of one slice share that XCD's L2).
+static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w) {
+    const size_t n = w.size(), q = n / 8, r = n % 8;
+    std::vector<WorkItem> out(n);
+    for (size_t x = 0; x < 8; x++) {
+        const size_t cnt = q + (x < r ? 1 : 0), start = x * q + std::min(x, r);
+        for (size_t k = 0; k < cnt; k++) out[8 * k + x] = w[start + k];
+    }
+    return out;
+}
+
+static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
+    const mrs_group_desc& d = g.desc;
+    const mrs_acquisition& a = d.acq;
+    const size_t snx = d.snx, sny = d.sny, nx = d.nx, ny = d.ny;
+    if (nx == 0 || ny == 0 || snx == 0 || sny == 0 || snx % nx != 0 || sny % ny != 0)
+        return fail(err, errlen, MRS_E_INVALID, "sim grid must be an integer multiple of the acquired matrix");
+    if (d.n_compartments < 1 || d.n_compartments > MRS_MAX_COMP)
+        return fail(err, errlen, MRS_E_UNSUPPORTED, "n_compartments must be 1..%d", MRS_MAX_COMP);
+    if (d.n_coils < 1 || d.n_coils > MRS_MAX_COILS)
+        return fail(err, errlen, MRS_E_UNSUPPORTED, "n_coils must be 1..%d", MRS_MAX_COILS);
+    std::vector<double> t(ny), trf(ny), tread(ny);
+    std::vector<int8_t> pol(ny);
+    if (d.timing) {
+        if (d.timing->ny != ny || !d.timing->t_ms || !d.timing->trf_ms || !d.timing->tread_ms || !d.timing->polarity)
+            return fail(err, errlen, MRS_E_INVALID, "line-timing table must have one entry per phase-encode line");
+        std::copy(d.timing->t_ms, d.timing->t_ms + ny, t.begin());
+        std::copy(d.timing->trf_ms, d.timing->trf_ms + ny, trf.begin());
+        std::copy(d.timing->tread_ms, d.timing->tread_ms + ny, tread.begin());
+        std::copy(d.timing->polarity, d.timing->polarity + ny, pol.begin());
+    } else {
+        timing_table(a, nx, ny, t.data(), trf.data(), tread.data(), pol.data());
+    }
+    std::vector<uint8_t> rows = acquired_rows(ny, a);
+    const size_t ys = ny / 2;
+    const size_t ox = snx / nx, oy = sny / ny;
+    const double xoff = ((double)ox - 1.0) / 2.0, yoff = ((double)oy - 1.0) / 2.0;
+    const bool eddy_possible = a.eddy_strength != 0.0;
+    std::vector<LineRec> seg[2];
+    g.zero_rows.clear();
+    g.acq_rows.clear();
+    for (size_t k = 0; k < ny; k++) {
+        if (!rows[k]) {
+            g.zero_rows.push_back((int32_t)k);
+            continue;
+        }
+        g.acq_rows.push_back((int32_t)k);
+        LineRec L{};
+        L.t_s = t[k] / 1000.0;
+        L.kyn = ((double)k - (double)ys) / (double)sny;
+        L.d = eddy_possible ? std::exp(-tread[k] / a.eddy_tau) * L.t_s : 0.0;
+        L.trf = (float)trf[k];
+        L.tau = a.echo == MRS_ECHO_SPIN ? (float)(std::fabs(L.t_s) * 1000.0) : (float)trf[k];
+        L.tb0 = a.echo == MRS_ECHO_SPIN ? (float)L.t_s : (float)(L.t_s + a.t_echo / 1000.0);
+        L.ky = (int32_t)k;
+        // src/kspace.rs:987 and :1109: table 1 iff the line's ghost shift is negative
+        const double shift = pol[k] < 0 ? -a.ghost_offset : a.ghost_offset;
+        L.table = (a.ghost_offset != 0.0 && shift < 0.0) ? 1 : 0;
+        seg[L.table].push_back(L);
+    }
+    g.lines.clear();
+    for (int s = 0; s < 2; s++) {
+        for (auto& L : seg[s]) g.lines.push_back(L);
+        while (g.lines.size() % 16) {
+            LineRec pad{};
+            pad.ky = -1;
+            pad.table = s;
+            g.lines.push_back(pad);
+        }
+    }
+    const int Ly = (int)(seg[0].size() + seg[1].size());
+    const int LyP = (int)g.lines.size();
+    if (LyP > 256) return fail(err, errlen, MRS_E_UNSUPPORTED, "at most 256 padded acquired lines per slice");
+    if (g.acq_rows.size() > 256) return fail(err, errlen, MRS_E_UNSUPPORTED, "at most 256 acquired lines");
+    if (LyP == 0) {  // keep one padded tile so the kernels' shapes stay valid
+        LineRec pad{};
+        pad.ky = -1;
+        for (int i = 0; i < 16; i++) g.lines.push_back(pad);
+    }
+    // twiddles (src/kspace.rs:1091-1101), divided by snx*sny (:990-994)
+    const size_t xs = nx / 2, sxs = ox * xs;
+    const double cctr = (double)sxs + xoff;
+    const size_t snxP = (snx + 15) & ~size_t(15), nxP = (nx + 7) & ~size_t(7);
+    const double ninv = 1.0 / (double)(snx * sny);
+    for (int tb = 0; tb < 2; tb++) {
+        const double gh = tb == 0 ? a.ghost_offset : -a.ghost_offset;
+        std::vector<uint32_t> hi(nxP * snxP, 0), lo(nxP * snxP, 0);
+        std::vector<float2> t32(nxP * snxP, make_float2(0.f, 0.f));
+        for (size_t kxi = 0; kxi < nx; kxi++) {
+            const double kxn = ((double)kxi - (double)xs + gh) / (double)snx;
+            for (size_t x = 0; x < snx; x++) {
+                const double ph = kTau * kxn * ((double)x - cctr);
+                const double re = std::cos(ph) * ninv, im = std::sin(ph) * ninv;
+                split_c(re, im, hi[kxi * snxP + x], lo[kxi * snxP + x]);
+                t32[kxi * snxP + x] = make_float2((float)re, (float)im);
+            }
+        }
+        int rc;
+        if ((rc = plan_alloc(p, &g.d_thi[tb], hi.size(), err, errlen))) return rc;
+        if ((rc = plan_alloc(p, &g.d_tlo[tb], lo.size(), err, errlen))) return rc;
+        if ((rc = plan_alloc(p, &g.d_t32[tb], t32.size(), err, errlen))) return rc;
+        HIP_TRY(hipMemcpy(g.d_thi[tb], hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.d_tlo[tb], lo.data(), lo.size() * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(g.d_t32[tb], t32.data(), t32.size() * 8, hipMemcpyHostToDevice));
+    }
+    int rc;
+    if ((rc = plan_alloc(p, &g.d_lines, g.lines.size(), err, errlen))) return rc;
+    HIP_TRY(hipMemcpy(g.d_lines, g.lines.data(), g.lines.size() * sizeof(LineRec), hipMemcpyHostToDevice));
+    if ((rc = plan_alloc(p, &g.d_zero, std::max<size_t>(g.zero_rows.size(), 1), err, errlen))) return rc;
+    if (!g.zero_rows.empty())
+        HIP_TRY(hipMemcpy(g.d_zero, g.zero_rows.data(), g.zero_rows.size() * 4, hipMemcpyHostToDevice));
+
+    DevGroup& dg = g.dev;
+    std::memset(&dg, 0, sizeof dg);
+    dg.snx = (int)snx; dg.sny = (int)sny; dg.nx = (int)nx; dg.ny = (int)ny;
+    dg.C = (int)d.n_coils; dg.Q = (int)d.n_compartments;
+    dg.RB = (int)((2 * d.n_coils + 15) / 16);
+    if (dg.RB == 3) dg.RB = 4;
+    if (dg.RB > 4) dg.RB = 8;
+    dg.Ly = Ly;
+    dg.LyP = (int)g.lines.size();
+    dg.cplx = d.complex_images ? 1 : 0;
+    dg.echo = a.echo; dg.do_relax = a.do_relaxation ? 1 : 0; dg.do_dist = a.do_distortions ? 1 : 0;
+    dg.sys_yoff = (double)(oy * ys) + yoff;
+    dg.sxs_xoff = (double)sxs + xoff;
+    dg.ox = (int)ox; dg.oy = (int)oy;
+    dg.t_echo = a.t_echo; dg.t_inhom = a.t_inhom; dg.signal_scale = a.signal_scale;
+    dg.eddy_strength = a.eddy_strength; dg.eddy_quad = a.eddy_quad; dg.eddy_phase = a.eddy_phase;
+    dg.lines = g.d_lines;
+    for (int tb = 0; tb < 2; tb++) { dg.t_hi[tb] = g.d_thi[tb]; dg.t_lo[tb] = g.d_tlo[tb]; dg.t32[tb] = g.d_t32[tb]; }
+    dg.zero_rows = g.d_zero;
+    dg.n_zero_rows = (int)g.zero_rows.size();
+    dg.noise = a.noise_variance > 0.0 ? 1 : 0;
+    dg.n_spikes = (int)a.n_spikes;
+    dg.noise_sigma = std::sqrt(a.noise_variance / (double)(nx * ny));
+    dg.spike_amplitude = a.spike_amplitude;
+    return MRS_OK;
+}
+
+static void fill_slice_inputs(DevSlice& s, const mrs_slice_desc& d) {
+    s.fmap = d.fmap;
+    s.phase0 = d.phase0;
+    s.t2 = d.t2;
+    s.t_inhom = d.t_inhom;
+    s.coil_maps = d.coil_maps;
+    s.log_amp = d.log_amp;
+    for (int q = 0; q < MRS_MAX_COMP; q++) { s.t2_uniform[q] = d.t2_uniform[q]; s.ti_uniform[q] = d.t_inhom_uniform[q]; }
+    s.has_t_inhom = d.has_t_inhom;
+    s.group = d.group;
+    s.z = (int32_t)d.z;
+    s.nz = (int32_t)d.nz;
+    s.has_eddy = d.has_eddy_drive;
+    for (int i = 0; i < 3; i++) s.eddy[i] = d.eddy_drive[i];
+}
+
+static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
+    HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
+    return MRS_OK;
+}
+
+static int check_pointers(mrs_plan* p, const mrs_slice_desc* slices, const mrs_unit_desc* units, char* err,
+                          size_t errlen) {
+    for (size_t s = 0; s < p->slices.size(); s++) {
+        const mrs_slice_desc& d = slices[s];
+        if (d.group < 0 || (size_t)d.group >= p->groups.size())
+            return fail(err, errlen, MRS_E_INVALID, "slice %zu names group %d of %zu", s, d.group, p->groups.size());
+        if (!d.fmap || !d.coil_maps)
+            return fail(err, errlen, MRS_E_INVALID, "slice %zu: fmap and coil_maps are required", s);
+    }
+    for (size_t u = 0; u < p->units.size(); u++) {
+        const mrs_unit_desc& d = units[u];
+        if (d.slice < 0 || (size_t)d.slice >= p->slices.size())
+            return fail(err, errlen, MRS_E_INVALID, "unit %zu names slice %d of %zu", u, d.slice, p->slices.size());
+        if (!d.images || !d.out) return fail(err, errlen, MRS_E_INVALID, "unit %zu: images and out are required", u);
+    }
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_set_pointers(mrs_plan* p, const mrs_slice_desc* slices, const mrs_unit_desc* units,
+                                     char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    int rc = check_pointers(p, slices, units, err, errlen);
+    if (rc) return rc;
+    for (size_t s = 0; s < p->slices.size(); s++) {
+        if (slices[s].group != p->slices[s].group)
+            return fail(err, errlen, MRS_E_INVALID, "slice %zu changed group", s);
+        fill_slice_inputs(p->slices[s], slices[s]);
+        p->slice_desc[s] = slices[s];
+    }
+    for (size_t u = 0; u < p->units.size(); u++) {
+        if (units[u].slice != p->units[u].slice) return fail(err, errlen, MRS_E_INVALID, "unit %zu changed slice", u);
+        p->units[u].images = units[u].images;
+        p->units[u].out = reinterpret_cast<float2*>(units[u].out);
+        p->units[u].seed = units[u].slice_seed;
+        p->units[u].coil0 = units[u].coil0;
+        p->unit_desc[u] = units[u];
+    }
+    if (p->graph_exec) {
+        hipGraphExecDestroy(p->graph_exec);
+        hipGraphDestroy(p->graph);
+        p->graph_exec = nullptr;
+        p->graph = nullptr;
+    }
+    return upload_descs(p, err, errlen);
+}
+
+extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, const mrs_slice_desc* slices,
+                               size_t nslices, const mrs_unit_desc* units, size_t nunits, int32_t precision,
+                               mrs_plan** out, char* err, size_t errlen) {
+    if (!out) return fail(err, errlen, MRS_E_INVALID, "null output plan");
+    *out = nullptr;
+    if (!groups || !slices || !units || ngroups == 0 || nslices == 0 || nunits == 0)
+        return fail(err, errlen, MRS_E_INVALID, "a plan needs at least one group, slice and unit");
+    if (precision != MRS_PREC_SPLIT_BF16 && precision != MRS_PREC_FP32)
+        return fail(err, errlen, MRS_E_INVALID, "unknown precision %d", precision);
+    mrs_plan* p = new mrs_plan();
+    p->precision = precision;
+    p->groups.resize(ngroups);
+    int rc = MRS_OK;
+    for (size_t g = 0; g < ngroups && rc == MRS_OK; g++) {
+        p->groups[g].desc = groups[g];
+        p->groups[g].desc.timing = nullptr;
+        GroupHost& gh = p->groups[g];
+        gh.desc.timing = groups[g].timing;
+        rc = build_group(p, gh, err, errlen);
+        gh.desc.timing = nullptr;
+    }
+    if (rc) { free_plan_memory(p); delete p; return rc; }
+    p->slices.resize(nslices);
+    p->units.resize(nunits);
+    p->slice_desc.assign(slices, slices + nslices);
+    p->unit_desc.assign(units, units + nunits);
+    for (size_t s = 0; s < nslices; s++) p->slices[s].group = slices[s].group;
+    for (size_t u = 0; u < nunits; u++) p->units[u].slice = units[u].slice;
+    rc = check_pointers(p, slices, units, err, errlen);
+    // per-slice planes
+    for (size_t s = 0; s < nslices && rc == MRS_OK; s++) {
+        const DevGroup& dg = p->groups[slices[s].group].dev;
+        DevSlice& sl = p->slices[s];
+        std::memset(&sl, 0, sizeof sl);
+        fill_slice_inputs(sl, slices[s]);
+        const size_t snxP = (dg.snx + 15) & ~15, snyP = (dg.sny + 15) & ~15, PP = snxP * snyP;
+        p->maxvox = std::max(p->maxvox, (int)PP);
+        if ((rc = plan_alloc(p, &sl.base, PP, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.rate, PP, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.poly, PP, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.nr2, PP * dg.Q, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.nri, PP * dg.Q, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.s_hi, PP * dg.C, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.s_lo, PP * dg.C, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.s32, PP * dg.C, err, errlen))) break;
+    }
+    // per-unit scratch and work lists
+    std::map<std::tuple<int, int, int, int>, size_t> cls;
+    for (size_t u = 0; u < nunits && rc == MRS_OK; u++) {
+        DevUnit& du = p->units[u];
+        const int gi = slices[units[u].slice].group;
+        const GroupHost& gh = p->groups[gi];
+        const DevGroup& dg = gh.dev;
+        du.group = gi;
+        du.images = units[u].images;
+        du.out = reinterpret_cast<float2*>(units[u].out);
+        du.seed = units[u].slice_seed;
+        du.coil0 = units[u].coil0;
+        const size_t snxP = (dg.snx + 15) & ~15;
+        const size_t gsz = snxP * dg.LyP * dg.C;
+        if ((rc = plan_alloc(p, &du.gbuf, gsz, err, errlen))) break;
+        if (hipMemset(du.gbuf, 0, gsz * sizeof(float2)) != hipSuccess) {
+            rc = fail(err, errlen, MRS_E_DEVICE, "hipMemset of the y-stage scratch failed");
+            break;
+        }
+        p->maxC = std::max(p->maxC, dg.C);
+        p->any_zero_rows |= dg.n_zero_rows > 0;
+        p->any_epilogue |= dg.noise || dg.n_spikes > 0;
+        p->terms += (double)dg.C * dg.nx * dg.Ly * (double)dg.snx * dg.sny;
+        p->nlines += dg.Ly;
+        auto key = std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0);
+        if (!cls.count(key)) {
+            cls[key] = p->yclasses.size();
+            p->yclasses.push_back({dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0, {}, nullptr});
+        }
+    }
+    if (rc == MRS_OK) {
+        rc = plan_alloc(p, &p->d_groups, ngroups, err, errlen);
+        if (!rc) rc = plan_alloc(p, &p->d_slices, nslices, err, errlen);
+        if (!rc) rc = plan_alloc(p, &p->d_units, nunits, err, errlen);
+        if (!rc) rc = plan_alloc(p, &p->d_rows, 256 * ngroups, err, errlen);
+    }
+    if (rc == MRS_OK) {
+        std::vector<DevGroup> dgs(ngroups);
+        std::vector<int32_t> rows(256 * ngroups, 0);
+        for (size_t g = 0; g < ngroups; g++) {
+            dgs[g] = p->groups[g].dev;
+            std::copy(p->groups[g].acq_rows.begin(), p->groups[g].acq_rows.end(), rows.begin() + 256 * g);
+        }
+        if (hipMemcpy(p->d_groups, dgs.data(), ngroups * sizeof(DevGroup), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->d_rows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(err, errlen, MRS_E_DEVICE, "upload of group tables failed");
+    }
+    if (rc == MRS_OK) rc = upload_descs(p, err, errlen);
+    if (rc == MRS_OK) {
+        // work lists: units sorted by slice so an XCD's run covers the volumes of few slices
+        std::vector<size_t> order(nunits);
+        for (size_t u = 0; u < nunits; u++) order[u] = u;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return units[a].slice < units[b].slice; });
+        std::vector<std::vector<WorkItem>> yw(p->yclasses.size());
+        p->xitems.clear();
+        for (size_t k = 0; k < nunits; k++) {
+            const size_t u = order[k];
+            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            const size_t c = cls[std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0)];
+            const int ntile = (dg.snx + p->xt - 1) / p->xt;
+            for (int t = 0; t < ntile; t++) yw[c].push_back({(int32_t)u, t});
+            const int ncol = dg.LyP * dg.C;
+            const int xtiles = (ncol / 16 + 3) / 4;
+            for (int t = 0; t < xtiles; t++) p->xitems.push_back({(int32_t)u, t});
+        }
+        for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
+            p->yclasses[c].items = xcd_order(yw[c]);
+            rc = plan_alloc(p, &p->yclasses[c].d_items, p->yclasses[c].items.size(), err, errlen);
+            if (!rc && hipMemcpy(p->yclasses[c].d_items, p->yclasses[c].items.data(),
+                                 p->yclasses[c].items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
+        }
+        if (rc == MRS_OK) {
+            p->xitems = xcd_order(p->xitems);
+            rc = plan_alloc(p, &p->d_xitems, p->xitems.size(), err, errlen);
+            if (!rc && hipMemcpy(p->d_xitems, p->xitems.data(), p->xitems.size() * sizeof(WorkItem),
+                                 hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
+        }
+    }
+    if (rc) { free_plan_memory(p); delete p; return rc; }
+    *out = p;
+    return MRS_OK;
+}
+
+// Rebuild the y-stage work lists for a new x tile (autotuner).
+static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
+    p->xt = xt;
+    p->gp = gp;
+    for (auto& c : p->yclasses) {
+        std::vector<WorkItem> items;
+        std::vector<size_t> order(p->units.size());
+        for (size_t u = 0; u < order.size(); u++) order[u] = u;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t a, size_t b) { return p->units[a].slice < p->units[b].slice; });
+        for (size_t u : order) {
+            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            if (dg.Q != c.Q || dg.RB != c.RB || dg.cplx != c.cplx || (dg.eddy_strength != 0.0) != (c.eddy != 0))
+                continue;
+            const int ntile = (dg.snx + xt - 1) / xt;
+            for (int t = 0; t < ntile; t++) items.push_back({(int32_t)u, t});
+        }
+        items = xcd_order(items);
+        if (items.size() > c.items.size()) {
+            int rc = plan_alloc(p, &c.d_items, items.size(), err, errlen);
+            if (rc) return rc;
+        }
+        c.items = items;
+        HIP_TRY(hipMemcpy(c.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+    }
+    if (p->graph_exec) {
+        hipGraphExecDestroy(p->graph_exec);
+        hipGraphDestroy(p->graph);
+        p->graph_exec = nullptr;
+        p->graph = nullptr;
+    }
+    return MRS_OK;
+}
+
+static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen) {
+    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+    if (p->precision == MRS_PREC_SPLIT_BF16) {
+        for (auto& c : p->yclasses) {
+            int gp = p->gp;
+            HIP_TRY(launch_ystage_mfma(c.Q, c.RB, gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
+                                       p->d_units, p->d_slices, p->d_groups, c.d_items, st));
+        }
+        HIP_TRY(launch_xstage_mfma((int)p->xitems.size(), p->d_units, p->d_groups, p->d_xitems, st));
+    } else {
+        for (size_t u = 0; u < p->units.size(); u++) {
+            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            HIP_TRY(launch_plain_unit((int)u, dg.C, dg.LyP, dg.snx, dg.nx, p->d_units, p->d_slices, p->d_groups, st));
+        }
+    }
+    if (p->any_zero_rows) HIP_TRY(launch_zero_rows(0, (int)p->units.size(), p->d_units, p->d_groups, st));
+    if (p->any_epilogue)
+        HIP_TRY(launch_spikes_noise(0, (int)p->units.size(), p->maxC, p->d_units, p->d_groups, p->d_rows, st));
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_forward(mrs_plan* p, void* stream, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    return enqueue_forward(p, (hipStream_t)stream, err, errlen);
+}
+
+extern "C" int mrs_plan_forward_graph(mrs_plan* p, void* stream, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    hipStream_t st = (hipStream_t)stream;
+    if (!p->graph_exec) {
+        hipStream_t cap = st;
+        bool own = false;
+        if (!cap) {  // the null stream cannot be captured: record on a private stream
+            HIP_TRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            own = true;
+        }
+        HIP_TRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_forward(p, cap, err, errlen);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(cap, &g);
+        if (own) hipStreamDestroy(cap);
+        if (rc) { if (g) hipGraphDestroy(g); return rc; }
+        if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        p->graph = g;
+        HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(p->graph_exec, st));
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images,
+                            float* const* grad_fmap, float* const* grad_t_inhom, void* stream, char* err,
+                            size_t errlen) {
+    (void)grad_out; (void)grad_images; (void)grad_fmap; (void)grad_t_inhom; (void)stream;
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    return fail(err, errlen, MRS_E_UNSUPPORTED, "adjoint not built yet");
+}
+
+// Average duration of each forward stage over `iters` launches, timed with HIP events recorded on
+// `stream` around each stage: ms[0] prep, ms[1] y-stage, ms[2] x-stage, ms[3] epilogues.
+extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, double* ms, char* err, size_t errlen) {
+    if (!p || !ms) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    hipStream_t st = (hipStream_t)stream;
+    if (iters < 1) iters = 1;
+    hipEvent_t ev[5];
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    double acc[4] = {0, 0, 0, 0};
+    int rc = MRS_OK;
+    for (int it = 0; it < iters && rc == MRS_OK; it++) {
+        HIP_TRY(hipEventRecord(ev[0], st));
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+        HIP_TRY(hipEventRecord(ev[1], st));
+        if (p->precision == MRS_PREC_SPLIT_BF16) {
+            for (auto& c : p->yclasses)
+                HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
+                                           p->d_units, p->d_slices, p->d_groups, c.d_items, st));
+            HIP_TRY(hipEventRecord(ev[2], st));
+            HIP_TRY(launch_xstage_mfma((int)p->xitems.size(), p->d_units, p->d_groups, p->d_xitems, st));
+        } else {
+            for (size_t u = 0; u < p->units.size(); u++) {
+                const DevGroup& dg = p->groups[p->units[u].group].dev;
+                HIP_TRY(launch_plain_unit((int)u, dg.C, dg.LyP, dg.snx, dg.nx, p->d_units, p->d_slices, p->d_groups,
+                                          st));
+            }
+            HIP_TRY(hipEventRecord(ev[2], st));
+        }
+        HIP_TRY(hipEventRecord(ev[3], st));
+        if (p->any_zero_rows) HIP_TRY(launch_zero_rows(0, (int)p->units.size(), p->d_units, p->d_groups, st));
+        if (p->any_epilogue)
+            HIP_TRY(launch_spikes_noise(0, (int)p->units.size(), p->maxC, p->d_units, p->d_groups, p->d_rows, st));
+        HIP_TRY(hipEventRecord(ev[4], st));
+        HIP_TRY(hipEventSynchronize(ev[4]));
+        for (int k = 0; k < 4; k++) {
+            float t = 0.f;
+            HIP_TRY(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+            acc[k] += t;
+        }
+    }
+    for (auto& e : ev) hipEventDestroy(e);
+    for (int k = 0; k < 4; k++) ms[k] = acc[k] / iters;
+    return rc;
+}
+
+extern "C" int mrs_plan_stats(const mrs_plan* p, double* terms, double* lines, double* units) {
+    if (!p) return MRS_E_INVALID;
+    if (terms) *terms = p->terms;
+    if (lines) *lines = p->nlines;
+    if (units) *units = (double)p->units.size();
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_destroy(mrs_plan* p) {
+    if (!p) return MRS_OK;
+    free_plan_memory(p);
+    delete p;
+    return MRS_OK;
+}
+
+// Autotuner: time each (x tile, line tiles per pass) candidate with HIP events on `stream` and
+// keep the fastest.  The report lists every candidate's average time.
+extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* report, size_t reportlen) {
+    char err[256];
+    const size_t errlen = sizeof err;
+    if (!p) return MRS_E_INVALID;
+    if (p->precision != MRS_PREC_SPLIT_BF16) return MRS_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (trials < 1) trials = 3;
+    int maxRB = 1;
+    for (auto& c : p->yclasses) maxRB = std::max(maxRB, c.RB);
+    const int xts[] = {4, 8, 16};
+    std::vector<int> gps;
+    if (maxRB == 1) gps = {1, 2, 4};
+    else if (maxRB == 2) gps = {1, 2};
+    else gps = {1};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    int bxt = p->xt, bgp = p->gp;
+    std::string rep;
+    for (int xt : xts) {
+        for (int gp : gps) {
+            int rc = set_tiles(p, xt, gp, err, errlen);
+            if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+            rc = enqueue_forward(p, st, err, errlen);  // warm-up
+            if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+            HIP_TRY(hipEventRecord(e0, st));
+            for (int t = 0; t < trials; t++) enqueue_forward(p, st, err, errlen);
+            HIP_TRY(hipEventRecord(e1, st));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            ms /= (float)trials;
+            char line[96];
+            snprintf(line, sizeof line, "xt=%d gp=%d %.4f ms\n", xt, gp, ms);
+            rep += line;
+            if (ms < best) { best = ms; bxt = xt; bgp = gp; }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    int rc = set_tiles(p, bxt, bgp, err, errlen);
+    char line[96];
+    snprintf(line, sizeof line, "best xt=%d gp=%d %.4f ms\n", bxt, bgp, best);
+    rep += line;
+    if (report && reportlen) {
+        std::strncpy(report, rep.c_str(), reportlen - 1);
+        report[reportlen - 1] = 0;
+    }
+    return rc;
+}
+
+extern "C" int mrs_plan_set_tiles(mrs_plan* p, int32_t xt, int32_t gp, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    if (xt < 1 || xt > 64) return fail(err, errlen, MRS_E_INVALID, "x tile must be 1..64");
+    if (gp != 1 && gp != 2 && gp != 4) return fail(err, errlen, MRS_E_INVALID, "lines tiles per pass: 1, 2 or 4");
+    return set_tiles(p, xt, gp, err, errlen);
+}
+
+extern "C" int mrs_plan_get_tiles(const mrs_plan* p, int32_t* xt, int32_t* gp) {
+    if (!p) return MRS_E_INVALID;
+    if (xt) *xt = p->xt;
+    if (gp) *gp = p->gp;
+    return MRS_OK;
+}
+
+// ------------------------------------------------------------------ host tables (C ABI)
+
+extern "C" int mrs_acquisition_default(mrs_acquisition* a) {
+    if (!a) return MRS_E_INVALID;
+    std::memset(a, 0, sizeof *a);
+    a->t_line = 1.0;
+    a->t_echo = 90.0;
+    a->t_inhom = 50.0;
+    a->signal_scale = 100.0;
+    a->do_distortions = 1;
+    a->do_relaxation = 1;
+    a->partial_fourier = 1.0;
+    a->pf_mode = MRS_PF_FIBERFOX;
+    a->eddy_tau = 70.0;
+    a->spike_amplitude = 1.0;
+    a->window = MRS_WINDOW_NONE;
+    a->n_coils = 1;
+    a->accel = 1;
+    a->acs_lines = 24;
+    a->echo = MRS_ECHO_SPIN;
+    return MRS_OK;
+}
+
+extern "C" int mrs_line_timing_for_acquisition(const mrs_acquisition* acq, size_t nx, size_t ny, double* t_ms,
+                                               double* trf_ms, double* tread_ms, int8_t* polarity) {
+    if (!acq || !t_ms || !trf_ms || !tread_ms || !polarity || nx == 0) return MRS_E_INVALID;
+    timing_table(*acq, nx, ny, t_ms, trf_ms, tread_ms, polarity);
+    return MRS_OK;
+}
+
+extern "C" int mrs_sampling_mask(size_t nx, size_t ny, const mrs_acquisition* acq, uint8_t* mask) {
+    if (!acq || !mask) return MRS_E_INVALID;
+    std::vector<uint8_t> rows = acquired_rows(ny, *acq);
+    for (size_t k = 0; k < ny; k++)
+        for (size_t x = 0; x < nx; x++) mask[x + nx * k] = rows[k];
+    return MRS_OK;
+}
+
+extern "C" int mrs_ring_coil_maps(size_t coil0, size_t n, size_t ncoils, size_t snx, size_t sny, size_t nx, size_t ny,
+                                  double scale, mrs_c32* out) {
+    if (!out || nx == 0 || ny == 0 || snx % nx || sny % ny) return MRS_E_INVALID;
+    const size_t ox = snx / nx, oy = sny / ny;
+    const double xoff = ((double)ox - 1.0) / 2.0, yoff = ((double)oy - 1.0) / 2.0;
+    for (size_t c = 0; c < n; c++)
+        for (size_t y = 0; y < sny; y++)
+            for (size_t x = 0; x < snx; x++) {
+                const double xa = ((double)x - xoff) / (double)ox, ya = ((double)y - yoff) / (double)oy;
+                const double s = scale * ring_sensitivity(coil0 + c, ncoils, xa, ya, nx, ny);
+                out[c * snx * sny + x + snx * y] = mrs_c32{(float)s, 0.0f};
+            }
+    return MRS_OK;
+}
+
+extern "C" const char* mrs_version(void) { return "mrsim-acq-mi355x 0.1 (gfx950)"; }
+
+extern "C" size_t mrs_sizeof(const char* name) {
+    if (!name) return 0;
+    std::string n(name);
+    if (n == "mrs_acquisition") return sizeof(mrs_acquisition);
+    if (n == "mrs_t2_slice") return sizeof(mrs_t2_slice);
+    if (n == "mrs_slice_input") return sizeof(mrs_slice_input);
+    if (n == "mrs_line_timing") return sizeof(mrs_line_timing);
+    if (n == "mrs_group_desc") return sizeof(mrs_group_desc);
+    if (n == "mrs_slice_desc") return sizeof(mrs_slice_desc);
+    if (n == "mrs_unit_desc") return sizeof(mrs_unit_desc);
+    return 0;
+}
+
+// Accuracy probe of the device phase evaluation: sin/cos(2 pi cyc[i]) for host arrays.
+extern "C" int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, char* err, size_t errlen) {
+    double* dc = nullptr;
+    float *ds = nullptr, *dcs = nullptr;
+    if (dev_alloc(&dc, n, err, errlen) || dev_alloc(&ds, n, err, errlen) || dev_alloc(&dcs, n, err, errlen)) {
+        hipFree(dc); hipFree(ds); hipFree(dcs);
+        return MRS_E_NOMEM;
+    }
+    int rc = MRS_OK;
+    if (hipMemcpy(dc, cyc, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        launch_probe_sincos(dc, ds, dcs, (int)n, nullptr) != hipSuccess ||
+        hipMemcpy(s, ds, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(c, dcs, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(err, errlen, MRS_E_DEVICE, "sincos probe failed");
+    hipFree(dc); hipFree(ds); hipFree(dcs);
+    return rc;
+}
+
+// ------------------------------------------------------------------ drop-in per-slice forward
+
+namespace {
+
+struct DevBuf {
+    std::vector<void*> ptrs;
+    ~DevBuf() { for (void* p : ptrs) hipFree(p); }
+    template <class T>
+    int up(T** d, const T* h, size_t n, char* err, size_t errlen) {
+        if (dev_alloc(d, n, err, errlen)) return MRS_E_NOMEM;
+        ptrs.push_back((void*)*d);
+        if (h && n) {
+            hipError_t e = hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+        }
+        return MRS_OK;
+    }
+};
+
+int drop_in(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil0, size_t ncoils_out,
+            size_t ncoils_total, const mrs_line_timing* timing, const double* log_amp, mrs_c64* out, char* err,
+            size_t errlen) {
+    if (!inp || !acq || !out) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    const size_t snx = inp->sim[0], sny = inp->sim[1], nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    if (nx == 0 || ny == 0 || snx % nx != 0 || sny % ny != 0)
+        return fail(err, errlen, MRS_E_INVALID, "sim grid must be an integer multiple of the acquired matrix");
+    if (timing && (timing->ny != ny))
+        return fail(err, errlen, MRS_E_INVALID, "line-timing table must have one entry per phase-encode line");
+    const size_t Q = inp->n_compartments;
+    if (Q < 1 || Q > MRS_MAX_COMP) return fail(err, errlen, MRS_E_UNSUPPORTED, "n_compartments must be 1..%d", MRS_MAX_COMP);
+    if (!inp->compartments || !inp->t2 || !inp->fmap) return fail(err, errlen, MRS_E_INVALID, "missing slice inputs");
+    const size_t P = snx * sny;
+    DevBuf b;
+    // images: Q planes
+    std::vector<float> comp(Q * P);
+    for (size_t q = 0; q < Q; q++) std::copy(inp->compartments[q], inp->compartments[q] + P, comp.begin() + q * P);
+    float* d_img = nullptr;
+    if (b.up(&d_img, comp.data(), comp.size(), err, errlen)) return MRS_E_DEVICE;
+    float* d_fmap = nullptr;
+    if (b.up(&d_fmap, inp->fmap, P, err, errlen)) return MRS_E_DEVICE;
+    double* d_p0 = nullptr;
+    if (inp->phase0 && b.up(&d_p0, inp->phase0, P, err, errlen)) return MRS_E_DEVICE;
+    double* d_la = nullptr;
+    if (log_amp && b.up(&d_la, log_amp, P, err, errlen)) return MRS_E_DEVICE;
+    mrs_slice_desc sd;
+    std::memset(&sd, 0, sizeof sd);
+    bool t2map = false, timap = false;
+    for (size_t q = 0; q < Q; q++) {
+        t2map |= inp->t2[q].map != nullptr;
+        if (inp->t_inhom) timap |= inp->t_inhom[q].map != nullptr;
+    }
+    std::vector<float> t2(t2map ? Q * P : 0), ti(timap ? Q * P : 0);
+    for (size_t q = 0; q < Q; q++) {
+        sd.t2_uniform[q] = inp->t2[q].value;
+        if (t2map) {
+            if (inp->t2[q].map) std::copy(inp->t2[q].map, inp->t2[q].map + P, t2.begin() + q * P);
+            else std::fill(t2.begin() + q * P, t2.begin() + (q + 1) * P, inp->t2[q].value);
+        }
+        if (inp->t_inhom) {
+            sd.t_inhom_uniform[q] = inp->t_inhom[q].value;
+            if (timap) {
+                if (inp->t_inhom[q].map) std::copy(inp->t_inhom[q].map, inp->t_inhom[q].map + P, ti.begin() + q * P);
+                else std::fill(ti.begin() + q * P, ti.begin() + (q + 1) * P, inp->t_inhom[q].value);
+            }
+        }
+    }
+    float *d_t2 = nullptr, *d_ti = nullptr;
+    if (t2map && b.up(&d_t2, t2.data(), t2.size(), err, errlen)) return MRS_E_DEVICE;
+    if (timap && b.up(&d_ti, ti.data(), ti.size(), err, errlen)) return MRS_E_DEVICE;
+    std::vector<mrs_c32> maps(ncoils_out * P);
+    mrs_ring_coil_maps(coil0, ncoils_out, ncoils_total, snx, sny, nx, ny, 1.0, maps.data());
+    mrs_c32* d_maps = nullptr;
+    if (b.up(&d_maps, maps.data(), maps.size(), err, errlen)) return MRS_E_DEVICE;
+    mrs_c32* d_out = nullptr;
+    if (b.up(&d_out, (const mrs_c32*)nullptr, ncoils_out * nx * ny, err, errlen)) return MRS_E_DEVICE;
+
+    mrs_group_desc gd;
+    std::memset(&gd, 0, sizeof gd);
+    gd.snx = snx; gd.sny = sny; gd.nx = nx; gd.ny = ny;
+    gd.n_coils = ncoils_out;
+    gd.n_compartments = Q;
+    gd.complex_images = 0;
+    gd.acq = *acq;
+    gd.timing = timing;
+    sd.group = 0;
+    sd.fmap = d_fmap;
+    sd.phase0 = d_p0;
+    sd.t2 = d_t2;
+    sd.t_inhom = d_ti;
+    sd.has_t_inhom = inp->t_inhom ? 1 : 0;
+    sd.coil_maps = d_maps;
+    sd.log_amp = d_la;
+    sd.z = inp->z;
+    sd.nz = inp->nz;
+    sd.has_eddy_drive = inp->has_eddy_drive;
+    for (int i = 0; i < 3; i++) sd.eddy_drive[i] = inp->eddy_drive[i];
+    mrs_unit_desc ud;
+    std::memset(&ud, 0, sizeof ud);
+    ud.slice = 0;
+    ud.images = d_img;
+    ud.out = d_out;
+    ud.slice_seed = inp->slice_seed;
+    ud.coil0 = coil0;
+    mrs_plan* plan = nullptr;
+    int rc = mrs_plan_create(&gd, 1, &sd, 1, &ud, 1, MRS_PREC_SPLIT_BF16, &plan, err, errlen);
+    if (rc) return rc;
+    rc = mrs_plan_forward(plan, nullptr, err, errlen);
+    std::vector<mrs_c32> h(ncoils_out * nx * ny);
+    if (rc == MRS_OK) {
+        hipError_t e = hipMemcpy(h.data(), d_out, h.size() * sizeof(mrs_c32), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(err, errlen, MRS_E_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    }
+    mrs_plan_destroy(plan);
+    if (rc) return rc;
+    for (size_t i = 0; i < h.size(); i++) out[i] = mrs_c64{(double)h[i].re, (double)h[i].im};
+    return MRS_OK;
+}
+
+}  // namespace
+
+extern "C" int mrs_build_coil_kspace(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil,
+                                     size_t ncoils, const mrs_line_timing* timing, const double* log_amp, mrs_c64* out,
+                                     char* err, size_t errlen) {
+    if (ncoils == 0 || coil >= std::max<size_t>(ncoils, 1))
+        return fail(err, errlen, MRS_E_INVALID, "coil %zu out of range for %zu coils", coil, ncoils);
+    return drop_in(inp, acq, coil, 1, ncoils, timing, log_amp, out, err, errlen);
+}
+
+extern "C" int mrs_build_coil_kspaces(const mrs_slice_input* inp, const mrs_acquisition* acq,
+                                      const mrs_line_timing* timing, const double* log_amp, mrs_c64* out, char* err,
+                                      size_t errlen) {
+    if (!acq) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    const size_t nc = std::max<uint64_t>(acq->n_coils, 1);
+    return drop_in(inp, acq, 0, nc, nc, timing, log_amp, out, err, errlen);
+}

@@ -1,0 +1,373 @@
+"""Python binding of libmrsim_gpu.so (the HIP kernels behind include/mrsim_gpu.h).
+
+The library must be built (``python -m mrsim_amd.build`` or ``__graft_entry__.build()``); loading
+fails loudly otherwise -- there is no CPU fallback.  torch supplies device memory and streams only.
+
+Entry points mirror the reference's per-slice forward:
+  build_coil_kspace   <-> build_coil_kspace_timed / simulate_slice_kspace (src/kspace.rs:597-609, 1144)
+  build_coil_kspaces  <-> build_coil_kspaces (src/kspace.rs:1248-1251)
+and add the batched ``Plan`` (many slice x volume x echo units in one launch) with its adjoint.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import abi
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(PKG), "build", "libmrsim_gpu.so")
+
+_lib = None
+
+
+class MrsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mrsim_gpu error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "(the HIP extension is required; there is no CPU fallback)")
+    L = ct.CDLL(LIB_PATH)
+    P = ct.POINTER
+    E = [ct.c_char_p, ct.c_size_t]
+    L.mrs_acquisition_default.argtypes = [P(abi.Acquisition)]
+    L.mrs_line_timing_for_acquisition.argtypes = [P(abi.Acquisition), ct.c_size_t, ct.c_size_t, P(ct.c_double),
+                                                  P(ct.c_double), P(ct.c_double), P(ct.c_int8)]
+    L.mrs_sampling_mask.argtypes = [ct.c_size_t, ct.c_size_t, P(abi.Acquisition), P(ct.c_uint8)]
+    L.mrs_ring_coil_maps.argtypes = [ct.c_size_t] * 7 + [ct.c_double, P(abi.C32)]
+    L.mrs_build_coil_kspace.argtypes = [P(abi.SliceInput), P(abi.Acquisition), ct.c_size_t, ct.c_size_t,
+                                        P(abi.LineTiming), P(ct.c_double), P(abi.C64)] + E
+    L.mrs_build_coil_kspaces.argtypes = [P(abi.SliceInput), P(abi.Acquisition), P(abi.LineTiming), P(ct.c_double),
+                                         P(abi.C64)] + E
+    L.mrs_plan_create.argtypes = [P(abi.GroupDesc), ct.c_size_t, P(abi.SliceDesc), ct.c_size_t, P(abi.UnitDesc),
+                                  ct.c_size_t, ct.c_int32, P(ct.c_void_p)] + E
+    L.mrs_plan_set_pointers.argtypes = [ct.c_void_p, P(abi.SliceDesc), P(abi.UnitDesc)] + E
+    L.mrs_plan_forward.argtypes = [ct.c_void_p, ct.c_void_p] + E
+    L.mrs_plan_forward_graph.argtypes = [ct.c_void_p, ct.c_void_p] + E
+    L.mrs_plan_vjp.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), P(ct.c_void_p), P(ct.c_void_p),
+                               ct.c_void_p] + E
+    L.mrs_plan_stage_times.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, P(ct.c_double)] + E
+    L.mrs_plan_stats.argtypes = [ct.c_void_p, P(ct.c_double), P(ct.c_double), P(ct.c_double)]
+    L.mrs_plan_destroy.argtypes = [ct.c_void_p]
+    L.mrs_plan_tune.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_char_p, ct.c_size_t]
+    L.mrs_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32] + E
+    L.mrs_plan_get_tiles.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32)]
+    L.mrs_version.restype = ct.c_char_p
+    L.mrs_sizeof.argtypes = [ct.c_char_p]
+    L.mrs_sizeof.restype = ct.c_size_t
+    L.mrs_probe_sincos.argtypes = [P(ct.c_double), P(ct.c_float), P(ct.c_float), ct.c_size_t] + E
+    _lib = L
+    return L
+
+
+def _check(rc, buf):
+    if rc != 0:
+        raise MrsError(rc, buf.value.decode(errors="replace"))
+
+
+def _errbuf():
+    return ct.create_string_buffer(512)
+
+
+# ----------------------------------------------------------------------------- host tables
+
+def line_timing(acq: abi.Acquisition, nx: int, ny: int):
+    """LineTiming::for_acquisition (src/kspace.rs:310-326): (t_ms, trf_ms, tread_ms, polarity)."""
+    t, trf, tread = np.zeros(ny), np.zeros(ny), np.zeros(ny)
+    pol = np.zeros(ny, dtype=np.int8)
+    rc = lib().mrs_line_timing_for_acquisition(ct.byref(acq), nx, ny, abi.dptr(t), abi.dptr(trf), abi.dptr(tread),
+                                               pol.ctypes.data_as(ct.POINTER(ct.c_int8)))
+    if rc:
+        raise MrsError(rc, "invalid acquisition")
+    return t, trf, tread, pol
+
+
+def sampling_mask(nx: int, ny: int, acq: abi.Acquisition) -> np.ndarray:
+    m = np.zeros(nx * ny, dtype=np.uint8)
+    lib().mrs_sampling_mask(nx, ny, ct.byref(acq), m.ctypes.data_as(ct.POINTER(ct.c_uint8)))
+    return m.astype(bool)
+
+
+def ring_coil_maps(n_coils: int, snx: int, sny: int, nx: int, ny: int, coil0: int = 0, n: Optional[int] = None,
+                   scale: float = 1.0) -> np.ndarray:
+    """coil_sensitivity (src/kspace.rs:226-237) on the sim grid, (n, snx*sny) complex64."""
+    n = n_coils if n is None else n
+    out = np.zeros((n, snx * sny), dtype=np.complex64)
+    rc = lib().mrs_ring_coil_maps(coil0, n, n_coils, snx, sny, nx, ny, scale,
+                                  out.ctypes.data_as(ct.POINTER(abi.C32)))
+    if rc:
+        raise MrsError(rc, "invalid coil map request")
+    return out
+
+
+def _timing_struct(timing, ny):
+    if timing is None:
+        return None, []
+    keep = [np.ascontiguousarray(v, dtype=np.float64) for v in timing[:3]]
+    pol = np.ascontiguousarray(timing[3], dtype=np.int8)
+    keep.append(pol)
+    lt = abi.LineTiming(abi.dptr(keep[0]), abi.dptr(keep[1]), abi.dptr(keep[2]),
+                        pol.ctypes.data_as(ct.POINTER(ct.c_int8)), len(keep[0]))
+    return lt, keep
+
+
+# ----------------------------------------------------------------------------- drop-in calls
+
+def build_coil_kspace(b: abi.SliceInputBuilder, acq: abi.Acquisition, coil: int = 0, ncoils: int = 1,
+                      timing=None, log_amp=None) -> np.ndarray:
+    """One coil's acquired k-space (nx*ny complex128, layout kx + nx*ky) computed on the GPU."""
+    nx, ny = b.acq_matrix
+    out = np.zeros(nx * ny, dtype=np.complex128)
+    lt, keep = _timing_struct(timing, ny)
+    la = np.ascontiguousarray(log_amp, dtype=np.float64) if log_amp is not None else None
+    err = _errbuf()
+    rc = lib().mrs_build_coil_kspace(ct.byref(b.inp), ct.byref(acq), coil, ncoils,
+                                     ct.byref(lt) if lt is not None else None,
+                                     abi.dptr(la) if la is not None else None,
+                                     out.ctypes.data_as(ct.POINTER(abi.C64)), err, len(err))
+    _check(rc, err)
+    return out
+
+
+def build_coil_kspaces(b: abi.SliceInputBuilder, acq: abi.Acquisition, timing=None, log_amp=None) -> np.ndarray:
+    """All coils of one slice in one launch: (n_coils, nx*ny) complex128."""
+    nx, ny = b.acq_matrix
+    nc = max(int(acq.n_coils), 1)
+    out = np.zeros((nc, nx * ny), dtype=np.complex128)
+    lt, keep = _timing_struct(timing, ny)
+    la = np.ascontiguousarray(log_amp, dtype=np.float64) if log_amp is not None else None
+    err = _errbuf()
+    rc = lib().mrs_build_coil_kspaces(ct.byref(b.inp), ct.byref(acq), ct.byref(lt) if lt is not None else None,
+                                      abi.dptr(la) if la is not None else None,
+                                      out.ctypes.data_as(ct.POINTER(abi.C64)), err, len(err))
+    _check(rc, err)
+    return out
+
+
+def probe_sincos(cyc: np.ndarray):
+    cyc = np.ascontiguousarray(cyc, dtype=np.float64)
+    s = np.zeros(cyc.size, dtype=np.float32)
+    c = np.zeros(cyc.size, dtype=np.float32)
+    err = _errbuf()
+    rc = lib().mrs_probe_sincos(abi.dptr(cyc), abi.fptr(s), abi.fptr(c), cyc.size, err, len(err))
+    _check(rc, err)
+    return s, c
+
+
+# ----------------------------------------------------------------------------- batched plan
+
+@dataclass
+class Group:
+    """Geometry and protocol shared by a set of units."""
+    snx: int
+    sny: int
+    nx: int
+    ny: int
+    n_coils: int
+    n_comp: int
+    acq: abi.Acquisition
+    timing: Optional[tuple] = None
+    complex_images: bool = False
+
+
+@dataclass
+class Slice:
+    """Static per-slice inputs: torch device tensors (flattened sim-grid planes)."""
+    group: int
+    fmap: object                     # (P,) float32
+    coil_maps: object                # (C, P) complex64
+    phase0: object = None            # (P,) float64
+    t2: object = None                # (Q, P) float32 maps
+    t2_uniform: Sequence[float] = (100.0,) * abi.MRS_MAX_COMP
+    t_inhom: object = None           # (Q, P) float32 maps
+    t_inhom_uniform: Optional[Sequence[float]] = None
+    log_amp: object = None           # (P,) float64
+    z: int = 0
+    nz: int = 1
+    eddy_drive: Optional[Sequence[float]] = None
+
+
+@dataclass
+class Unit:
+    """One (slice, volume, echo) forward: images (Q, P) float32 or complex64 -> out (C, ny, nx) complex64."""
+    slice: int
+    images: object
+    out: object
+    slice_seed: int = 0
+    coil0: int = 0
+
+
+def _ptr(t):
+    return None if t is None else ct.c_void_p(t.data_ptr())
+
+
+def _check_dev(name, t, dtype, numel=None):
+    import torch
+    if t is None:
+        return
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise TypeError(f"{name} must be a CUDA (HIP) tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if numel is not None and t.numel() != numel:
+        raise ValueError(f"{name} has {t.numel()} elements, expected {numel}")
+
+
+class Plan:
+    """Batched forward (and adjoint) of many units in one launch per stage."""
+
+    def __init__(self, groups: Sequence[Group], slices: Sequence[Slice], units: Sequence[Unit],
+                 precision: int = abi.PREC_SPLIT_BF16):
+        self.groups, self.slices, self.units = list(groups), list(slices), list(units)
+        self.precision = precision
+        self._keep = []
+        gd = (abi.GroupDesc * len(groups))()
+        for i, g in enumerate(groups):
+            gd[i].snx, gd[i].sny, gd[i].nx, gd[i].ny = g.snx, g.sny, g.nx, g.ny
+            gd[i].n_coils, gd[i].n_compartments = g.n_coils, g.n_comp
+            gd[i].complex_images = 1 if g.complex_images else 0
+            gd[i].acq = g.acq
+            if g.timing is not None:
+                lt, keep = _timing_struct(g.timing, g.ny)
+                self._keep += [lt, keep]
+                gd[i].timing = ct.pointer(lt)
+        sd, ud = self._descs()
+        h = ct.c_void_p()
+        err = _errbuf()
+        rc = lib().mrs_plan_create(gd, len(groups), sd, len(slices), ud, len(units), precision, ct.byref(h), err,
+                                   len(err))
+        _check(rc, err)
+        self._h = h
+
+    def _descs(self):
+        import torch
+        sd = (abi.SliceDesc * len(self.slices))()
+        for i, s in enumerate(self.slices):
+            g = self.groups[s.group]
+            P = g.snx * g.sny
+            _check_dev(f"slice {i} fmap", s.fmap, torch.float32, P)
+            _check_dev(f"slice {i} coil_maps", s.coil_maps, torch.complex64, g.n_coils * P)
+            _check_dev(f"slice {i} phase0", s.phase0, torch.float64, P)
+            _check_dev(f"slice {i} t2", s.t2, torch.float32, g.n_comp * P)
+            _check_dev(f"slice {i} t_inhom", s.t_inhom, torch.float32, g.n_comp * P)
+            _check_dev(f"slice {i} log_amp", s.log_amp, torch.float64, P)
+            d = sd[i]
+            d.group = s.group
+            d.fmap, d.coil_maps = _ptr(s.fmap), _ptr(s.coil_maps)
+            d.phase0, d.t2, d.t_inhom, d.log_amp = _ptr(s.phase0), _ptr(s.t2), _ptr(s.t_inhom), _ptr(s.log_amp)
+            for q in range(min(len(s.t2_uniform), abi.MRS_MAX_COMP)):
+                d.t2_uniform[q] = s.t2_uniform[q]
+            if s.t_inhom_uniform is not None:
+                d.has_t_inhom = 1
+                for q in range(min(len(s.t_inhom_uniform), abi.MRS_MAX_COMP)):
+                    d.t_inhom_uniform[q] = s.t_inhom_uniform[q]
+            if s.t_inhom is not None:
+                d.has_t_inhom = 1
+            d.z, d.nz = s.z, s.nz
+            if s.eddy_drive is not None:
+                d.has_eddy_drive = 1
+                for k in range(3):
+                    d.eddy_drive[k] = float(s.eddy_drive[k])
+        ud = (abi.UnitDesc * len(self.units))()
+        for i, u in enumerate(self.units):
+            g = self.groups[self.slices[u.slice].group]
+            P = g.snx * g.sny
+            _check_dev(f"unit {i} images", u.images, torch.complex64 if g.complex_images else torch.float32,
+                       g.n_comp * P)
+            _check_dev(f"unit {i} out", u.out, torch.complex64, g.n_coils * g.ny * g.nx)
+            ud[i].slice = u.slice
+            ud[i].images, ud[i].out = _ptr(u.images), _ptr(u.out)
+            ud[i].slice_seed, ud[i].coil0 = u.slice_seed, u.coil0
+        self._sd, self._ud = sd, ud
+        return sd, ud
+
+    def set_pointers(self, slices=None, units=None):
+        if slices is not None:
+            self.slices = list(slices)
+        if units is not None:
+            self.units = list(units)
+        sd, ud = self._descs()
+        err = _errbuf()
+        _check(lib().mrs_plan_set_pointers(self._h, sd, ud, err, len(err)), err)
+
+    @staticmethod
+    def _stream(stream):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        return ct.c_void_p(stream.cuda_stream)
+
+    def forward(self, stream=None):
+        err = _errbuf()
+        _check(lib().mrs_plan_forward(self._h, self._stream(stream), err, len(err)), err)
+
+    def forward_graph(self, stream=None):
+        err = _errbuf()
+        _check(lib().mrs_plan_forward_graph(self._h, self._stream(stream), err, len(err)), err)
+
+    def vjp(self, grad_out, grad_images=None, grad_fmap=None, grad_t_inhom=None, stream=None):
+        n = len(self.units)
+
+        def arr(ts):
+            a = (ct.c_void_p * n)()
+            if ts is not None:
+                for i, t in enumerate(ts):
+                    a[i] = None if t is None else t.data_ptr()
+            return a
+
+        err = _errbuf()
+        rc = lib().mrs_plan_vjp(self._h, arr(grad_out), arr(grad_images), arr(grad_fmap), arr(grad_t_inhom),
+                                self._stream(stream), err, len(err))
+        _check(rc, err)
+
+    def tune(self, trials=3, stream=None) -> str:
+        rep = ct.create_string_buffer(4096)
+        rc = lib().mrs_plan_tune(self._h, self._stream(stream), trials, rep, len(rep))
+        if rc:
+            raise MrsError(rc, "tuning failed")
+        return rep.value.decode()
+
+    def set_tiles(self, x_tile, line_tiles):
+        err = _errbuf()
+        _check(lib().mrs_plan_set_tiles(self._h, x_tile, line_tiles, err, len(err)), err)
+
+    def tiles(self):
+        a, b = ct.c_int32(), ct.c_int32()
+        lib().mrs_plan_get_tiles(self._h, ct.byref(a), ct.byref(b))
+        return a.value, b.value
+
+    def stage_times(self, iters=5, stream=None):
+        ms = (ct.c_double * 4)()
+        err = _errbuf()
+        _check(lib().mrs_plan_stage_times(self._h, self._stream(stream), iters, ms, err, len(err)), err)
+        return {"prep": ms[0], "ystage": ms[1], "xstage": ms[2], "epilogue": ms[3]}
+
+    def stats(self):
+        t, l, u = ct.c_double(), ct.c_double(), ct.c_double()
+        lib().mrs_plan_stats(self._h, ct.byref(t), ct.byref(l), ct.byref(u))
+        return {"terms": t.value, "lines": l.value, "units": u.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mrs_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,211 @@
+"""Synthetic inputs of the benchmark workloads, seeded from the reference's SplitMix64 generator
+(src/kspace.rs:60-77), evaluated counter-wise and vectorised: draw n of a stream seeded s is
+mix(s + n * 0x9E3779B97F4A7C15).
+
+Workloads (sizes are the benchmark definitions):
+  slice_128      one slice, 128^2 -> 64^2, 1 coil, 1 compartment, T2/T2' maps, spin echo
+  batch_128      30 slices x 20 volumes x 8 coils x 3 compartments, 128^2 -> 64^2
+  pf_256         24 slices x 16 coils, 256^2 -> 128^2, partial Fourier 6/8 (97 lines), Hann recon window
+  grad_128       one slice, 128^2 -> 64^2, 8 coils, 3 compartments, complex64 images (adjoint checks)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+GAMMA = np.uint64(0x9E3779B97F4A7C15)
+M1 = np.uint64(0xBF58476D1CE4E5B9)
+M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix(seed: int, n: int, start: int = 1) -> np.ndarray:
+    """Draws start..start+n-1 of Rng(seed) as uint64."""
+    with np.errstate(over="ignore"):
+        k = np.arange(start, start + n, dtype=np.uint64)
+        z = np.uint64(seed) + k * GAMMA
+        z = (z ^ (z >> np.uint64(30))) * M1
+        z = (z ^ (z >> np.uint64(27))) * M2
+        return z ^ (z >> np.uint64(31))
+
+
+class Stream:
+    """A SplitMix64 stream (Rng) with vectorised draws."""
+
+    def __init__(self, seed: int):
+        self.seed = seed & 0xFFFFFFFFFFFFFFFF
+        self.pos = 1
+
+    def u64(self, n):
+        v = splitmix(self.seed, n, self.pos)
+        self.pos += n
+        return v
+
+    def unit(self, n=None):
+        m = 1 if n is None else n
+        v = ((self.u64(m) >> np.uint64(11)).astype(np.float64) + 0.5) / float(1 << 53)
+        return v[0] if n is None else v
+
+    def uniform(self, lo, hi, n=None):
+        return lo + (hi - lo) * self.unit(n)
+
+    def gauss(self, n):
+        u = self.unit(2 * n).reshape(n, 2)
+        return np.sqrt(-2.0 * np.log(u[:, 0])) * np.cos(2 * np.pi * u[:, 1])
+
+
+def blob_image(st: Stream, snx: int, sny: int, nblobs: int = 8) -> np.ndarray:
+    """Sum of Gaussian blobs, amplitude U(0,1), sigma U(4,20) px, centre U(16, n-16), phase U(-pi,pi)."""
+    y, x = np.mgrid[0:sny, 0:snx].astype(np.float64)
+    m = np.zeros((sny, snx), dtype=np.complex128)
+    for _ in range(nblobs):
+        a, sg = st.uniform(0, 1), st.uniform(4, 20)
+        cx, cy = st.uniform(16, snx - 16), st.uniform(16, sny - 16)
+        ph = st.uniform(-np.pi, np.pi)
+        m += a * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * sg * sg)) * np.exp(1j * ph)
+    return m.reshape(-1)
+
+
+def fieldmap(st: Stream, snx: int, sny: int, n: int = 4) -> np.ndarray:
+    """Sum of Gaussians of amplitude +-U(20,120) Hz, width U(10,40) px."""
+    y, x = np.mgrid[0:sny, 0:snx].astype(np.float64)
+    f = np.zeros((sny, snx))
+    for _ in range(n):
+        amp = st.uniform(20, 120) * (1 if st.unit() < 0.5 else -1)
+        s = st.uniform(10, 40)
+        cx, cy = st.uniform(16, snx - 16), st.uniform(16, sny - 16)
+        f += amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * s * s))
+    return f.reshape(-1).astype(np.float32)
+
+
+def gaussian_ring_coils(n: int, snx: int, sny: int, sigma: float, radius: float, centre=None, z_gain=None):
+    """Complex Gaussian sensitivities on a ring, phase 2 pi c / n: (n, P) complex64."""
+    cx0, cy0 = centre if centre is not None else (snx / 2, sny / 2)
+    y, x = np.mgrid[0:sny, 0:snx].astype(np.float64)
+    out = np.zeros((n, snx * sny), dtype=np.complex64)
+    for c in range(n):
+        ang = 2 * np.pi * c / n
+        px, py = cx0 + radius * np.cos(ang), cy0 + radius * np.sin(ang)
+        g = np.exp(-((x - px) ** 2 + (y - py) ** 2) / (2 * sigma * sigma)) * np.exp(1j * 2 * np.pi * c / n)
+        if z_gain is not None:
+            g = g * z_gain[c]
+        out[c] = g.reshape(-1)
+    return out
+
+
+@dataclass
+class SliceData:
+    fmap: np.ndarray
+    phase0: np.ndarray
+    t2: np.ndarray          # (Q, P)
+    t_inhom: np.ndarray     # (Q, P)
+    coil_maps: np.ndarray   # (C, P)
+    mag: np.ndarray         # (P,)
+    weights: np.ndarray     # (Q, P) Dirichlet
+
+
+@dataclass
+class Workload:
+    name: str
+    snx: int
+    sny: int
+    nx: int
+    ny: int
+    n_coils: int
+    n_comp: int
+    acq: abi.Acquisition
+    slices: list = field(default_factory=list)   # SliceData
+    units: list = field(default_factory=list)    # (slice index, images (Q,P) float32 or complex64)
+    complex_images: bool = False
+    recon_window: int = abi.WINDOW_NONE
+
+
+def _slice(st: Stream, snx, sny, Q, C, coil_sigma, coil_radius):
+    P = snx * sny
+    m = blob_image(st, snx, sny)
+    fm = fieldmap(st, snx, sny)
+    t2 = st.uniform(40, 120, Q * P).reshape(Q, P).astype(np.float32)
+    ti = st.uniform(20, 80, Q * P).reshape(Q, P).astype(np.float32)
+    e = -np.log(st.unit(Q * P).reshape(Q, P))
+    w = e / e.sum(axis=0, keepdims=True)
+    coils = gaussian_ring_coils(C, snx, sny, coil_sigma, coil_radius) if C > 1 else np.ones((1, P), np.complex64)
+    return SliceData(fmap=fm, phase0=np.angle(m), t2=t2, t_inhom=ti, coil_maps=coils, mag=np.abs(m), weights=w)
+
+
+def _volume_images(st: Stream, sd: SliceData, g: int):
+    Q, P = sd.weights.shape
+    pert = 1.0 + 0.02 * (2.0 * st.unit(Q * P).reshape(Q, P) - 1.0)
+    return ((1.0 + 0.01 * g) * sd.weights * sd.mag[None, :] * pert).astype(np.float32)
+
+
+def make(name: str, n_slices=None, n_volumes=None, seed_base=0x5EED0000) -> Workload:
+    if name == "slice_128":
+        acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=1)
+        w = Workload(name, 128, 128, 64, 64, 1, 1, acq)
+        st = Stream(seed_base + 0)
+        sd = _slice(st, 128, 128, 1, 1, 60.0, 90.0)
+        sd.weights[:] = 1.0
+        w.slices.append(sd)
+        w.units.append((0, (sd.mag[None, :]).astype(np.float32)))
+        return w
+    if name == "batch_128":
+        ns = 30 if n_slices is None else n_slices
+        nv = 20 if n_volumes is None else n_volumes
+        acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=8)
+        w = Workload(name, 128, 128, 64, 64, 8, 3, acq)
+        for s in range(ns):
+            st = Stream(seed_base + 1 + 7919 * s)
+            sd = _slice(st, 128, 128, 3, 8, 60.0, 90.0)
+            w.slices.append(sd)
+            for g in range(nv):
+                w.units.append((s, _volume_images(st, sd, g)))
+        return w
+    if name == "pf_256":
+        ns = 24 if n_slices is None else n_slices
+        acq = abi.Acquisition.default(t_line=0.5, t_echo=50.0, signal_scale=1.0, n_coils=16,
+                                      partial_fourier=0.75, window=abi.WINDOW_HANN)
+        w = Workload(name, 256, 256, 128, 128, 16, 1, acq, recon_window=abi.WINDOW_HANN)
+        for s in range(ns):
+            st = Stream(seed_base + 2 + 7919 * s)
+            sd = _slice(st, 256, 256, 1, 16, 120.0, 180.0)
+            sd.weights[:] = 1.0
+            w.slices.append(sd)
+            w.units.append((s, sd.mag[None, :].astype(np.float32)))
+        return w
+    if name == "grad_128":
+        acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=8)
+        w = Workload(name, 128, 128, 64, 64, 8, 3, acq, complex_images=True)
+        st = Stream(seed_base + 3)
+        sd = _slice(st, 128, 128, 3, 8, 60.0, 90.0)
+        w.slices.append(sd)
+        img = (sd.weights * (sd.mag * np.exp(1j * sd.phase0))[None, :]).astype(np.complex64)
+        w.units.append((0, img))
+        return w
+    raise KeyError(name)
+
+
+def to_plan(w: Workload, precision=abi.PREC_SPLIT_BF16, device="cuda"):
+    """Upload a workload and build its Plan.  Returns (plan, outputs, keep-alive tensors)."""
+    import torch
+
+    from .gpu import Group, Plan, Slice, Unit
+    g = Group(w.snx, w.sny, w.nx, w.ny, w.n_coils, w.n_comp, w.acq, complex_images=w.complex_images)
+    keep, slices, units, outs = [], [], [], []
+    for sd in w.slices:
+        t = dict(fmap=torch.from_numpy(sd.fmap).to(device),
+                 coil_maps=torch.from_numpy(np.ascontiguousarray(sd.coil_maps)).to(device),
+                 phase0=None if w.complex_images else torch.from_numpy(sd.phase0).to(device),
+                 t2=torch.from_numpy(np.ascontiguousarray(sd.t2)).to(device),
+                 t_inhom=torch.from_numpy(np.ascontiguousarray(sd.t_inhom)).to(device))
+        keep.append(t)
+        slices.append(Slice(0, t["fmap"], t["coil_maps"], phase0=t["phase0"], t2=t["t2"], t_inhom=t["t_inhom"]))
+    for si, img in w.units:
+        ti = torch.from_numpy(np.ascontiguousarray(img)).to(device)
+        out = torch.zeros((w.n_coils, w.ny, w.nx), dtype=torch.complex64, device=device)
+        keep.append(ti)
+        outs.append(out)
+        units.append(Unit(si, ti, out))
+    plan = Plan([g], slices, units, precision)
+    return plan, outs, keep

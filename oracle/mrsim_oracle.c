@@ -904,3 +904,109 @@ void orc_box_hires(size_t snx, size_t sny, double x0, double x1, double y0, doub
         }
     }
 }
+
+/* ---------------------------------------------------------------- direct FP64 forward, plan inputs
+ * The forward of build_coil_kspace_timed (src/kspace.rs:607-994) evaluated directly per (voxel,
+ * line) in FP64 -- the literal sum's closed-form phase (src/kspace.rs:2345-2383) -- on the inputs of
+ * the batched plan API: explicit complex coil maps (mrs_c32, times signal_scale), real or complex
+ * compartment images, T2/T2' maps or uniforms, optional timing table and log_amp.  Host pointers in
+ * the descriptors.  No spikes or noise.  out: C x ny x nx.  Pinned against the restatement above
+ * (ring maps) and by closed forms (tests/test_oracle.py). */
+int orc_forward_direct(const mrs_group_desc* g, const mrs_slice_desc* s, const void* images, C* out) {
+    const size_t snx = g->snx, sny = g->sny, nx = g->nx, ny = g->ny, Q = g->n_compartments, NC = g->n_coils;
+    if (snx % nx != 0 || sny % ny != 0) return -1;
+    const mrs_acquisition* acq = &g->acq;
+    const size_t P = snx * sny;
+    double* t_ms = malloc(sizeof(double) * ny * 3);
+    double* trf_ms = t_ms + ny;
+    double* tread_ms = t_ms + 2 * ny;
+    int8_t* pol = malloc(ny);
+    if (g->timing) {
+        memcpy(t_ms, g->timing->t_ms, ny * 8);
+        memcpy(trf_ms, g->timing->trf_ms, ny * 8);
+        memcpy(tread_ms, g->timing->tread_ms, ny * 8);
+        memcpy(pol, g->timing->polarity, ny);
+    } else {
+        orc_line_timing(acq, nx, ny, t_ms, trf_ms, tread_ms, pol);
+    }
+    uint8_t* mask = malloc(nx * ny);
+    orc_sampling_mask(nx, ny, acq, mask);
+    const int eddy_on = acq->eddy_strength != 0.0 && s->has_eddy_drive;
+    const int eddy_ph = acq->eddy_phase != 0.0 && s->has_eddy_drive;
+    const double g0 = s->eddy_drive[0], g1 = s->eddy_drive[1], g2 = s->eddy_drive[2];
+    const size_t xs = nx / 2, ys = ny / 2, zs = s->nz / 2;
+    const size_t ox = snx / nx, oy = sny / ny;
+    const size_t sxs = ox * xs, sys = oy * ys;
+    const double xoff = ((double)ox - 1.0) / 2.0, yoff = ((double)oy - 1.0) / 2.0;
+    const double zc = (double)s->z - (double)zs, zc_c = (double)s->z - ((double)s->nz - 1.0) / 2.0;
+    double* phi0 = malloc(sizeof(double) * P);
+    double* poly = malloc(sizeof(double) * P);
+    double* rate = malloc(sizeof(double) * P);
+    for (size_t y = 0; y < sny; y++)
+        for (size_t x = 0; x < snx; x++) {
+            size_t i = x + snx * y;
+            double xc = ((double)x - (double)sxs - xoff) / (double)ox, yc = ((double)y - (double)sys - yoff) / (double)oy;
+            double p0 = s->phase0 ? s->phase0[i] : 0.0;
+            if (eddy_ph) p0 += acq->eddy_phase * (g0 * xc + g1 * yc + g2 * zc_c);
+            if (acq->echo == MRS_ECHO_GRADIENT && acq->do_distortions) p0 += TAU * (double)s->fmap[i] * (acq->t_echo / 1000.0);
+            phi0[i] = p0;
+            rate[i] = acq->do_distortions ? (double)s->fmap[i] : 0.0;
+            poly[i] = eddy_on ? acq->eddy_strength * (g0 * xc + g1 * yc + g2 * zc) +
+                                    acq->eddy_quad * (g0 * xc * xc + g1 * yc * yc + g2 * zc * zc)
+                              : 0.0;
+        }
+    C* G = malloc(sizeof(C) * NC * snx);
+    const double cctr = (double)sxs + xoff, ninv = 1.0 / (double)P;
+    for (size_t i = 0; i < NC * nx * ny; i++) { out[i].re = 0.0; out[i].im = 0.0; }
+    for (size_t ky = 0; ky < ny; ky++) {
+        if (!mask[nx * ky]) continue;
+        const double t = t_ms[ky] / 1000.0, trf = trf_ms[ky];
+        const double tau = acq->echo == MRS_ECHO_SPIN ? fabs(t) * 1000.0 : trf;
+        const double d = eddy_on ? exp(-tread_ms[ky] / acq->eddy_tau) * t : 0.0;
+        const double kyn = ((double)ky - (double)ys) / (double)sny;
+        memset(G, 0, sizeof(C) * NC * snx);
+        for (size_t y = 0; y < sny; y++) {
+            const double yy = (double)y - (double)sys - yoff;
+            for (size_t x = 0; x < snx; x++) {
+                const size_t i = x + snx * y;
+                C w = {0.0, 0.0};
+                for (size_t q = 0; q < Q; q++) {
+                    double r = 1.0;
+                    if (acq->do_relaxation) {
+                        double t2 = s->t2 ? (double)s->t2[q * P + i] : (double)s->t2_uniform[q];
+                        double ti = s->t_inhom ? (double)s->t_inhom[q * P + i]
+                                               : (s->has_t_inhom ? (double)s->t_inhom_uniform[q] : acq->t_inhom);
+                        r = exp(-trf / t2 - tau / ti);
+                    }
+                    if (g->complex_images) {
+                        const float* m = (const float*)images + 2 * (q * P + i);
+                        w.re += r * (double)m[0];
+                        w.im += r * (double)m[1];
+                    } else {
+                        w.re += r * (double)((const float*)images)[q * P + i];
+                    }
+                }
+                if (s->log_amp) { double e = exp(s->log_amp[i]); w.re *= e; w.im *= e; }
+                const double ph = phi0[i] + TAU * (rate[i] * t + kyn * yy + poly[i] * d);
+                const C e = c_mul(w, c_cis(ph));
+                for (size_t c = 0; c < NC; c++) {
+                    const mrs_c32 sv = s->coil_maps[c * P + i];
+                    const C sc = {(double)sv.re * acq->signal_scale, (double)sv.im * acq->signal_scale};
+                    G[c * snx + x] = c_add(G[c * snx + x], c_mul(sc, e));
+                }
+            }
+        }
+        const double shift = pol[ky] < 0 ? -acq->ghost_offset : acq->ghost_offset;
+        const double gh = (shift < 0.0) ? -acq->ghost_offset : acq->ghost_offset;
+        for (size_t kx = 0; kx < nx; kx++) {
+            const double kxn = ((double)kx - (double)xs + gh) / (double)snx;
+            for (size_t c = 0; c < NC; c++) {
+                C acc = {0.0, 0.0};
+                for (size_t x = 0; x < snx; x++) acc = c_add(acc, c_mul(G[c * snx + x], c_cis(TAU * kxn * ((double)x - cctr))));
+                out[(c * ny + ky) * nx + kx] = c_scale(acc, ninv);
+            }
+        }
+    }
+    free(t_ms); free(pol); free(mask); free(phi0); free(poly); free(rate); free(G);
+    return 0;
+}

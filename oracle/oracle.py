@@ -204,3 +204,72 @@ def golden_slice_builder(shape_idx, eddy_drive, maps, slice_seed=7):
     ti = [tim, 45.0, tim] if maps else None
     return abi.SliceInputBuilder(list(comps), t2, fmap, (nx * o, ny * o), (nx, ny), z=3, nz=9, t_inhom=ti,
                                  phase0=phase0, eddy_drive=eddy_drive, slice_seed=slice_seed)
+
+
+def forward_direct(snx, sny, nx, ny, n_coils, n_comp, acq, images, fmap, coil_maps, phase0=None, t2=None,
+                   t2_uniform=None, t_inhom=None, t_inhom_uniform=None, log_amp=None, z=0, nz=1, eddy_drive=None,
+                   timing=None, complex_images=False):
+    """FP64 forward on the batched plan's inputs (orc_forward_direct): (C, ny, nx) complex128."""
+    L = lib()
+    if not hasattr(L, "_fd"):
+        L.orc_forward_direct.argtypes = [ct.POINTER(abi.GroupDesc), ct.POINTER(abi.SliceDesc), ct.c_void_p,
+                                         ct.POINTER(C64)]
+        L._fd = True
+    keep = []
+
+    def arr(a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    g = abi.GroupDesc()
+    g.snx, g.sny, g.nx, g.ny, g.n_coils, g.n_compartments = snx, sny, nx, ny, n_coils, n_comp
+    g.complex_images = 1 if complex_images else 0
+    g.acq = acq
+    if timing is not None:
+        tk = [np.ascontiguousarray(v, dtype=np.float64) for v in timing[:3]]
+        pol = np.ascontiguousarray(timing[3], dtype=np.int8)
+        keep += tk + [pol]
+        lt = LineTiming(abi.dptr(tk[0]), abi.dptr(tk[1]), abi.dptr(tk[2]),
+                        pol.ctypes.data_as(ct.POINTER(ct.c_int8)), ny)
+        keep.append(lt)
+        g.timing = ct.pointer(lt)
+    s = abi.SliceDesc()
+    s.group = 0
+    s.fmap = arr(fmap, np.float32)
+    s.coil_maps = arr(coil_maps, np.complex64)
+    s.phase0 = arr(phase0, np.float64)
+    s.t2 = arr(t2, np.float32)
+    s.t_inhom = arr(t_inhom, np.float32)
+    s.log_amp = arr(log_amp, np.float64)
+    if t2_uniform is not None:
+        for q, v in enumerate(t2_uniform):
+            s.t2_uniform[q] = v
+    if t_inhom_uniform is not None:
+        s.has_t_inhom = 1
+        for q, v in enumerate(t_inhom_uniform):
+            s.t_inhom_uniform[q] = v
+    if t_inhom is not None:
+        s.has_t_inhom = 1
+    s.z, s.nz = z, nz
+    if eddy_drive is not None:
+        s.has_eddy_drive = 1
+        for k in range(3):
+            s.eddy_drive[k] = eddy_drive[k]
+    img = arr(images, np.complex64 if complex_images else np.float32)
+    out = np.zeros(n_coils * ny * nx, dtype=np.complex128)
+    rc = L.orc_forward_direct(ct.byref(g), ct.byref(s), img, _c64(out))
+    if rc != 0:
+        raise ValueError("oracle direct forward refused the input")
+    return out.reshape(n_coils, ny, nx)
+
+
+def forward_workload_unit(w, unit_index):
+    """FP64 oracle of one unit of a workload (mrsim_amd.workloads.Workload)."""
+    si, img = w.units[unit_index]
+    sd = w.slices[si]
+    return forward_direct(w.snx, w.sny, w.nx, w.ny, w.n_coils, w.n_comp, w.acq, img, sd.fmap, sd.coil_maps,
+                          phase0=None if w.complex_images else sd.phase0, t2=sd.t2, t_inhom=sd.t_inhom,
+                          complex_images=w.complex_images)

@@ -1,0 +1,177 @@
+"""GPU parity of the forward model against the FP64 oracle and the reference's golden fixtures.
+
+Tolerances (relative L2 over each coil's k-space, or the reconstructed image):
+  split-BF16 MFMA path  <= 1e-5   (three-term split products, FP32 accumulation, FP64 phases)
+  plain FP32 path       <= 1e-5
+Unacquired rows must be exactly zero.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from mrsim_amd import abi, gpu, workloads
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    gpu.lib()
+
+
+def test_phase_evaluation_accuracy():
+    rng = np.random.default_rng(1)
+    cyc = np.concatenate([rng.uniform(-300, 300, 200000), np.linspace(-2, 2, 4097),
+                          np.array([0.0, 0.25, 0.5, -0.25, 0.125, 1e-9])])
+    s, c = gpu.probe_sincos(cyc)
+    r = cyc - np.rint(cyc)
+    es = np.abs(s - np.sin(2 * np.pi * r)).max()
+    ec = np.abs(c - np.cos(2 * np.pi * r)).max()
+    print(f"sincos max abs error: sin {es:.2e} cos {ec:.2e}", flush=True)
+    assert es < 1e-6 and ec < 1e-6
+
+
+CASES = [(s, c[0]) for s in range(3) for c in O.golden_cases()]
+
+
+@pytest.mark.parametrize("shape,name", CASES, ids=[f"{O.GOLDEN_SHAPES[s][0]}x{O.GOLDEN_SHAPES[s][1]}-{n}"
+                                                     for s, n in CASES])
+def test_drop_in_coil_kspaces_match_oracle(shape, name):
+    _, acq, ed, maps = {c[0]: c for c in O.golden_cases()}[name]
+    b = O.golden_slice_builder(shape, ed, maps)
+    got = gpu.build_coil_kspaces(b, acq)
+    want = O.coil_kspaces(b, acq)
+    nx, ny = b.acq_matrix
+    mask = O.sampling_mask(nx, ny, acq)
+    for c in range(got.shape[0]):
+        e = rel(got[c], want[c])
+        assert e <= TOL, f"coil {c}: rel-L2 {e:.2e}"
+        assert np.all(got[c][~mask] == 0)
+
+
+@pytest.mark.parametrize("shape,name", CASES, ids=[f"{O.GOLDEN_SHAPES[s][0]}x{O.GOLDEN_SHAPES[s][1]}-{n}"
+                                                     for s, n in CASES])
+def test_gpu_kspace_reconstructs_to_golden_image(shape, name):
+    # simulate_slice (src/kspace.rs:1229-1284) with every coil k-space from the GPU, the
+    # reconstruction from the oracle, against the reference's golden record.
+    import os
+    nx, ny, o = O.GOLDEN_SHAPES[shape]
+    _, acq, ed, maps = {c[0]: c for c in O.golden_cases()}[name]
+    b = O.golden_slice_builder(shape, ed, maps)
+    weights = None
+    if acq.accel > 1:
+        ref = O.golden_slice_builder(shape, None, maps, slice_seed=7 ^ 0x47525043414C)
+        acq_ref = acq.copy(accel=1, n_spikes=0)
+        coils_ref = gpu.build_coil_kspaces(ref, acq_ref)
+        lines = O.sampling_mask(1, ny, acq_ref)
+        weights = O.grappa_fit(coils_ref, nx, ny, acq.accel, acq.acs_lines, lines)
+    coils = gpu.build_coil_kspaces(b, acq)
+    img = O.reconstruct(coils, nx, ny, acq, weights)
+    gold = np.fromfile(os.path.join(os.path.dirname(__file__), "golden", "p4_baseline", "default",
+                                    f"slice-{nx}x{ny}o{o}-{name}.bin"), dtype=np.float32)
+    gold = gold[0::2] + 1j * gold[1::2]
+    e = rel(img, gold)
+    assert e <= TOL, f"rel-L2 {e:.2e}"
+
+
+def test_single_coil_call_spikes_timing_and_log_amp():
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(0)
+    nx, ny, o = O.GOLDEN_SHAPES[0]
+    b = abi.SliceInputBuilder(list(comps), [t2m, 100.0, t2m], fmap, (nx * o, ny * o), (nx, ny), z=3, nz=9,
+                              t_inhom=[tim, 45.0, tim], phase0=phase0, eddy_drive=(0.3, -0.8, 0.5), slice_seed=11)
+    acq = abi.Acquisition.default(n_coils=4, eddy_strength=1.0, ghost_offset=0.01, n_spikes=5, noise_variance=1.0,
+                                  echo=abi.ECHO_GRADIENT, t_echo=40.0)
+    got = gpu.build_coil_kspace(b, acq, coil=2, ncoils=4)
+    want = O.coil_kspace(b, acq, 2, 4)
+    # spikes (full-peak samples) at other positions would leave an O(1) relative error
+    assert rel(got, want) <= TOL
+    nxny = nx * ny
+    # explicit timing table (a shifted echo) and a log-amplitude map
+    t, trf, tread, pol = O.line_timing(acq, nx, ny)
+    timing = (t, trf + 3.0, tread, pol)
+    la = 0.1 * np.sin(np.arange(nx * o * ny * o) / 17.0)
+    acq2 = acq.copy(n_spikes=0, noise_variance=0.0)
+    got = gpu.build_coil_kspace(b, acq2, coil=1, ncoils=4, timing=timing, log_amp=la)
+    want = O.coil_kspace(b, acq2, 1, 4, timing=timing, log_amp=la)
+    assert rel(got, want) <= TOL
+    assert nxny == got.size
+
+
+def _plan_vs_oracle(w, precision, units):
+    import torch
+    plan, outs, keep = workloads.to_plan(w, precision)
+    plan.forward()
+    torch.cuda.synchronize()
+    errs = []
+    for u in units:
+        want = O.forward_workload_unit(w, u)
+        errs.append(rel(outs[u].cpu().numpy(), want))
+    plan.close()
+    return errs
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_SPLIT_BF16, abi.PREC_FP32])
+def test_batch_128_units_match_fp64_oracle(precision):
+    w = workloads.make("batch_128", n_slices=2, n_volumes=2)
+    errs = _plan_vs_oracle(w, precision, [0, 3])
+    print(f"batch_128 precision={precision} rel-L2 vs FP64: {errs}", flush=True)
+    assert max(errs) <= TOL
+
+
+def test_slice_128_matches_fp64_oracle():
+    w = workloads.make("slice_128")
+    for prec in (abi.PREC_SPLIT_BF16, abi.PREC_FP32):
+        errs = _plan_vs_oracle(w, prec, [0])
+        print(f"slice_128 precision={prec} rel-L2 vs FP64: {errs}", flush=True)
+        assert max(errs) <= TOL
+
+
+def test_pf_256_matches_fp64_oracle():
+    w = workloads.make("pf_256", n_slices=1)
+    for prec in (abi.PREC_SPLIT_BF16, abi.PREC_FP32):
+        errs = _plan_vs_oracle(w, prec, [0])
+        print(f"pf_256 precision={prec} rel-L2 vs FP64: {errs}", flush=True)
+        assert max(errs) <= TOL
+
+
+def test_complex_images_match_fp64_oracle():
+    w = workloads.make("grad_128")
+    errs = _plan_vs_oracle(w, abi.PREC_SPLIT_BF16, [0])
+    print(f"grad_128 rel-L2 vs FP64: {errs}", flush=True)
+    assert max(errs) <= TOL
+
+
+def test_graph_replay_tuning_and_determinism():
+    import torch
+    w = workloads.make("batch_128", n_slices=2, n_volumes=3)
+    plan, outs, keep = workloads.to_plan(w)
+    plan.forward()
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    for rep in range(3):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        plan.forward_graph()
+        torch.cuda.synchronize()
+        for o, r in zip(outs, ref):
+            assert torch.equal(o, r), f"graph replay {rep} differs"
+        _ = (ref[0].abs() * 2).sum().item()  # eager work between replays
+    report = plan.tune(trials=2)
+    print(report, flush=True)
+    plan.forward()
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref):
+        assert torch.equal(o, r), "tile choice changed the result"
+    plan.close()

@@ -231,3 +231,53 @@ def test_planted_mistake_is_caught_by_the_golden_record():
     got = O.simulate_slice(b, bad)
     want = _golden("default", f"slice-{nx}x{ny}o{o}-distortion+relaxation")
     assert np.any(got.view(np.uint32) != want.view(np.uint32))
+
+
+def _ring_maps(nc, snx, sny, nx, ny):
+    ox, oy = snx // nx, sny // ny
+    y, x = np.mgrid[0:sny, 0:snx].astype(float)
+    xa, ya = (x - (ox - 1) / 2) / ox, (y - (oy - 1) / 2) / oy
+    out = np.zeros((nc, snx * sny), np.complex64)
+    for c in range(nc):
+        v = np.vectorize(lambda a, b: O.lib().orc_coil_sensitivity(c, nc, a, b, nx, ny))(xa, ya)
+        out[c] = v.reshape(-1).astype(np.float32)
+    return out
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_direct_forward_matches_restatement(shape):
+    # The plan-input FP64 forward (closed-form phase, explicit coil maps) against the bit-exact
+    # restatement, every effect; the f32 coil maps bound the agreement at ~1e-7.
+    nx, ny, o = O.GOLDEN_SHAPES[shape]
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(shape)
+    for name, acq, ed, coil, nc in _literal_cases():
+        b = O.golden_slice_builder(shape, ed, False, slice_seed=0)
+        want = O.coil_kspace(b, acq, coil, nc).reshape(ny, nx)
+        maps = _ring_maps(nc, nx * o, ny * o, nx, ny)[coil:coil + 1]
+        got = O.forward_direct(nx * o, ny * o, nx, ny, 1, 3, acq, comps, fmap, maps, phase0=phase0,
+                               t2_uniform=[70.0, 100.0, 2000.0], eddy_drive=ed, z=3, nz=9)[0]
+        assert np.linalg.norm(got - want) <= 3e-7 * np.linalg.norm(want), name
+
+
+def test_direct_forward_maps_complex_images_and_coil_phase():
+    # T2/T2' maps, complex images: (a) an image m e^{i phi0} with phase0 = 0 equals the real image
+    # with phase0; (b) a coil map i*s gives i times the map s.
+    nx, ny, o = O.GOLDEN_SHAPES[1]
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(1)
+    acq = abi.Acquisition.default(ghost_offset=0.01)
+    maps = _ring_maps(1, nx, ny, nx, ny)
+    t2 = np.stack([t2m, t2m * 1.5, t2m * 0.7]).astype(np.float32)
+    ti = np.stack([tim, tim, tim * 2]).astype(np.float32)
+    real = O.forward_direct(nx, ny, nx, ny, 1, 3, acq, comps, fmap, maps, phase0=phase0, t2=t2, t_inhom=ti)
+    cimg = (comps * np.exp(1j * phase0)[None, :]).astype(np.complex64)
+    cplx = O.forward_direct(nx, ny, nx, ny, 1, 3, acq, cimg, fmap, maps, t2=t2, t_inhom=ti, complex_images=True)
+    assert np.linalg.norm(cplx - real) <= 1e-6 * np.linalg.norm(real)
+    rot = O.forward_direct(nx, ny, nx, ny, 1, 3, acq, comps, fmap, (1j * maps).astype(np.complex64),
+                           phase0=phase0, t2=t2, t_inhom=ti)
+    assert np.abs(rot - 1j * real).max() <= 1e-12 * np.abs(real).max()
+    # T2 map path agrees with the restatement's map path (one compartment)
+    b = abi.SliceInputBuilder([comps[0]], [t2m], fmap, (nx, ny), (nx, ny), t_inhom=[tim], phase0=phase0)
+    want = O.coil_kspace(b, acq).reshape(ny, nx)
+    got = O.forward_direct(nx, ny, nx, ny, 1, 1, acq, comps[:1], fmap, maps, phase0=phase0, t2=t2m[None],
+                           t_inhom=tim[None])[0]
+    assert np.linalg.norm(got - want) <= 3e-7 * np.linalg.norm(want)
