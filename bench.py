@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--check-units", type=int, default=2)
+    ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles (skips the autotuner)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the forward has no cross-GPU step, and multi-GPU runs are not "
@@ -96,8 +97,11 @@ def main():
     log(f"workload generated in {time.perf_counter() - t_gen:.1f} s: {len(w.units)} units")
     plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
     st = torch.cuda.current_stream()
-    tune_report = plan.tune(trials=2)
-    log(tune_report.strip())
+    if args.tiles:
+        xt, gp = (int(v) for v in args.tiles.split(","))
+        plan.set_tiles(xt, gp)
+    else:
+        log(plan.tune(trials=2).strip())
     nunits = len(w.units)
 
     # warm-up, then K graph replays bracketed by synchronisation
