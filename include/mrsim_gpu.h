@@ -234,9 +234,10 @@ int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tile
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);
 
-/* Device evaluation of sin/cos(2 pi cyc[i]) exactly as the kernels evaluate their phases (host
- * buffers); lets callers measure the phase arithmetic's error. */
-int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, char* err, size_t errlen);
+/* Device evaluation of sin/cos(2 pi cyc[i]) (host buffers), to measure the phase arithmetic's
+ * error: mode 0 = as the kernels evaluate it, 1 = FP64 quarter-turn reduction + FP32 polynomials,
+ * 2 = FP64 reduction + hardware v_sin_f32/v_cos_f32. */
+int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, int32_t mode, char* err, size_t errlen);
 
 /* Library version string. */
 const char* mrs_version(void);

@@ -34,7 +34,7 @@ struct DevSlice {
     float* nri;        // Q*P: -log2(e)/T2' (1/ms)
     uint32_t* s_hi;    // [x][c][y] packed bf16 (re | im<<16) of signal_scale*exp(log_amp)*S_c
     uint32_t* s_lo;    // residual of the same after s_hi, bf16
-    float2* s32;       // [c][y][x] the same in FP32 (plain path)
+    float2* s32;       // [c][x][y] the same in FP32 (plain path)
     // user inputs (device)
     const float* fmap;
     const double* phase0;

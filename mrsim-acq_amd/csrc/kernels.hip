@@ -66,36 +66,47 @@ __device__ __forceinline__ f32x4 mfma3(u32x4 ahi, u32x4 alo, u32x4 bhi, u32x4 bl
     return acc;
 }
 
-// sin and cos of 2*pi*cyc.  The argument is reduced in FP64 to the nearest quarter turn, and the
-// FP32 residual f in [-1/8, 1/8] turn is evaluated by Taylor polynomials of sin and cos about 0
-// (degree 9 and 10 in theta = 2 pi f; truncation below 2e-9 at |theta| = pi/4).
-__device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
-#if MRS_HW_SINCOS
-    double rq = cyc - __builtin_rint(cyc);
-    float r = (float)rq;
+// sin and cos of 2*pi*cyc, two evaluations of the FP64-reduced argument:
+//  poly: reduce in FP64 to the nearest quarter turn; the FP32 residual f in [-1/8, 1/8] turn is
+//        evaluated by Taylor polynomials of sin and cos about 0 (degree 9 and 10 in theta = 2 pi f,
+//        truncation below 2e-9 at |theta| = pi/4), then rotated by the quarter turns.
+//  hw:   reduce in FP64 to [-1/2, 1/2] turn and use v_sin_f32 / v_cos_f32, which take turns.
+__device__ __forceinline__ void sincos_turns_hw(double cyc, float& s, float& c) {
+    const float r = (float)(cyc - __builtin_rint(cyc));
     s = __builtin_amdgcn_sinf(r);
     c = __builtin_amdgcn_cosf(r);
-#else
-    double q4 = __builtin_rint(cyc * 4.0);
-    float f = (float)__builtin_fma(q4, -0.25, cyc);
-    int k = (int)q4 & 3;
-    float th = f * 6.2831853071795864769f;
-    float t2 = th * th;
+}
+
+__device__ __forceinline__ void sincos_turns_poly(double cyc, float& s, float& c) {
+    const double q4 = __builtin_rint(cyc * 4.0);
+    const float f = (float)__builtin_fma(q4, -0.25, cyc);
+    const int k = (int)q4 & 3;
+    const float th = f * 6.2831853071795864769f;
+    const float t2 = th * th;
     float ps = __builtin_fmaf(t2, 2.7557319223985893e-06f, -1.9841269841269841e-04f);
     ps = __builtin_fmaf(t2, ps, 8.3333333333333333e-03f);
     ps = __builtin_fmaf(t2, ps, -1.6666666666666667e-01f);
     ps = t2 * ps;
-    float sn = __builtin_fmaf(th, ps, th);
+    const float sn = __builtin_fmaf(th, ps, th);
     float pc = __builtin_fmaf(t2, -2.7557319223985891e-07f, 2.4801587301587302e-05f);
     pc = __builtin_fmaf(t2, pc, -1.3888888888888889e-03f);
     pc = __builtin_fmaf(t2, pc, 4.1666666666666667e-02f);
     pc = __builtin_fmaf(t2, pc, -0.5f);
-    float cs = __builtin_fmaf(t2, pc, 1.0f);
-    // rotate by k quarter turns
-    float s1 = (k & 1) ? cs : sn;
-    float c1 = (k & 1) ? sn : cs;
+    const float cs = __builtin_fmaf(t2, pc, 1.0f);
+    const float s1 = (k & 1) ? cs : sn;
+    const float c1 = (k & 1) ? sn : cs;
     s = (k & 2) ? -s1 : s1;
     c = ((k + 1) & 2) ? -c1 : c1;
+}
+
+#ifndef MRS_HW_SINCOS
+#define MRS_HW_SINCOS 0
+#endif
+__device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
+#if MRS_HW_SINCOS
+    sincos_turns_hw(cyc, s, c);
+#else
+    sincos_turns_poly(cyc, s, c);
 #endif
 }
 
@@ -165,7 +176,7 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
         const size_t so = ((size_t)x * C + c) * snyP + y;
         sl.s_hi[so] = hi;
         sl.s_lo[so] = lo;
-        sl.s32[((size_t)c * snyP + y) * snxP + x] = make_float2(re, im);
+        sl.s32[((size_t)c * snxP + x) * snyP + y] = make_float2(re, im);
     }
     (void)nx;
     (void)ny;
@@ -347,66 +358,85 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 }
 
 // ----------------------------------------------------------------------------- x-stage, MFMA
-// Block = 4 waves; each wave owns a tile of 16 output columns n = line*C + coil and up to 8
-// M-tiles (8 kx x re/im each) per pass, contracting over x in chunks of 16.
+// Block = (unit, block of 8*MT kx rows, share `part` of the column tiles).  The block stages its
+// twiddle rows (hi and lo, both ghost tables when the group has a ghost) in LDS once, then each
+// wave streams 16-column tiles n = line*C + coil of G through them, contracting over x in chunks
+// of 16 with three split-BF16 MFMAs per (M-tile, chunk).
+template <int MT>
 __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ units,
                                                    const DevGroup* __restrict__ groups,
-                                                   const WorkItem* __restrict__ work) {
+                                                   const WorkItem* __restrict__ work, int nparts) {
+    extern __shared__ uint32_t tl[];
     const WorkItem wi = work[blockIdx.x];
     const DevUnit& u = units[wi.unit];
     const DevGroup& gr = groups[u.group];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int lq = lane >> 4, ln = lane & 15;
     const int C = gr.C, LyP = gr.LyP;
-    const int ncol = LyP * C;
-    const int n0 = (wi.tile * 4 + wave) * 16;
-    if (n0 >= ncol) return;
-    const int n = n0 + ln;
-    const int lp = n / C, c = n % C;
-    const int tb = gr.lines[n0 / C].table;
-    const LineRec Lrow = gr.lines[lp];
     const int snx = gr.snx, snxP = (snx + 15) & ~15;
     const int nx = gr.nx, nxP = (nx + 7) & ~7;
-    const int mtiles = nxP >> 3;
-    const uint32_t* __restrict__ thi = gr.t_hi[tb];
-    const uint32_t* __restrict__ tlo = gr.t_lo[tb];
-    const int part = ln >> 3;
-    for (int m0 = 0; m0 < mtiles; m0 += 8) {
-        f32x4 acc[8];
+    const int stride = snxP + 4;             // u32 per LDS row (16-byte pad against bank conflicts)
+    constexpr int rows = 8 * MT;
+    const int kxb = wi.tile / nparts, part = wi.tile % nparts;
+    const int kx0 = kxb * rows;
+    const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
+    // stage [table][hi|lo][row][x]
+    for (int tb = 0; tb < ntab; tb++) {
+        for (int i = threadIdx.x; i < rows * (snxP / 4); i += 256) {
+            const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
+            const int kx = kx0 + r;
+            u32x4 h = u32x4{0u, 0u, 0u, 0u}, l = u32x4{0u, 0u, 0u, 0u};
+            if (kx < nxP) {
+                h = *reinterpret_cast<const u32x4*>(gr.t_hi[tb] + (size_t)kx * snxP + x4);
+                l = *reinterpret_cast<const u32x4*>(gr.t_lo[tb] + (size_t)kx * snxP + x4);
+            }
+            *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h;
+            *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l;
+        }
+    }
+    __syncthreads();
+    const int ntiles = (LyP * C) >> 4;
+    const int mrow = ln & 7, mpart = ln >> 3;
+    for (int nt = part * 4 + wave; nt < ntiles; nt += nparts * 4) {
+        const int n0 = nt * 16;
+        const int n = n0 + ln;
+        const int lp = n / C, c = n % C;
+        const int tb = ntab == 2 ? gr.lines[n0 / C].table : 0;
+        const uint32_t* __restrict__ th = tl + (tb * 2 + 0) * rows * stride;
+        const uint32_t* __restrict__ tlo = tl + (tb * 2 + 1) * rows * stride;
+        f32x4 acc[MT];
 #pragma unroll
-        for (int m = 0; m < 8; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < MT; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const float2* __restrict__ gcol = u.gbuf + (size_t)lp * C + c;
+        const size_t xstep = (size_t)LyP * C;
         for (int ch = 0; ch < snxP; ch += 16) {
             const int x0 = ch + lq * 4;
             u32x4 bhi, blo;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                float2 gv = u.gbuf[((size_t)(x0 + j) * LyP + lp) * C + c];
+                const float2 gv = gcol[(size_t)(x0 + j) * xstep];
                 uint32_t h, l;
                 split2(gv.x, gv.y, h, l);
                 bhi[j] = h;
                 blo[j] = l;
             }
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                if (m0 + m < mtiles) {
-                    const int kx = (m0 + m) * 8 + (ln & 7);
-                    const size_t to = (size_t)kx * snxP + x0;
-                    u32x4 h = *reinterpret_cast<const u32x4*>(thi + to);
-                    u32x4 l = *reinterpret_cast<const u32x4*>(tlo + to);
-                    acc[m] = mfma3(rows_of(h, part), rows_of(l, part), bhi, blo, acc[m]);
-                }
+            for (int m = 0; m < MT; m++) {
+                const int r = m * 8 + mrow;
+                const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
+                const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
+                acc[m] = mfma3(rows_of(h, mpart), rows_of(l, mpart), bhi, blo, acc[m]);
             }
         }
+        const LineRec Lrow = gr.lines[lp];
         if (Lrow.ky >= 0) {
             float* orow = reinterpret_cast<float*>(u.out + ((size_t)c * gr.ny + Lrow.ky) * nx);
 #pragma unroll
-            for (int m = 0; m < 8; m++) {
-                if (m0 + m < mtiles) {
+            for (int m = 0; m < MT; m++) {
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int kx = (m0 + m) * 8 + 4 * (lq & 1) + r;
-                        if (kx < nx) orow[2 * kx + (lq >> 1)] = acc[m][r];
-                    }
+                for (int r = 0; r < 4; r++) {
+                    const int kx = kx0 + m * 8 + 4 * (lq & 1) + r;
+                    if (kx < nx) orow[2 * kx + (lq >> 1)] = acc[m][r];
                 }
             }
         }
@@ -458,7 +488,7 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
 #pragma unroll
             for (int c = 0; c < CM; c++) {
                 if (c < C) {
-                    const float2 s = sl.s32[((size_t)c * snyP + y) * snxP + x];
+                    const float2 s = sl.s32[((size_t)c * snxP + x) * snyP + y];
                     acc[c].x = __builtin_fmaf(s.x, er, __builtin_fmaf(-s.y, ei, acc[c].x));
                     acc[c].y = __builtin_fmaf(s.x, ei, __builtin_fmaf(s.y, er, acc[c].y));
                 }
@@ -578,9 +608,13 @@ __global__ __launch_bounds__(256) void spikes_noise(const DevUnit* __restrict__ 
 }
 
 // Accuracy probe of the phase evaluation (used by the GPU tests): sin/cos of 2*pi*cyc.
-__global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__ s, float* __restrict__ c, int n) {
+__global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__ s, float* __restrict__ c, int n,
+                             int mode) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sincos_turns(cyc[i], s[i], c[i]);
+    if (i >= n) return;
+    if (mode == 0) sincos_turns(cyc[i], s[i], c[i]);
+    else if (mode == 1) sincos_turns_poly(cyc[i], s[i], c[i]);
+    else sincos_turns_hw(cyc[i], s[i], c[i]);
 }
 
 }  // namespace mrs
@@ -640,10 +674,25 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int n
                 : ylaunch_q<false, false>(Q, RB, GP, grid, st, u, s, g, w, xt);
 }
 
-hipError_t launch_xstage_mfma(int nwork, const DevUnit* u, const DevGroup* g, const WorkItem* w, hipStream_t st) {
+hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
+                              const WorkItem* w, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
-    hipLaunchKernelGGL(xstage_mfma, dim3(nwork), dim3(256), 0, st, u, g, w);
+    switch (mt) {
+    case 1: hipLaunchKernelGGL((xstage_mfma<1>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    case 2: hipLaunchKernelGGL((xstage_mfma<2>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    case 4: hipLaunchKernelGGL((xstage_mfma<4>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    case 8: hipLaunchKernelGGL((xstage_mfma<8>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
+}
+
+hipError_t set_xstage_lds_limit(size_t bytes) {
+    hipError_t e;
+    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
+    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
+    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
+    return hipFuncSetAttribute((const void*)xstage_mfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
@@ -674,8 +723,8 @@ hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u
     return hipGetLastError();
 }
 
-hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, hipStream_t st) {
-    hipLaunchKernelGGL(probe_sincos, dim3((n + 255) / 256), dim3(256), 0, st, cyc, s, c, n);
+hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, int mode, hipStream_t st) {
+    hipLaunchKernelGGL(probe_sincos, dim3((n + 255) / 256), dim3(256), 0, st, cyc, s, c, n, mode);
     return hipGetLastError();
 }
 

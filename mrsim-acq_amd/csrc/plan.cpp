@@ -18,13 +18,15 @@ namespace mrs {
 hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, hipStream_t st);
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st);
-hipError_t launch_xstage_mfma(int nwork, const DevUnit* u, const DevGroup* g, const WorkItem* w, hipStream_t st);
+hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
+                              const WorkItem* w, hipStream_t st);
+hipError_t set_xstage_lds_limit(size_t bytes);
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st);
 hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st);
 hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u, const DevGroup* g,
                                const int32_t* rows, hipStream_t st);
-hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, hipStream_t st);
+hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, int mode, hipStream_t st);
 }  // namespace mrs
 
 using namespace mrs;
@@ -211,8 +213,13 @@ struct mrs_plan {
         WorkItem* d_items = nullptr;
     };
     std::vector<YClass> yclasses;
-    std::vector<WorkItem> xitems;
-    WorkItem* d_xitems = nullptr;
+    struct XClass {
+        int mt, nparts;
+        size_t lds;
+        std::vector<WorkItem> items;
+        WorkItem* d_items = nullptr;
+    };
+    std::vector<XClass> xclasses;
     int xt = 8, gp = 4;
     int maxvox = 0;
     int maxC = 1;
@@ -251,6 +258,8 @@ static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w) {
     }
     return out;
 }
+
+static bool a_ghost(const GroupHost& g) { return g.desc.acq.ghost_offset != 0.0; }
 
 static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     const mrs_group_desc& d = g.desc;
@@ -370,7 +379,13 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     dg.t_echo = a.t_echo; dg.t_inhom = a.t_inhom; dg.signal_scale = a.signal_scale;
     dg.eddy_strength = a.eddy_strength; dg.eddy_quad = a.eddy_quad; dg.eddy_phase = a.eddy_phase;
     dg.lines = g.d_lines;
-    for (int tb = 0; tb < 2; tb++) { dg.t_hi[tb] = g.d_thi[tb]; dg.t_lo[tb] = g.d_tlo[tb]; dg.t32[tb] = g.d_t32[tb]; }
+    for (int tb = 0; tb < 2; tb++) {
+        // without a ghost both polarities read table 0, and the x-stage stages one table
+        const int src = a.ghost_offset != 0.0 ? tb : 0;
+        dg.t_hi[tb] = g.d_thi[src];
+        dg.t_lo[tb] = g.d_tlo[src];
+        dg.t32[tb] = g.d_t32[src];
+    }
     dg.zero_rows = g.d_zero;
     dg.n_zero_rows = (int)g.zero_rows.size();
     dg.noise = a.noise_variance > 0.0 ? 1 : 0;
@@ -549,16 +564,47 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         std::stable_sort(order.begin(), order.end(),
                          [&](size_t a, size_t b) { return units[a].slice < units[b].slice; });
         std::vector<std::vector<WorkItem>> yw(p->yclasses.size());
-        p->xitems.clear();
-        for (size_t k = 0; k < nunits; k++) {
+        std::map<std::tuple<int, int, size_t>, size_t> xcls;
+        // x-stage tiling per group: 8*MT kx rows of twiddles staged in LDS (<= 80 KiB so two blocks
+        // share a CU), the unit's column tiles split over `nparts` blocks
+        std::vector<int> g_mt(ngroups), g_np(ngroups);
+        std::vector<size_t> g_lds(ngroups);
+        std::vector<size_t> g_units(ngroups, 0);
+        for (size_t u = 0; u < nunits; u++) g_units[p->units[u].group]++;
+        for (size_t gi = 0; gi < ngroups; gi++) {
+            const DevGroup& dg = p->groups[gi].dev;
+            const int snxP = (dg.snx + 15) & ~15, nxP = (dg.nx + 7) & ~7;
+            const int ntab = a_ghost(p->groups[gi]) ? 2 : 1;
+            int mt = 1;
+            while (mt < nxP / 8 && mt < 8) mt *= 2;
+            auto lds_of = [&](int m) { return (size_t)ntab * 2 * 8 * m * (snxP + 4) * 4; };
+            while (mt > 1 && lds_of(mt) > 80 * 1024) mt /= 2;
+            const int nkxb = (nxP + 8 * mt - 1) / (8 * mt);
+            const int ntiles = dg.LyP * dg.C / 16;
+            const size_t want = 1024 / std::max<size_t>(1, g_units[gi] * nkxb) + 1;
+            const int np = (int)std::max<size_t>(1, std::min<size_t>(want, std::max(1, ntiles / 4)));
+            g_mt[gi] = mt;
+            g_np[gi] = np;
+            g_lds[gi] = lds_of(mt);
+        }
+        if (set_xstage_lds_limit(160 * 1024) != hipSuccess)
+            rc = fail(err, errlen, MRS_E_DEVICE, "cannot raise the x-stage LDS limit");
+        for (size_t k = 0; k < nunits && rc == MRS_OK; k++) {
             const size_t u = order[k];
-            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            const int gi = p->units[u].group;
+            const DevGroup& dg = p->groups[gi].dev;
             const size_t c = cls[std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0)];
             const int ntile = (dg.snx + p->xt - 1) / p->xt;
             for (int t = 0; t < ntile; t++) yw[c].push_back({(int32_t)u, t});
-            const int ncol = dg.LyP * dg.C;
-            const int xtiles = (ncol / 16 + 3) / 4;
-            for (int t = 0; t < xtiles; t++) p->xitems.push_back({(int32_t)u, t});
+            const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi]);
+            if (!xcls.count(xkey)) {
+                xcls[xkey] = p->xclasses.size();
+                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], {}, nullptr});
+            }
+            auto& xc = p->xclasses[xcls[xkey]];
+            const int nxP = (dg.nx + 7) & ~7;
+            const int nkxb = (nxP + 8 * g_mt[gi] - 1) / (8 * g_mt[gi]);
+            for (int t = 0; t < nkxb * g_np[gi]; t++) xc.items.push_back({(int32_t)u, t});
         }
         for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
             p->yclasses[c].items = xcd_order(yw[c]);
@@ -567,10 +613,11 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
                                  p->yclasses[c].items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
                 rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
         }
-        if (rc == MRS_OK) {
-            p->xitems = xcd_order(p->xitems);
-            rc = plan_alloc(p, &p->d_xitems, p->xitems.size(), err, errlen);
-            if (!rc && hipMemcpy(p->d_xitems, p->xitems.data(), p->xitems.size() * sizeof(WorkItem),
+        for (auto& xc : p->xclasses) {
+            if (rc) break;
+            xc.items = xcd_order(xc.items);
+            rc = plan_alloc(p, &xc.d_items, xc.items.size(), err, errlen);
+            if (!rc && hipMemcpy(xc.d_items, xc.items.data(), xc.items.size() * sizeof(WorkItem),
                                  hipMemcpyHostToDevice) != hipSuccess)
                 rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
         }
@@ -622,7 +669,9 @@ static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen
             HIP_TRY(launch_ystage_mfma(c.Q, c.RB, gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
                                        p->d_units, p->d_slices, p->d_groups, c.d_items, st));
         }
-        HIP_TRY(launch_xstage_mfma((int)p->xitems.size(), p->d_units, p->d_groups, p->d_xitems, st));
+        for (auto& xc : p->xclasses)
+            HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)xc.items.size(), p->d_units, p->d_groups,
+                                       xc.d_items, st));
     } else {
         for (size_t u = 0; u < p->units.size(); u++) {
             const DevGroup& dg = p->groups[p->units[u].group].dev;
@@ -691,7 +740,9 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
                 HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
                                            p->d_units, p->d_slices, p->d_groups, c.d_items, st));
             HIP_TRY(hipEventRecord(ev[2], st));
-            HIP_TRY(launch_xstage_mfma((int)p->xitems.size(), p->d_units, p->d_groups, p->d_xitems, st));
+            for (auto& xc : p->xclasses)
+                HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)xc.items.size(), p->d_units, p->d_groups,
+                                           xc.d_items, st));
         } else {
             for (size_t u = 0; u < p->units.size(); u++) {
                 const DevGroup& dg = p->groups[p->units[u].group].dev;
@@ -869,7 +920,8 @@ extern "C" size_t mrs_sizeof(const char* name) {
 }
 
 // Accuracy probe of the device phase evaluation: sin/cos(2 pi cyc[i]) for host arrays.
-extern "C" int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, char* err, size_t errlen) {
+extern "C" int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n, int32_t mode, char* err,
+                                size_t errlen) {
     double* dc = nullptr;
     float *ds = nullptr, *dcs = nullptr;
     if (dev_alloc(&dc, n, err, errlen) || dev_alloc(&ds, n, err, errlen) || dev_alloc(&dcs, n, err, errlen)) {
@@ -878,7 +930,7 @@ extern "C" int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n,
     }
     int rc = MRS_OK;
     if (hipMemcpy(dc, cyc, n * 8, hipMemcpyHostToDevice) != hipSuccess ||
-        launch_probe_sincos(dc, ds, dcs, (int)n, nullptr) != hipSuccess ||
+        launch_probe_sincos(dc, ds, dcs, (int)n, mode, nullptr) != hipSuccess ||
         hipMemcpy(s, ds, n * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(c, dcs, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
         rc = fail(err, errlen, MRS_E_DEVICE, "sincos probe failed");

@@ -67,7 +67,7 @@ def lib():
     L.mrs_version.restype = ct.c_char_p
     L.mrs_sizeof.argtypes = [ct.c_char_p]
     L.mrs_sizeof.restype = ct.c_size_t
-    L.mrs_probe_sincos.argtypes = [P(ct.c_double), P(ct.c_float), P(ct.c_float), ct.c_size_t] + E
+    L.mrs_probe_sincos.argtypes = [P(ct.c_double), P(ct.c_float), P(ct.c_float), ct.c_size_t, ct.c_int32] + E
     _lib = L
     return L
 
@@ -156,12 +156,12 @@ def build_coil_kspaces(b: abi.SliceInputBuilder, acq: abi.Acquisition, timing=No
     return out
 
 
-def probe_sincos(cyc: np.ndarray):
+def probe_sincos(cyc: np.ndarray, mode: int = 0):
     cyc = np.ascontiguousarray(cyc, dtype=np.float64)
     s = np.zeros(cyc.size, dtype=np.float32)
     c = np.zeros(cyc.size, dtype=np.float32)
     err = _errbuf()
-    rc = lib().mrs_probe_sincos(abi.dptr(cyc), abi.fptr(s), abi.fptr(c), cyc.size, err, len(err))
+    rc = lib().mrs_probe_sincos(abi.dptr(cyc), abi.fptr(s), abi.fptr(c), cyc.size, mode, err, len(err))
     _check(rc, err)
     return s, c
 

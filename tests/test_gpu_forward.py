@@ -34,11 +34,16 @@ def test_phase_evaluation_accuracy():
     rng = np.random.default_rng(1)
     cyc = np.concatenate([rng.uniform(-300, 300, 200000), np.linspace(-2, 2, 4097),
                           np.array([0.0, 0.25, 0.5, -0.25, 0.125, 1e-9])])
-    s, c = gpu.probe_sincos(cyc)
     r = cyc - np.rint(cyc)
+    for mode, name in [(1, "poly"), (2, "hardware")]:
+        s, c = gpu.probe_sincos(cyc, mode)
+        es = np.abs(s - np.sin(2 * np.pi * r)).max()
+        ec = np.abs(c - np.cos(2 * np.pi * r)).max()
+        rs = np.sqrt(np.mean((s - np.sin(2 * np.pi * r)) ** 2))
+        print(f"sincos {name}: max abs error sin {es:.2e} cos {ec:.2e}, rms {rs:.2e}", flush=True)
+    s, c = gpu.probe_sincos(cyc, 0)
     es = np.abs(s - np.sin(2 * np.pi * r)).max()
     ec = np.abs(c - np.cos(2 * np.pi * r)).max()
-    print(f"sincos max abs error: sin {es:.2e} cos {ec:.2e}", flush=True)
     assert es < 1e-6 and ec < 1e-6
 
 
