@@ -55,6 +55,7 @@ struct DevGroup {
     int32_t snx, sny, nx, ny;
     int32_t C, Q, RB;          // coils, compartments, 16-row blocks of 2C
     int32_t Ly;                // acquired lines
+    int32_t rotor_passes;      // leading 64-line passes whose lanes' 4 lines are arithmetic progressions
     int32_t LyP;               // padded lines (multiple of 16)
     int32_t cplx;              // complex images
     int32_t echo, do_relax, do_dist;

@@ -313,6 +313,67 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                     }
                 }
                 const double yy0 = (double)y0 - sys_yoff;
+                if constexpr (GP == 4 && !EDDY) {
+                    if ((g0 >> 2) < gr.rotor_passes) {
+                        // The lane's four lines form an arithmetic progression in (t, kyn, trf, tau):
+                        // evaluate line 0 directly, then advance the phase by the rotor
+                        // cis(2 pi (rate dt + dkyn yy)) and each decay by exp2(dtrf nr2 + dtau nri).
+                        const LineRec& L0 = lines_s[g0 * 16 + ln];
+                        const LineRec& L1 = lines_s[(g0 + 1) * 16 + ln];
+                        const double dt = L1.t_s - L0.t_s, dk = L1.kyn - L0.kyn;
+                        const float dtrf = L1.trf - L0.trf, dtau = L1.tau - L0.tau;
+                        u32x4 bh[4], bl[4];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            const double yy = yy0 + (double)j;
+                            double c0 = __builtin_fma(L0.kyn, yy, v.base[j]);
+                            c0 = __builtin_fma((double)v.rate[j], L0.t_s, c0);
+                            const double dc = __builtin_fma(dk, yy, (double)v.rate[j] * dt);
+                            float ps, pc, rs, rc;
+                            sincos_turns(c0, ps, pc);
+                            sincos_turns(dc, rs, rc);
+                            float dq[Q], rq[Q];
+#pragma unroll
+                            for (int q = 0; q < Q; q++) {
+                                dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(L0.trf, v.nr2[q][j], L0.tau * v.nri[q][j]));
+                                rq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(dtrf, v.nr2[q][j], dtau * v.nri[q][j]));
+                            }
+#pragma unroll
+                            for (int g = 0; g < 4; g++) {
+                                if (g) {
+                                    const float npc = pc * rc - ps * rs;
+                                    ps = __builtin_fmaf(pc, rs, ps * rc);
+                                    pc = npc;
+#pragma unroll
+                                    for (int q = 0; q < Q; q++) dq[q] *= rq[q];
+                                }
+                                float wr = 0.0f, wi = 0.0f;
+#pragma unroll
+                                for (int q = 0; q < Q; q++) {
+                                    wr = __builtin_fmaf(v.cr[q][j], dq[q], wr);
+                                    if constexpr (CPLX) wi = __builtin_fmaf(v.ci[q][j], dq[q], wi);
+                                }
+                                float er, ei;
+                                if constexpr (CPLX) {
+                                    er = wr * pc - wi * ps;
+                                    ei = wr * ps + wi * pc;
+                                } else {
+                                    er = wr * pc;
+                                    ei = wr * ps;
+                                }
+                                uint32_t h, l;
+                                split2(er, ei, h, l);
+                                bh[g][j] = h;
+                                bl[g][j] = l;
+                            }
+                        }
+#pragma unroll
+                        for (int g = 0; g < 4; g++)
+#pragma unroll
+                            for (int rb = 0; rb < RB; rb++) acc[rb][g] = mfma3(ahi[rb], alo[rb], bh[g], bl[g], acc[rb][g]);
+                        continue;
+                    }
+                }
 #pragma unroll
                 for (int g = 0; g < GP; g++) {
                     if (g0 + g < G) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -261,6 +262,54 @@ static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w) {
 
 static bool a_ghost(const GroupHost& g) { return g.desc.acq.ghost_offset != 0.0; }
 
+// Four lines form a chain when their (t, kyn, d, trf, tau) advance by equal steps, so the
+// y-stage can evaluate the first directly and advance the rest by per-voxel rotors.
+static bool is_chain(const LineRec* c) {
+    auto ap = [](double a, double b, double d, double e) {
+        const double s1 = b - a, s2 = d - b, s3 = e - d;
+        const double tol = 1e-9 * std::max({std::fabs(a), std::fabs(b), std::fabs(d), std::fabs(e), 1e-12});
+        return std::fabs(s2 - s1) <= tol && std::fabs(s3 - s1) <= tol;
+    };
+    return ap(c[0].t_s, c[1].t_s, c[2].t_s, c[3].t_s) && ap(c[0].kyn, c[1].kyn, c[2].kyn, c[3].kyn) &&
+           ap(c[0].d, c[1].d, c[2].d, c[3].d) && ap(c[0].trf, c[1].trf, c[2].trf, c[3].trf) &&
+           ap(c[0].tau, c[1].tau, c[2].tau, c[3].tau);
+}
+
+// Order one x-stage table segment: full passes of 16 chains first (line 16*(4p+e)+ln is element e
+// of chain 16p+ln), then the remaining lines by ky.  Chains come from runs of same-parity lines on
+// one side of the echo.  Returns the number of chain passes.
+static int order_segment(std::vector<LineRec>& seg) {
+    std::map<std::pair<int, int>, std::vector<LineRec>> cls;
+    for (const LineRec& L : seg) cls[std::make_pair(L.t_s >= 0.0 ? 0 : 1, L.ky & 1)].push_back(L);
+    std::vector<std::array<LineRec, 4>> chains;
+    std::vector<LineRec> rest;
+    for (auto& kv : cls) {
+        std::vector<LineRec>& v = kv.second;
+        size_t i = 0;
+        while (i < v.size()) {
+            size_t j = i + 1;
+            while (j < v.size() && v[j].ky == v[j - 1].ky + 2) j++;
+            size_t k = i;
+            for (; k + 4 <= j; k += 4) {
+                std::array<LineRec, 4> c{v[k], v[k + 1], v[k + 2], v[k + 3]};
+                if (is_chain(c.data())) chains.push_back(c);
+                else rest.insert(rest.end(), c.begin(), c.end());
+            }
+            for (; k < j; k++) rest.push_back(v[k]);
+            i = j;
+        }
+    }
+    const size_t npass = chains.size() / 16;
+    for (size_t c = npass * 16; c < chains.size(); c++) rest.insert(rest.end(), chains[c].begin(), chains[c].end());
+    std::sort(rest.begin(), rest.end(), [](const LineRec& a, const LineRec& b) { return a.ky < b.ky; });
+    seg.clear();
+    for (size_t pss = 0; pss < npass; pss++)
+        for (int e = 0; e < 4; e++)
+            for (int ln = 0; ln < 16; ln++) seg.push_back(chains[pss * 16 + ln][e]);
+    seg.insert(seg.end(), rest.begin(), rest.end());
+    return (int)npass;
+}
+
 static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     const mrs_group_desc& d = g.desc;
     const mrs_acquisition& a = d.acq;
@@ -311,7 +360,11 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
         seg[L.table].push_back(L);
     }
     g.lines.clear();
+    int rotor_passes = 0;
     for (int s = 0; s < 2; s++) {
+        // chain passes only where they start the table (the y-stage passes are 64-line aligned)
+        const int np = order_segment(seg[s]);
+        if (s == 0 || seg[0].empty()) rotor_passes = np;
         for (auto& L : seg[s]) g.lines.push_back(L);
         while (g.lines.size() % 16) {
             LineRec pad{};
@@ -371,6 +424,7 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     if (dg.RB > 4) dg.RB = 8;
     dg.Ly = Ly;
     dg.LyP = (int)g.lines.size();
+    dg.rotor_passes = rotor_passes;
     dg.cplx = d.complex_images ? 1 : 0;
     dg.echo = a.echo; dg.do_relax = a.do_relaxation ? 1 : 0; dg.do_dist = a.do_distortions ? 1 : 0;
     dg.sys_yoff = (double)(oy * ys) + yoff;
