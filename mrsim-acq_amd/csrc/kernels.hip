@@ -584,6 +584,167 @@ __global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ 
     u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = make_float2(ar, ai);
 }
 
+// ----------------------------------------------------------------------------- adjoint (VJP)
+// Cotangent g = dL/dK (PyTorch's conjugate-Wirtinger convention), per unit.
+// x-adjoint: Gb_c(x, l) = sum_kx conj(T_tab(l)[kx, x]) g_c[kx, ky_l]  (FP32), thread per (x, line, coil).
+__global__ __launch_bounds__(256) void xadjoint(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups,
+                                                const VjpUnit* __restrict__ vj, int unit0) {
+    const int unit = unit0 + blockIdx.y;
+    const DevUnit& u = units[unit];
+    const VjpUnit& v = vj[unit];
+    const DevGroup& gr = groups[u.group];
+    const int C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15, nx = gr.nx;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = t % C, lp = (t / C) % LyP, x = t / (C * LyP);
+    if (x >= snx) return;
+    const LineRec L = gr.lines[lp];
+    float2 acc = make_float2(0.f, 0.f);
+    if (L.ky >= 0 && v.gout) {
+        const float2* __restrict__ tw = gr.t32[L.table];
+        const float2* __restrict__ g = v.gout + ((size_t)c * gr.ny + L.ky) * nx;
+        for (int kx = 0; kx < nx; kx++) {
+            const float2 w = tw[(size_t)kx * snxP + x], gv = g[kx];
+            acc.x = __builtin_fmaf(w.x, gv.x, __builtin_fmaf(w.y, gv.y, acc.x));
+            acc.y = __builtin_fmaf(w.x, gv.y, __builtin_fmaf(-w.y, gv.x, acc.y));
+        }
+    }
+    v.gbar[((size_t)x * LyP + lp) * C + c] = acc;
+}
+
+// y-adjoint: one lane per voxel (x, y), looping over the lines.  With Eb = sum_c conj(S_c) Gb_c,
+// t_l = conj(P_l) Eb_l and D_q,l = exp(-trf/T2_q - tau/T2'_q):
+//   dL/dm_q   = sum_l D_q,l t_l                      (complex images; real images take the real part)
+//   dL/dfmap  = 2 pi Im( sum_q conj(m_q) sum_l tb0_l D_q,l t_l )
+//   dL/dT2'_q = Re( conj(m_q) sum_l tau_l D_q,l t_l ) / T2'_q^2
+// Lines of a chain pass (four-line arithmetic progressions) advance P and D by rotors.
+template <int Q, bool CPLX, int CM>
+__global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
+                                                const DevGroup* __restrict__ groups, const VjpUnit* __restrict__ vj,
+                                                int unit0) {
+    __shared__ LineRec lines_s[256];
+    const int unit = unit0 + blockIdx.y;
+    const DevUnit& u = units[unit];
+    const VjpUnit& vv = vj[unit];
+    const DevGroup& gr = groups[u.group];
+    const DevSlice& sl = slices[u.slice];
+    for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
+    __syncthreads();
+    const int snx = gr.snx, sny = gr.sny, snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
+    const int C = gr.C, LyP = gr.LyP;
+    const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
+    const int vox = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = vox % sny, x = vox / sny;
+    if (x >= snx) return;
+    const size_t o = (size_t)x * snyP + y, i = (size_t)x + (size_t)snx * y;
+    const double base = sl.base[o], rate = (double)sl.rate[o], poly = sl.poly[o];
+    const double yy = (double)y - gr.sys_yoff;
+    float nr2[Q], nri[Q], mr[Q], mi[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        nr2[q] = sl.nr2[q * PP + o];
+        nri[q] = sl.nri[q * PP + o];
+        if constexpr (CPLX) {
+            const float2 m = reinterpret_cast<const float2*>(u.images)[q * P + i];
+            mr[q] = m.x;
+            mi[q] = m.y;
+        } else {
+            mr[q] = reinterpret_cast<const float*>(u.images)[q * P + i];
+            mi[q] = 0.0f;
+        }
+    }
+    float2 s[CM];
+#pragma unroll
+    for (int c = 0; c < CM; c++) s[c] = c < C ? sl.s32[((size_t)c * snxP + x) * snyP + y] : make_float2(0.f, 0.f);
+    float ar[Q], ai[Q], br[Q], bi[Q], fr[Q], fi[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) ar[q] = ai[q] = br[q] = bi[q] = fr[q] = fi[q] = 0.0f;
+    const float2* __restrict__ gb = vv.gbar + (size_t)x * LyP * C;
+    const int G = LyP >> 4;
+    for (int g0 = 0; g0 < G; g0 += 4) {
+        const bool chain = (g0 >> 2) < gr.rotor_passes && g0 + 4 <= G;
+        for (int ln = 0; ln < 16; ln++) {
+            float pc = 0.f, ps = 0.f, rc = 1.f, rs = 0.f;
+            float dq[Q], rq[Q];
+            const int ng = chain ? 4 : (G - g0 < 4 ? G - g0 : 4);
+            for (int e = 0; e < ng; e++) {
+                const int lp = (g0 + e) * 16 + ln;
+                const LineRec& L = lines_s[lp];
+                if (L.ky < 0) continue;
+                if (!chain || e == 0) {
+                    double cyc = __builtin_fma(L.kyn, yy, base);
+                    cyc = __builtin_fma(rate, L.t_s, cyc);
+                    cyc = __builtin_fma(poly, L.d, cyc);
+                    sincos_turns(cyc, ps, pc);
+#pragma unroll
+                    for (int q = 0; q < Q; q++)
+                        dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, nr2[q], L.tau * nri[q]));
+                    if (chain) {
+                        const LineRec& L1 = lines_s[lp + 16];
+                        const double dc = __builtin_fma(L1.kyn - L.kyn, yy, rate * (L1.t_s - L.t_s));
+                        sincos_turns(dc, rs, rc);
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+                            rq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(L1.trf - L.trf, nr2[q], (L1.tau - L.tau) * nri[q]));
+                    }
+                } else {
+                    const float npc = pc * rc - ps * rs;
+                    ps = __builtin_fmaf(pc, rs, ps * rc);
+                    pc = npc;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) dq[q] *= rq[q];
+                }
+                // Eb = sum_c conj(S_c) Gb_c
+                const float2* __restrict__ gl = gb + (size_t)lp * C;
+                float er = 0.f, ei = 0.f;
+#pragma unroll
+                for (int c = 0; c < CM; c++) {
+                    if (c < C) {
+                        const float2 gv = gl[c];
+                        er = __builtin_fmaf(s[c].x, gv.x, __builtin_fmaf(s[c].y, gv.y, er));
+                        ei = __builtin_fmaf(s[c].x, gv.y, __builtin_fmaf(-s[c].y, gv.x, ei));
+                    }
+                }
+                // t = conj(P) Eb
+                const float tr = pc * er + ps * ei, ti = pc * ei - ps * er;
+                const float tb0 = L.tb0, tau = L.tau;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const float d = dq[q];
+                    ar[q] = __builtin_fmaf(d, tr, ar[q]);
+                    ai[q] = __builtin_fmaf(d, ti, ai[q]);
+                    const float dt = d * tau, df = d * tb0;
+                    br[q] = __builtin_fmaf(dt, tr, br[q]);
+                    bi[q] = __builtin_fmaf(dt, ti, bi[q]);
+                    fr[q] = __builtin_fmaf(df, tr, fr[q]);
+                    fi[q] = __builtin_fmaf(df, ti, fi[q]);
+                }
+            }
+        }
+    }
+    if (vv.gimg) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if constexpr (CPLX) reinterpret_cast<float2*>(vv.gimg)[q * P + i] = make_float2(ar[q], ai[q]);
+            else reinterpret_cast<float*>(vv.gimg)[q * P + i] = ar[q];
+        }
+    }
+    if (vv.gfmap) {
+        float acc = 0.f;
+        // Im(conj(m) f) = mr fi - mi fr
+#pragma unroll
+        for (int q = 0; q < Q; q++) acc += mr[q] * fi[q] - mi[q] * fr[q];
+        vv.gfmap[i] = gr.do_dist ? (float)(kTau * (double)acc) : 0.0f;
+    }
+    if (vv.gti) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            float ti_ms = sl.t_inhom ? sl.t_inhom[q * P + i] : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
+            const float re = mr[q] * br[q] + mi[q] * bi[q];  // Re(conj(m) b)
+            vv.gti[q * P + i] = gr.do_relax ? re / (ti_ms * ti_ms) : 0.0f;
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------- epilogues
 // Unacquired rows are exactly zero (src/kspace.rs:839-843).  grid.y = unit.
 __global__ void zero_rows(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups, int unit0) {
@@ -782,6 +943,41 @@ hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u
     if (nunits <= 0) return hipSuccess;
     hipLaunchKernelGGL(spikes_noise, dim3(maxC, nunits), dim3(256), 0, st, u, g, unit0, rows);
     return hipGetLastError();
+}
+
+
+template <int Q, bool CPLX>
+static hipError_t yadj_c(int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                         const VjpUnit* v, int unit0) {
+    if (C <= 8) hipLaunchKernelGGL((yadjoint<Q, CPLX, 8>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    else if (C <= 16) hipLaunchKernelGGL((yadjoint<Q, CPLX, 16>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    else if (C <= 32) hipLaunchKernelGGL((yadjoint<Q, CPLX, 32>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    else hipLaunchKernelGGL((yadjoint<Q, CPLX, 64>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+static hipError_t yadj_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                         const DevGroup* g, const VjpUnit* v, int unit0) {
+    switch (Q) {
+    case 1: return yadj_c<1, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 2: return yadj_c<2, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 3: return yadj_c<3, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 4: return yadj_c<4, CPLX>(C, grid, st, u, s, g, v, unit0);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Adjoint of units [unit0, unit0 + n) sharing one group shape.
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, const DevUnit* u,
+                            const DevSlice* s, const DevGroup* g, const VjpUnit* v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    const int tx = snx * LyP * C;
+    hipLaunchKernelGGL(xadjoint, dim3((tx + 255) / 256, n), dim3(256), 0, st, u, g, v, unit0);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 grid((snx * sny + 255) / 256, n);
+    return cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0) : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0);
 }
 
 hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, int mode, hipStream_t st) {

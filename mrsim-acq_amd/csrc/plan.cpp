@@ -22,6 +22,8 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int n
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
                               const WorkItem* w, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, const DevUnit* u,
+                            const DevSlice* s, const DevGroup* g, const VjpUnit* v, hipStream_t st);
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st);
 hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st);
@@ -770,9 +772,43 @@ extern "C" int mrs_plan_forward_graph(mrs_plan* p, void* stream, char* err, size
 extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images,
                             float* const* grad_fmap, float* const* grad_t_inhom, void* stream, char* err,
                             size_t errlen) {
-    (void)grad_out; (void)grad_images; (void)grad_fmap; (void)grad_t_inhom; (void)stream;
     if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
-    return fail(err, errlen, MRS_E_UNSUPPORTED, "adjoint not built yet");
+    if (!grad_out) return fail(err, errlen, MRS_E_INVALID, "grad_out is required");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = p->units.size();
+    if (p->vjp_units.empty()) {
+        p->vjp_units.resize(n);
+        for (size_t u = 0; u < n; u++) {
+            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            const size_t snxP = (dg.snx + 15) & ~15;
+            int rc = plan_alloc(p, &p->vjp_units[u].gbar, snxP * dg.LyP * dg.C, err, errlen);
+            if (rc) { p->vjp_units.clear(); return rc; }
+        }
+        int rc = plan_alloc(p, &p->d_vjp_units, n, err, errlen);
+        if (rc) { p->vjp_units.clear(); return rc; }
+    }
+    for (size_t u = 0; u < n; u++) {
+        VjpUnit& v = p->vjp_units[u];
+        v.gout = reinterpret_cast<const float2*>(grad_out[u]);
+        v.gimg = grad_images ? reinterpret_cast<float2*>(grad_images[u]) : nullptr;
+        v.gfmap = grad_fmap ? grad_fmap[u] : nullptr;
+        v.gti = grad_t_inhom ? grad_t_inhom[u] : nullptr;
+        if (!v.gout) return fail(err, errlen, MRS_E_INVALID, "unit %zu has no cotangent", u);
+    }
+    HIP_TRY(hipMemcpyAsync(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice, st));
+    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+    size_t u0 = 0;
+    while (u0 < n) {
+        size_t u1 = u0 + 1;
+        while (u1 < n && p->units[u1].group == p->units[u0].group) u1++;
+        const DevGroup& dg = p->groups[p->units[u0].group].dev;
+        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.snx, dg.sny, dg.LyP,
+                                 p->d_units, p->d_slices, p->d_groups, p->d_vjp_units, st));
+        u0 = u1;
+    }
+    // the descriptor upload reads host memory asynchronously: wait before the vector can change
+    HIP_TRY(hipStreamSynchronize(st));
+    return MRS_OK;
 }
 
 // Average duration of each forward stage over `iters` launches, timed with HIP events recorded on
