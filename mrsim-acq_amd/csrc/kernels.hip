@@ -99,8 +99,10 @@ __device__ __forceinline__ void sincos_turns_poly(double cyc, float& s, float& c
     c = ((k + 1) & 2) ? -c1 : c1;
 }
 
+// v_sin/v_cos on the FP64-reduced argument: max abs error 1.7e-7 measured on MI355X against FP64
+// over 2e5 arguments in [-300, 300] turns (the polynomial: 9.4e-8), far inside the 1e-5 budget.
 #ifndef MRS_HW_SINCOS
-#define MRS_HW_SINCOS 0
+#define MRS_HW_SINCOS 1
 #endif
 __device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
 #if MRS_HW_SINCOS

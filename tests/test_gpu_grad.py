@@ -64,8 +64,9 @@ def test_adjoint_identity(setup):
     k, gi, gf, gt = _grads(model, img, fmap, ti, g)
     lhs = np.vdot(g.cpu().numpy(), k.cpu().numpy())          # <g, A m>
     rhs = np.vdot(gi.cpu().numpy(), img.cpu().numpy())       # <A^H g, m>
-    print(f"adjoint identity: {lhs:.6e} vs {rhs:.6e}", flush=True)
-    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    scale = np.linalg.norm(g.cpu().numpy()) * np.linalg.norm(k.cpu().numpy())
+    print(f"adjoint identity: {lhs:.6e} vs {rhs:.6e} (|diff| / (|g||Am|) = {abs(lhs - rhs) / scale:.2e})", flush=True)
+    assert abs(lhs - rhs) <= 1e-5 * scale
 
 
 def test_gradients_match_fp64_central_differences(setup):
@@ -102,13 +103,19 @@ def test_gradients_match_fp64_central_differences(setup):
 
 
 def test_gradcheck_style_directional_derivatives(setup):
+    # The differences are taken through the plain FP32 forward (rel-L2 ~3e-7), whose round-off over a
+    # 1e-3 relative step stays well below the tolerance.
     import torch
+    from mrsim_amd.autograd import KspaceModel
     w, model, img, fmap, ti, g = setup
     k, gi, gf, gt = _grads(model, img, fmap, ti, g)
     gen = torch.Generator(device="cpu").manual_seed(3)
+    sd = w.slices[0]
+    fp32 = KspaceModel(w.snx, w.sny, w.nx, w.ny, torch.from_numpy(sd.coil_maps).to("cuda"), w.acq, w.n_comp,
+                       t2=torch.from_numpy(sd.t2).to("cuda"), complex_images=True, precision=abi.PREC_FP32)
 
     def loss(a, b, c):
-        return float(torch.real(torch.vdot(g.reshape(-1), model.forward_raw(a, b, c).reshape(-1))).item())
+        return float(torch.real(torch.vdot(g.reshape(-1), fp32.forward_raw(a, b, c).reshape(-1))).item())
 
     # fieldmap direction, step 1e-3 relative to the map's scale
     v = torch.randn(fmap.shape, generator=gen).to(fmap.device)
