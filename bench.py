@@ -71,6 +71,56 @@ def _cpu_unit(i):
     return i
 
 
+def grad_pass(iters: int):
+    """One slice, 128^2 -> 64^2, 8 coils, 3 compartments, complex64 images: forward time, forward +
+    adjoint (gradients for images, fieldmap and T2') time, and a timed 50-iteration Adam fit of the
+    fieldmap to the k-space of the true one."""
+    import torch
+
+    from mrsim_amd import workloads
+    from mrsim_amd.autograd import KspaceModel
+    w = workloads.make("grad_128")
+    sd = w.slices[0]
+    dev = "cuda"
+    model = KspaceModel(w.snx, w.sny, w.nx, w.ny, torch.from_numpy(sd.coil_maps).to(dev), w.acq, w.n_comp,
+                        t2=torch.from_numpy(sd.t2).to(dev), complex_images=True)
+    img = torch.from_numpy(w.units[0][1]).to(dev)
+    fmap = torch.from_numpy(sd.fmap).to(dev)
+    ti = torch.from_numpy(sd.t_inhom).to(dev)
+    g = torch.randn((w.n_coils, w.ny, w.nx), dtype=torch.complex64, device=dev)
+    for _ in range(3):
+        model.forward_raw(img, fmap, ti)
+        model.vjp_raw(img, fmap, ti, g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        model.forward_raw(img, fmap, ti)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / iters
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        model.forward_raw(img, fmap, ti)
+        model.vjp_raw(img, fmap, ti, g)
+    torch.cuda.synchronize()
+    t_both = (time.perf_counter() - t0) / iters
+    target = model.forward_raw(img, fmap, ti).detach()
+    f = (fmap + 5.0).clone().requires_grad_(True)
+    opt = torch.optim.Adam([f], lr=0.5)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        opt.zero_grad()
+        loss = (model(img, f, ti) - target).abs().pow(2).sum()
+        loss.backward()
+        opt.step()
+    l_end = float(loss)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    return {"workload": "grad_128: 1 slice 128^2->64^2, 8 coils, 3 compartments, complex64",
+            "forward_ms": round(1e3 * t_fwd, 4), "forward_adjoint_ms": round(1e3 * t_both, 4),
+            "ratio": round(t_both / t_fwd, 3), "fit50_ms": round(1e3 * t_fit, 2), "fit50_final_loss": l_end}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +130,7 @@ def main():
     ap.add_argument("--volumes", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--check-units", type=int, default=2)
     ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles (skips the autotuner)")
     args = ap.parse_args()
@@ -148,6 +199,10 @@ def main():
         log(f"plain fp32 per-slice: {1e3 * pdt / ksteps:.3f} ms/step, {plain:.0f} slices/s")
         pplan.close()
 
+    grad = None if args.no_grad else grad_pass(max(5, args.steps))
+    if grad:
+        log(f"gradient pass: {grad}")
+
     stats = plan.stats()
     terms = stats["terms"]
     lines = stats["lines"]
@@ -199,6 +254,7 @@ def main():
         "rel_l2_vs_fp64": errs,
         "stage_ms": stage_ms,
         "tiles": plan.tiles(),
+        "grad_pass": grad,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

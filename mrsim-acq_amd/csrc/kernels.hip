@@ -22,6 +22,17 @@ namespace mrs {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) double f64x2;
+
+// Pointers read from device descriptors are generic; casting them to the global address space
+// lets the compiler emit global loads with a scalar base instead of flat loads.
+#define GAS __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GAS T* gld(const T* p) { return (const GAS T*)p; }
+template <class T>
+__device__ __forceinline__ GAS T* gst(T* p) { return (GAS T*)p; }
 
 static constexpr double kTau = 6.283185307179586476925286766559;
 static constexpr float kLog2e = 1.4426950408889634f;
@@ -34,12 +45,13 @@ __device__ __forceinline__ uint32_t bf16_bits(float v) {
 }
 __device__ __forceinline__ float bf16_float(uint32_t b) { return __builtin_bit_cast(float, b << 16); }
 
-// (a, b) -> packed hi = bf16(a) | bf16(b) << 16, lo = bf16 of the residuals.
+// (a, b) -> packed hi = bf16(a) | bf16(b) << 16 (one v_cvt_pk_bf16_f32, round to nearest even),
+// lo = the same of the residuals a - hi_a, b - hi_b.
 __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
-    uint32_t ha = bf16_bits(a), hb = bf16_bits(b);
-    float ra = a - bf16_float(ha), rb = b - bf16_float(hb);
-    hi = ha | (hb << 16);
-    lo = bf16_bits(ra) | (bf16_bits(rb) << 16);
+    const uint32_t h = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+    const float ha = __builtin_bit_cast(float, h << 16), hb = __builtin_bit_cast(float, h & 0xffff0000u);
+    hi = h;
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a - ha, b - hb}, bf16x2));
 }
 
 // A-operand rows of a complex contraction out = sum S * v, split into (re row, im row):
@@ -200,20 +212,20 @@ template <int Q, bool CPLX, bool EDDY>
 __device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice& sl, const void* images, int snx,
                                           int sny, int x, int y0, int snyP, size_t PP) {
     const size_t o = (size_t)x * snyP + y0;
-    const double2* b2 = reinterpret_cast<const double2*>(sl.base + o);
-    double2 b01 = b2[0], b23 = b2[1];
+    const GAS f64x2* b2 = reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o);
+    const f64x2 b01 = b2[0], b23 = b2[1];
     v.base[0] = b01.x; v.base[1] = b01.y; v.base[2] = b23.x; v.base[3] = b23.y;
     if constexpr (EDDY) {
-        const double2* p2 = reinterpret_cast<const double2*>(sl.poly + o);
-        double2 p01 = p2[0], p23 = p2[1];
+        const GAS f64x2* p2 = reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o);
+        const f64x2 p01 = p2[0], p23 = p2[1];
         v.poly[0] = p01.x; v.poly[1] = p01.y; v.poly[2] = p23.x; v.poly[3] = p23.y;
     }
-    float4 r4 = *reinterpret_cast<const float4*>(sl.rate + o);
+    const f32x4 r4 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rate) + o);
     v.rate[0] = r4.x; v.rate[1] = r4.y; v.rate[2] = r4.z; v.rate[3] = r4.w;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        float4 a = *reinterpret_cast<const float4*>(sl.nr2 + q * PP + o);
-        float4 b = *reinterpret_cast<const float4*>(sl.nri + q * PP + o);
+        const f32x4 a = *reinterpret_cast<const GAS f32x4*>(gld(sl.nr2) + q * PP + o);
+        const f32x4 b = *reinterpret_cast<const GAS f32x4*>(gld(sl.nri) + q * PP + o);
         v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
         v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
     }
@@ -226,11 +238,12 @@ __device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             if constexpr (CPLX) {
-                float2 m = ok ? reinterpret_cast<const float2*>(images)[q * P + i] : make_float2(0.f, 0.f);
+                f32x2 m = {0.f, 0.f};
+                if (ok) m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images)) + 2 * (q * P + i));
                 v.cr[q][j] = m.x;
                 v.ci[q][j] = m.y;
             } else {
-                v.cr[q][j] = ok ? reinterpret_cast<const float*>(images)[q * P + i] : 0.0f;
+                v.cr[q][j] = ok ? gld(reinterpret_cast<const float*>(images))[q * P + i] : 0.0f;
             }
         }
     }
@@ -305,8 +318,8 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                     const int c = rb * 8 + (ln & 7);
                     if (c < C) {
                         const size_t so = ((size_t)x * C + c) * snyP + y0;
-                        u32x4 h = *reinterpret_cast<const u32x4*>(sl.s_hi + so);
-                        u32x4 l = *reinterpret_cast<const u32x4*>(sl.s_lo + so);
+                        u32x4 h = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_hi) + so);
+                        u32x4 l = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_lo) + so);
                         ahi[rb] = rows_of(h, part);
                         alo[rb] = rows_of(l, part);
                     } else {
@@ -409,7 +422,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         for (int r = 0; r < 4; r++) {
                             const int c = rb * 8 + 4 * (lq & 1) + r;
                             if (c < C) {
-                                float* dst = reinterpret_cast<float*>(u.gbuf + ((size_t)x * LyP + lp) * C + c);
+                                GAS float* dst = reinterpret_cast<GAS float*>(gst(u.gbuf) + ((size_t)x * LyP + lp) * C + c);
                                 dst[lq >> 1] = acc[rb][g][r];
                             }
                         }
@@ -450,8 +463,8 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
             const int kx = kx0 + r;
             u32x4 h = u32x4{0u, 0u, 0u, 0u}, l = u32x4{0u, 0u, 0u, 0u};
             if (kx < nxP) {
-                h = *reinterpret_cast<const u32x4*>(gr.t_hi[tb] + (size_t)kx * snxP + x4);
-                l = *reinterpret_cast<const u32x4*>(gr.t_lo[tb] + (size_t)kx * snxP + x4);
+                h = *reinterpret_cast<const GAS u32x4*>(gld(gr.t_hi[tb]) + (size_t)kx * snxP + x4);
+                l = *reinterpret_cast<const GAS u32x4*>(gld(gr.t_lo[tb]) + (size_t)kx * snxP + x4);
             }
             *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h;
             *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l;
@@ -470,14 +483,14 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
         f32x4 acc[MT];
 #pragma unroll
         for (int m = 0; m < MT; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const float2* __restrict__ gcol = u.gbuf + (size_t)lp * C + c;
+        const GAS f32x2* __restrict__ gcol = reinterpret_cast<const GAS f32x2*>(gld(u.gbuf) + (size_t)lp * C + c);
         const size_t xstep = (size_t)LyP * C;
         for (int ch = 0; ch < snxP; ch += 16) {
             const int x0 = ch + lq * 4;
             u32x4 bhi, blo;
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const float2 gv = gcol[(size_t)(x0 + j) * xstep];
+                const f32x2 gv = gcol[(size_t)(x0 + j) * xstep];
                 uint32_t h, l;
                 split2(gv.x, gv.y, h, l);
                 bhi[j] = h;
@@ -493,7 +506,7 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
         }
         const LineRec Lrow = gr.lines[lp];
         if (Lrow.ky >= 0) {
-            float* orow = reinterpret_cast<float*>(u.out + ((size_t)c * gr.ny + Lrow.ky) * nx);
+            GAS float* orow = reinterpret_cast<GAS float*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
 #pragma unroll
             for (int m = 0; m < MT; m++) {
 #pragma unroll
