@@ -434,10 +434,11 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 }
 
 // ----------------------------------------------------------------------------- x-stage, MFMA
-// Block = (unit, block of 8*MT kx rows, share `part` of the column tiles).  The block stages its
-// twiddle rows (hi and lo, both ghost tables when the group has a ghost) in LDS once, then each
-// wave streams 16-column tiles n = line*C + coil of G through them, contracting over x in chunks
-// of 16 with three split-BF16 MFMAs per (M-tile, chunk).
+// Block = (unit, block of 16*MT kx rows, share `part` of the column tiles).  The block stages its
+// packed twiddle rows (hi and lo; both ghost tables when the group has a ghost) in LDS once; each
+// wave then streams 16-column tiles n = line*C + coil of G through them.  The complex product is
+// carried by the B operand: for each 16-row kx tile, one MFMA chain against B_re = [G_re, -G_im]
+// gives Re K and one against B_im = [G_im, G_re] gives Im K, so the twiddles enter as stored.
 template <int MT>
 __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ units,
                                                    const DevGroup* __restrict__ groups,
@@ -452,11 +453,10 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
     const int snx = gr.snx, snxP = (snx + 15) & ~15;
     const int nx = gr.nx, nxP = (nx + 7) & ~7;
     const int stride = snxP + 4;             // u32 per LDS row (16-byte pad against bank conflicts)
-    constexpr int rows = 8 * MT;
+    constexpr int rows = 16 * MT;
     const int kxb = wi.tile / nparts, part = wi.tile % nparts;
     const int kx0 = kxb * rows;
     const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
-    // stage [table][hi|lo][row][x]
     for (int tb = 0; tb < ntab; tb++) {
         for (int i = threadIdx.x; i < rows * (snxP / 4); i += 256) {
             const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
@@ -472,7 +472,6 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
     }
     __syncthreads();
     const int ntiles = (LyP * C) >> 4;
-    const int mrow = ln & 7, mpart = ln >> 3;
     for (int nt = part * 4 + wave; nt < ntiles; nt += nparts * 4) {
         const int n0 = nt * 16;
         const int n = n0 + ln;
@@ -480,9 +479,9 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
         const int tb = ntab == 2 ? gr.lines[n0 / C].table : 0;
         const uint32_t* __restrict__ th = tl + (tb * 2 + 0) * rows * stride;
         const uint32_t* __restrict__ tlo = tl + (tb * 2 + 1) * rows * stride;
-        f32x4 acc[MT];
+        f32x4 accr[MT], acci[MT];
 #pragma unroll
-        for (int m = 0; m < MT; m++) acc[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
         const GAS f32x2* __restrict__ gcol = reinterpret_cast<const GAS f32x2*>(gld(u.gbuf) + (size_t)lp * C + c);
         const size_t xstep = (size_t)LyP * C;
         for (int ch = 0; ch < snxP; ch += 16) {
@@ -496,23 +495,27 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
                 bhi[j] = h;
                 blo[j] = l;
             }
+            // B_re = [G_re, -G_im] (negate the high halves), B_im = [G_im, G_re] (swap halves)
+            const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+            const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
 #pragma unroll
             for (int m = 0; m < MT; m++) {
-                const int r = m * 8 + mrow;
+                const int r = m * 16 + ln;
                 const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
                 const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
-                acc[m] = mfma3(rows_of(h, mpart), rows_of(l, mpart), bhi, blo, acc[m]);
+                accr[m] = mfma3(h, l, brh, brl, accr[m]);
+                acci[m] = mfma3(h, l, bih, bil, acci[m]);
             }
         }
         const LineRec Lrow = gr.lines[lp];
         if (Lrow.ky >= 0) {
-            GAS float* orow = reinterpret_cast<GAS float*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
+            GAS f32x2* orow = reinterpret_cast<GAS f32x2*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
 #pragma unroll
             for (int m = 0; m < MT; m++) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    const int kx = kx0 + m * 8 + 4 * (lq & 1) + r;
-                    if (kx < nx) orow[2 * kx + (lq >> 1)] = acc[m][r];
+                    const int kx = kx0 + m * 16 + 4 * lq + r;
+                    if (kx < nx) orow[kx] = f32x2{accr[m][r], acci[m][r]};
                 }
             }
         }

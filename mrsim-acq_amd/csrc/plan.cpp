@@ -234,6 +234,7 @@ struct mrs_plan {
     // adjoint scratch
     std::vector<VjpUnit> vjp_units;
     VjpUnit* d_vjp_units = nullptr;
+    bool vjp_fresh = true;
 };
 
 static void free_plan_memory(mrs_plan* p) {
@@ -496,20 +497,27 @@ extern "C" int mrs_plan_set_pointers(mrs_plan* p, const mrs_slice_desc* slices, 
     if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
     int rc = check_pointers(p, slices, units, err, errlen);
     if (rc) return rc;
+    bool changed = false;
     for (size_t s = 0; s < p->slices.size(); s++) {
         if (slices[s].group != p->slices[s].group)
             return fail(err, errlen, MRS_E_INVALID, "slice %zu changed group", s);
+        DevSlice before = p->slices[s];
         fill_slice_inputs(p->slices[s], slices[s]);
+        changed |= std::memcmp(&before, &p->slices[s], sizeof(DevSlice)) != 0;
         p->slice_desc[s] = slices[s];
     }
     for (size_t u = 0; u < p->units.size(); u++) {
         if (units[u].slice != p->units[u].slice) return fail(err, errlen, MRS_E_INVALID, "unit %zu changed slice", u);
-        p->units[u].images = units[u].images;
-        p->units[u].out = reinterpret_cast<float2*>(units[u].out);
-        p->units[u].seed = units[u].slice_seed;
-        p->units[u].coil0 = units[u].coil0;
+        DevUnit& d = p->units[u];
+        const DevUnit before = d;
+        d.images = units[u].images;
+        d.out = reinterpret_cast<float2*>(units[u].out);
+        d.seed = units[u].slice_seed;
+        d.coil0 = units[u].coil0;
+        changed |= std::memcmp(&before, &d, sizeof(DevUnit)) != 0;
         p->unit_desc[u] = units[u];
     }
+    if (!changed) return MRS_OK;  // descriptors and any recorded graph stay valid
     if (p->graph_exec) {
         hipGraphExecDestroy(p->graph_exec);
         hipGraphDestroy(p->graph);
@@ -631,11 +639,12 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const DevGroup& dg = p->groups[gi].dev;
             const int snxP = (dg.snx + 15) & ~15, nxP = (dg.nx + 7) & ~7;
             const int ntab = a_ghost(p->groups[gi]) ? 2 : 1;
+            // kx tiles of 16 rows; MT tiles per block
             int mt = 1;
-            while (mt < nxP / 8 && mt < 8) mt *= 2;
-            auto lds_of = [&](int m) { return (size_t)ntab * 2 * 8 * m * (snxP + 4) * 4; };
+            while (mt * 16 < nxP && mt < 4) mt *= 2;
+            auto lds_of = [&](int m) { return (size_t)ntab * 2 * 16 * m * (snxP + 4) * 4; };
             while (mt > 1 && lds_of(mt) > 80 * 1024) mt /= 2;
-            const int nkxb = (nxP + 8 * mt - 1) / (8 * mt);
+            const int nkxb = (nxP + 16 * mt - 1) / (16 * mt);
             const int ntiles = dg.LyP * dg.C / 16;
             const size_t want = 1024 / std::max<size_t>(1, g_units[gi] * nkxb) + 1;
             const int np = (int)std::max<size_t>(1, std::min<size_t>(want, std::max(1, ntiles / 4)));
@@ -659,7 +668,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             }
             auto& xc = p->xclasses[xcls[xkey]];
             const int nxP = (dg.nx + 7) & ~7;
-            const int nkxb = (nxP + 8 * g_mt[gi] - 1) / (8 * g_mt[gi]);
+            const int nkxb = (nxP + 16 * g_mt[gi] - 1) / (16 * g_mt[gi]);
             for (int t = 0; t < nkxb * g_np[gi]; t++) xc.items.push_back({(int32_t)u, t});
         }
         for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
@@ -786,16 +795,23 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
         }
         int rc = plan_alloc(p, &p->d_vjp_units, n, err, errlen);
         if (rc) { p->vjp_units.clear(); return rc; }
+        p->vjp_fresh = true;
     }
+    bool changed = p->vjp_fresh;
     for (size_t u = 0; u < n; u++) {
         VjpUnit& v = p->vjp_units[u];
+        const VjpUnit before = v;
         v.gout = reinterpret_cast<const float2*>(grad_out[u]);
         v.gimg = grad_images ? reinterpret_cast<float2*>(grad_images[u]) : nullptr;
         v.gfmap = grad_fmap ? grad_fmap[u] : nullptr;
         v.gti = grad_t_inhom ? grad_t_inhom[u] : nullptr;
         if (!v.gout) return fail(err, errlen, MRS_E_INVALID, "unit %zu has no cotangent", u);
+        changed |= std::memcmp(&before, &v, sizeof(VjpUnit)) != 0;
     }
-    HIP_TRY(hipMemcpyAsync(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice, st));
+    if (changed) {  // blocking copy: the host vector may change as soon as this call returns
+        HIP_TRY(hipMemcpy(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
+        p->vjp_fresh = false;
+    }
     HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
     size_t u0 = 0;
     while (u0 < n) {
@@ -806,8 +822,6 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
                                  p->d_units, p->d_slices, p->d_groups, p->d_vjp_units, st));
         u0 = u1;
     }
-    // the descriptor upload reads host memory asynchronously: wait before the vector can change
-    HIP_TRY(hipStreamSynchronize(st));
     return MRS_OK;
 }
 
