@@ -113,7 +113,7 @@ def grad_pass(iters: int):
         loss = (model(img, f, ti) - target).abs().pow(2).sum()
         loss.backward()
         opt.step()
-    l_end = float(loss)
+    l_end = float(loss.detach())
     torch.cuda.synchronize()
     t_fit = time.perf_counter() - t0
     return {"workload": "grad_128: 1 slice 128^2->64^2, 8 coils, 3 compartments, complex64",
@@ -133,7 +133,18 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--check-units", type=int, default=2)
     ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles (skips the autotuner)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run 5 device copies of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.empty(args.calibrate_copy // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.calibrate_copy, "copies": 5}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the forward has no cross-GPU step, and multi-GPU runs are not "
               "part of this build (each GPU would take its own share of units).", file=sys.stderr)
