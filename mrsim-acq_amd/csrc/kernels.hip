@@ -125,75 +125,108 @@ __device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
 }
 
 // ----------------------------------------------------------------------------- slice prep
-// Static per-slice planes (src/kspace.rs:678-725, 908-925): one thread per padded voxel.
+// Static per-slice planes (src/kspace.rs:678-725, 908-925).  Block = one 32 x 32 voxel tile of one
+// slice: user planes (x fastest) are read coalesced along x, staged in LDS, and written in the
+// kernels' [x][y] order coalesced along y.  grid = (tiles_x * tiles_y, slices).
 __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices, const DevGroup* __restrict__ groups,
                                                    int nslices) {
+    __shared__ double sd[2][32][33];
+    __shared__ float sf[1 + 2 * MRS_MAX_COMP][32][33];
+    __shared__ float2 sc[32][33];
+    __shared__ float samp[32][33];
     const int s = blockIdx.y;
     if (s >= nslices) return;
     DevSlice& sl = slices[s];
     const DevGroup& gr = groups[sl.group];
     const int snx = gr.snx, sny = gr.sny;
     const int snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
+    const int tiles_x = (snxP + 31) / 32, tiles_y = (snyP + 31) / 32;
+    if ((int)blockIdx.x >= tiles_x * tiles_y) return;
+    const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 32;
     const int C = gr.C, Q = gr.Q;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= snxP * snyP) return;
-    const int y = idx % snyP, x = idx / snyP;
-    const size_t o = (size_t)x * snyP + y;
-    const size_t PP = (size_t)snxP * snyP;
-    const bool valid = x < snx && y < sny;
-    const int i = x + snx * y;
-    const int nx = gr.nx, ny = gr.ny;
-    double base = 0.0, poly = 0.0;
-    float rate = 0.0f;
-    double amp = gr.signal_scale;
-    if (valid) {
-        const double xc = ((double)x - gr.sxs_xoff) / (double)gr.ox;
-        const double yc = ((double)y - gr.sys_yoff) / (double)gr.oy;
-        const double zc = (double)sl.z - (double)(sl.nz / 2);
-        const double zc_c = (double)sl.z - ((double)sl.nz - 1.0) / 2.0;
-        const double g0 = sl.eddy[0], g1 = sl.eddy[1], g2 = sl.eddy[2];
-        double p0 = sl.phase0 ? sl.phase0[i] : 0.0;
-        if (gr.eddy_phase != 0.0 && sl.has_eddy) p0 += gr.eddy_phase * (g0 * xc + g1 * yc + g2 * zc_c);
-        base = p0 / kTau;
-        if (gr.echo == MRS_ECHO_GRADIENT && gr.do_dist) base += (double)sl.fmap[i] * (gr.t_echo / 1000.0);
-        rate = gr.do_dist ? sl.fmap[i] : 0.0f;
-        if (gr.eddy_strength != 0.0 && sl.has_eddy) {
-            poly = gr.eddy_strength * (g0 * xc + g1 * yc + g2 * zc) +
-                   gr.eddy_quad * (g0 * xc * xc + g1 * yc * yc + g2 * zc * zc);
+    const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // read phase: x fastest
+    const double zc = (double)sl.z - (double)(sl.nz / 2);
+    const double zc_c = (double)sl.z - ((double)sl.nz - 1.0) / 2.0;
+    const double g0 = sl.eddy[0], g1 = sl.eddy[1], g2 = sl.eddy[2];
+    const bool eddy_ph = gr.eddy_phase != 0.0 && sl.has_eddy, eddy = gr.eddy_strength != 0.0 && sl.has_eddy;
+    for (int k = 0; k < 4; k++) {
+        const int yl = ty + 8 * k, x = x0 + tx, y = y0 + yl;
+        const bool valid = x < snx && y < sny;
+        const size_t i = (size_t)x + (size_t)snx * y;
+        double base = 0.0, poly = 0.0;
+        float rate = 0.0f, amp = (float)gr.signal_scale;
+        if (valid) {
+            const double xc = ((double)x - gr.sxs_xoff) / (double)gr.ox;
+            const double yc = ((double)y - gr.sys_yoff) / (double)gr.oy;
+            double p0 = sl.phase0 ? gld(sl.phase0)[i] : 0.0;
+            if (eddy_ph) p0 += gr.eddy_phase * (g0 * xc + g1 * yc + g2 * zc_c);
+            const float f = gld(sl.fmap)[i];
+            base = p0 / kTau;
+            if (gr.echo == MRS_ECHO_GRADIENT && gr.do_dist) base += (double)f * (gr.t_echo / 1000.0);
+            rate = gr.do_dist ? f : 0.0f;
+            if (eddy)
+                poly = gr.eddy_strength * (g0 * xc + g1 * yc + g2 * zc) +
+                       gr.eddy_quad * (g0 * xc * xc + g1 * yc * yc + g2 * zc * zc);
+            if (sl.log_amp) amp = (float)(gr.signal_scale * exp(gld(sl.log_amp)[i]));
         }
-        if (sl.log_amp) amp *= exp(sl.log_amp[i]);
+        sd[0][tx][yl] = base;
+        sd[1][tx][yl] = poly;
+        sf[0][tx][yl] = rate;
+        samp[tx][yl] = amp;
+        for (int q = 0; q < Q; q++) {
+            float a = 0.0f, b = 0.0f;
+            if (valid && gr.do_relax) {
+                const float t2 = sl.t2 ? gld(sl.t2)[q * P + i] : sl.t2_uniform[q];
+                const float ti = sl.t_inhom ? gld(sl.t_inhom)[q * P + i]
+                                            : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
+                a = -1.4426950408889634f / t2;
+                b = -1.4426950408889634f / ti;
+            }
+            sf[1 + 2 * q][tx][yl] = a;
+            sf[2 + 2 * q][tx][yl] = b;
+        }
     }
-    sl.base[o] = base;
-    sl.rate[o] = rate;
-    sl.poly[o] = poly;
-    for (int q = 0; q < Q; q++) {
-        float a = 0.0f, b = 0.0f;
-        if (valid && gr.do_relax) {
-            float t2 = sl.t2 ? sl.t2[(size_t)q * snx * sny + i] : sl.t2_uniform[q];
-            float ti = sl.t_inhom ? sl.t_inhom[(size_t)q * snx * sny + i]
-                                  : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
-            a = (float)(-1.4426950408889634 / (double)t2);
-            b = (float)(-1.4426950408889634 / (double)ti);
+    __syncthreads();
+    const int wy = threadIdx.x & 31, wx = threadIdx.x >> 5;   // write phase: y fastest
+    for (int k = 0; k < 4; k++) {
+        const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
+        if (x >= snxP || y >= snyP) continue;
+        const size_t o = (size_t)x * snyP + y;
+        gst(sl.base)[o] = sd[0][xl][wy];
+        gst(sl.poly)[o] = sd[1][xl][wy];
+        gst(sl.rate)[o] = sf[0][xl][wy];
+        for (int q = 0; q < Q; q++) {
+            gst(sl.nr2)[q * PP + o] = sf[1 + 2 * q][xl][wy];
+            gst(sl.nri)[q * PP + o] = sf[2 + 2 * q][xl][wy];
         }
-        sl.nr2[q * PP + o] = a;
-        sl.nri[q * PP + o] = b;
     }
     for (int c = 0; c < C; c++) {
-        float re = 0.0f, im = 0.0f;
-        if (valid) {
-            mrs_c32 v = sl.coil_maps[(size_t)c * snx * sny + i];
-            re = (float)((double)v.re * amp);
-            im = (float)((double)v.im * amp);
+        __syncthreads();
+        for (int k = 0; k < 4; k++) {
+            const int yl = ty + 8 * k, x = x0 + tx, y = y0 + yl;
+            float2 v = make_float2(0.f, 0.f);
+            if (x < snx && y < sny) {
+                const f32x2 m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(sl.coil_maps)) +
+                                                                    2 * ((size_t)c * P + (size_t)x + (size_t)snx * y));
+                const float a = samp[tx][yl];
+                v = make_float2(m.x * a, m.y * a);
+            }
+            sc[tx][yl] = v;
         }
-        uint32_t hi, lo;
-        split2(re, im, hi, lo);
-        const size_t so = ((size_t)x * C + c) * snyP + y;
-        sl.s_hi[so] = hi;
-        sl.s_lo[so] = lo;
-        sl.s32[((size_t)c * snxP + x) * snyP + y] = make_float2(re, im);
+        __syncthreads();
+        for (int k = 0; k < 4; k++) {
+            const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
+            if (x >= snxP || y >= snyP) continue;
+            const float2 v = sc[xl][wy];
+            uint32_t hi, lo;
+            split2(v.x, v.y, hi, lo);
+            const size_t so = ((size_t)x * C + c) * snyP + y;
+            gst(sl.s_hi)[so] = hi;
+            gst(sl.s_lo)[so] = lo;
+            reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
+        }
     }
-    (void)nx;
-    (void)ny;
 }
 
 // ----------------------------------------------------------------------------- E generation
@@ -862,8 +895,9 @@ __global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__
 // ----------------------------------------------------------------------------- launch wrappers
 namespace mrs {
 
-hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, hipStream_t st) {
-    dim3 grid((maxvox + 255) / 256, nslices);
+// `maxtiles`: the largest number of 32 x 32 tiles of any slice's padded grid.
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxtiles, hipStream_t st) {
+    dim3 grid(maxtiles, nslices);
     hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, nslices);
     return hipGetLastError();
 }

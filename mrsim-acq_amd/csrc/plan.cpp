@@ -212,8 +212,9 @@ struct mrs_plan {
     // y-stage launch classes keyed by (Q, RB, complex images, eddy)
     struct YClass {
         int Q, RB, cplx, eddy;
-        std::vector<WorkItem> items;
+        std::vector<WorkItem> items;  // per-chunk runs, each XCD-ordered; chunk k is [off[k], off[k+1])
         WorkItem* d_items = nullptr;
+        std::vector<size_t> off;
     };
     std::vector<YClass> yclasses;
     struct XClass {
@@ -221,7 +222,15 @@ struct mrs_plan {
         size_t lds;
         std::vector<WorkItem> items;
         WorkItem* d_items = nullptr;
+        std::vector<size_t> off;
     };
+    // units split into chunks whose y -> x stage pairs run on forked streams, so one chunk's
+    // memory-bound x-stage overlaps the next chunk's compute-bound y-stage
+    int nchunks = 1;
+    std::vector<int> unit_chunk;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> ev_chunk;
+    hipEvent_t ev_prep = nullptr;
     std::vector<XClass> xclasses;
     int xt = 8, gp = 4;
     int maxvox = 0;
@@ -240,6 +249,12 @@ struct mrs_plan {
 static void free_plan_memory(mrs_plan* p) {
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
     if (p->graph) hipGraphDestroy(p->graph);
+    for (hipStream_t s : p->streams) hipStreamDestroy(s);
+    for (hipEvent_t e : p->ev_chunk) hipEventDestroy(e);
+    if (p->ev_prep) hipEventDestroy(p->ev_prep);
+    p->streams.clear();
+    p->ev_chunk.clear();
+    p->ev_prep = nullptr;
     for (void* a : p->allocations) hipFree(a);
     p->allocations.clear();
 }
@@ -253,6 +268,20 @@ static int plan_alloc(mrs_plan* p, T** ptr, size_t n, char* err, size_t errlen) 
 
 // Reorder work so that the hardware's round-robin dispatch of consecutive blocks over the 8 XCDs
 // gives each XCD a contiguous run of items (units of one slice share that XCD's L2).
+static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w);
+
+// Concatenate per-chunk item lists, each XCD-ordered; off[k] is chunk k's first item.
+static std::vector<WorkItem> chunked(const std::vector<std::vector<WorkItem>>& per, std::vector<size_t>& off) {
+    std::vector<WorkItem> out;
+    off.assign(1, 0);
+    for (const auto& v : per) {
+        std::vector<WorkItem> o = xcd_order(v);
+        out.insert(out.end(), o.begin(), o.end());
+        off.push_back(out.size());
+    }
+    return out;
+}
+
 static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w) {
     const size_t n = w.size(), q = n / 8, r = n % 8;
     std::vector<WorkItem> out(n);
@@ -563,7 +592,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         std::memset(&sl, 0, sizeof sl);
         fill_slice_inputs(sl, slices[s]);
         const size_t snxP = (dg.snx + 15) & ~15, snyP = (dg.sny + 15) & ~15, PP = snxP * snyP;
-        p->maxvox = std::max(p->maxvox, (int)PP);
+        p->maxvox = std::max(p->maxvox, (int)(((snxP + 31) / 32) * ((snyP + 31) / 32)));  // prep tiles
         if ((rc = plan_alloc(p, &sl.base, PP, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.rate, PP, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.poly, PP, err, errlen))) break;
@@ -600,7 +629,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         auto key = std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0);
         if (!cls.count(key)) {
             cls[key] = p->yclasses.size();
-            p->yclasses.push_back({dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0, {}, nullptr});
+            p->yclasses.push_back({dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0, {}, nullptr, {}});
         }
     }
     if (rc == MRS_OK) {
@@ -627,7 +656,12 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         for (size_t u = 0; u < nunits; u++) order[u] = u;
         std::stable_sort(order.begin(), order.end(),
                          [&](size_t a, size_t b) { return units[a].slice < units[b].slice; });
-        std::vector<std::vector<WorkItem>> yw(p->yclasses.size());
+        p->nchunks = nunits >= 64 ? 4 : 1;
+        p->unit_chunk.assign(nunits, 0);
+        for (size_t k = 0; k < nunits; k++) p->unit_chunk[order[k]] = (int)(k * p->nchunks / nunits);
+        std::vector<std::vector<std::vector<WorkItem>>> yw(p->yclasses.size(),
+                                                           std::vector<std::vector<WorkItem>>(p->nchunks));
+        std::vector<std::vector<std::vector<WorkItem>>> xw;
         std::map<std::tuple<int, int, size_t>, size_t> xcls;
         // x-stage tiling per group: 8*MT kx rows of twiddles staged in LDS (<= 80 KiB so two blocks
         // share a CU), the unit's column tiles split over `nparts` blocks
@@ -659,20 +693,23 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const int gi = p->units[u].group;
             const DevGroup& dg = p->groups[gi].dev;
             const size_t c = cls[std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0)];
+            const int ch = p->unit_chunk[u];
             const int ntile = (dg.snx + p->xt - 1) / p->xt;
-            for (int t = 0; t < ntile; t++) yw[c].push_back({(int32_t)u, t});
+            for (int t = 0; t < ntile; t++) yw[c][ch].push_back({(int32_t)u, t});
             const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi]);
             if (!xcls.count(xkey)) {
                 xcls[xkey] = p->xclasses.size();
-                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], {}, nullptr});
+                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], {}, nullptr, {}});
+                xw.emplace_back(p->nchunks);
             }
-            auto& xc = p->xclasses[xcls[xkey]];
+            const size_t xi = xcls[xkey];
             const int nxP = (dg.nx + 7) & ~7;
             const int nkxb = (nxP + 16 * g_mt[gi] - 1) / (16 * g_mt[gi]);
-            for (int t = 0; t < nkxb * g_np[gi]; t++) xc.items.push_back({(int32_t)u, t});
+            for (int t = 0; t < nkxb * g_np[gi]; t++) xw[xi][ch].push_back({(int32_t)u, t});
         }
+        for (size_t c = 0; c < p->xclasses.size(); c++) p->xclasses[c].items = chunked(xw[c], p->xclasses[c].off);
         for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
-            p->yclasses[c].items = xcd_order(yw[c]);
+            p->yclasses[c].items = chunked(yw[c], p->yclasses[c].off);
             rc = plan_alloc(p, &p->yclasses[c].d_items, p->yclasses[c].items.size(), err, errlen);
             if (!rc && hipMemcpy(p->yclasses[c].d_items, p->yclasses[c].items.data(),
                                  p->yclasses[c].items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
@@ -680,12 +717,22 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         }
         for (auto& xc : p->xclasses) {
             if (rc) break;
-            xc.items = xcd_order(xc.items);
             rc = plan_alloc(p, &xc.d_items, xc.items.size(), err, errlen);
             if (!rc && hipMemcpy(xc.d_items, xc.items.data(), xc.items.size() * sizeof(WorkItem),
                                  hipMemcpyHostToDevice) != hipSuccess)
                 rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
         }
+    }
+    if (rc == MRS_OK && p->nchunks > 1) {
+        p->streams.resize(p->nchunks);
+        p->ev_chunk.resize(p->nchunks);
+        for (int k = 0; k < p->nchunks && rc == MRS_OK; k++) {
+            if (hipStreamCreateWithFlags(&p->streams[k], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&p->ev_chunk[k], hipEventDisableTiming) != hipSuccess)
+                rc = fail(err, errlen, MRS_E_DEVICE, "stream/event creation failed");
+        }
+        if (rc == MRS_OK && hipEventCreateWithFlags(&p->ev_prep, hipEventDisableTiming) != hipSuccess)
+            rc = fail(err, errlen, MRS_E_DEVICE, "event creation failed");
     }
     if (rc) { free_plan_memory(p); delete p; return rc; }
     *out = p;
@@ -697,7 +744,7 @@ static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
     p->xt = xt;
     p->gp = gp;
     for (auto& c : p->yclasses) {
-        std::vector<WorkItem> items;
+        std::vector<std::vector<WorkItem>> per(p->nchunks);
         std::vector<size_t> order(p->units.size());
         for (size_t u = 0; u < order.size(); u++) order[u] = u;
         std::stable_sort(order.begin(), order.end(),
@@ -707,14 +754,16 @@ static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
             if (dg.Q != c.Q || dg.RB != c.RB || dg.cplx != c.cplx || (dg.eddy_strength != 0.0) != (c.eddy != 0))
                 continue;
             const int ntile = (dg.snx + xt - 1) / xt;
-            for (int t = 0; t < ntile; t++) items.push_back({(int32_t)u, t});
+            for (int t = 0; t < ntile; t++) per[p->unit_chunk[u]].push_back({(int32_t)u, t});
         }
-        items = xcd_order(items);
+        std::vector<size_t> off;
+        std::vector<WorkItem> items = chunked(per, off);
         if (items.size() > c.items.size()) {
             int rc = plan_alloc(p, &c.d_items, items.size(), err, errlen);
             if (rc) return rc;
         }
         c.items = items;
+        c.off = off;
         HIP_TRY(hipMemcpy(c.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
     }
     if (p->graph_exec) {
@@ -726,17 +775,36 @@ static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
     return MRS_OK;
 }
 
+static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t errlen) {
+    for (auto& c : p->yclasses) {
+        const size_t a = c.off[k], b = c.off[k + 1];
+        HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0, (int)(b - a), p->xt, p->d_units,
+                                   p->d_slices, p->d_groups, c.d_items + a, st));
+    }
+    for (auto& xc : p->xclasses) {
+        const size_t a = xc.off[k], b = xc.off[k + 1];
+        HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(b - a), p->d_units, p->d_groups, xc.d_items + a,
+                                   st));
+    }
+    return MRS_OK;
+}
+
 static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen) {
     HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
     if (p->precision == MRS_PREC_SPLIT_BF16) {
-        for (auto& c : p->yclasses) {
-            int gp = p->gp;
-            HIP_TRY(launch_ystage_mfma(c.Q, c.RB, gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
-                                       p->d_units, p->d_slices, p->d_groups, c.d_items, st));
+        if (p->nchunks == 1) {
+            int rc = enqueue_chunk(p, 0, st, err, errlen);
+            if (rc) return rc;
+        } else {
+            HIP_TRY(hipEventRecord(p->ev_prep, st));
+            for (int k = 0; k < p->nchunks; k++) {
+                HIP_TRY(hipStreamWaitEvent(p->streams[k], p->ev_prep, 0));
+                int rc = enqueue_chunk(p, k, p->streams[k], err, errlen);
+                if (rc) return rc;
+                HIP_TRY(hipEventRecord(p->ev_chunk[k], p->streams[k]));
+            }
+            for (int k = 0; k < p->nchunks; k++) HIP_TRY(hipStreamWaitEvent(st, p->ev_chunk[k], 0));
         }
-        for (auto& xc : p->xclasses)
-            HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)xc.items.size(), p->d_units, p->d_groups,
-                                       xc.d_items, st));
     } else {
         for (size_t u = 0; u < p->units.size(); u++) {
             const DevGroup& dg = p->groups[p->units[u].group].dev;
@@ -840,13 +908,17 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
         HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
         HIP_TRY(hipEventRecord(ev[1], st));
         if (p->precision == MRS_PREC_SPLIT_BF16) {
+            // every chunk's y-stage, then every chunk's x-stage, serially on `stream`
             for (auto& c : p->yclasses)
-                HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0, (int)c.items.size(), p->xt,
-                                           p->d_units, p->d_slices, p->d_groups, c.d_items, st));
+                for (int k = 0; k < p->nchunks; k++)
+                    HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0,
+                                               (int)(c.off[k + 1] - c.off[k]), p->xt, p->d_units, p->d_slices,
+                                               p->d_groups, c.d_items + c.off[k], st));
             HIP_TRY(hipEventRecord(ev[2], st));
             for (auto& xc : p->xclasses)
-                HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)xc.items.size(), p->d_units, p->d_groups,
-                                           xc.d_items, st));
+                for (int k = 0; k < p->nchunks; k++)
+                    HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(xc.off[k + 1] - xc.off[k]),
+                                               p->d_units, p->d_groups, xc.d_items + xc.off[k], st));
         } else {
             for (size_t u = 0; u < p->units.size(); u++) {
                 const DevGroup& dg = p->groups[p->units[u].group].dev;

@@ -196,3 +196,25 @@ def test_graph_replay_tuning_and_determinism():
     for o, r in zip(outs, ref):
         assert torch.equal(o, r), "tile choice changed the result"
     plan.close()
+
+
+def test_chunked_streams_batch_matches_oracle_and_replays():
+    # 64 units: the plan splits them into 4 chunks whose y -> x stages run on forked streams
+    import torch
+    w = workloads.make("batch_128", n_slices=4, n_volumes=16)
+    plan, outs, keep = workloads.to_plan(w)
+    plan.forward()
+    torch.cuda.synchronize()
+    ref = [o.clone() for o in outs]
+    for u in (0, 37, 63):
+        want = O.forward_workload_unit(w, u)
+        e = rel(outs[u].cpu().numpy(), want)
+        assert e <= TOL, f"unit {u}: rel-L2 {e:.2e}"
+    for rep in range(2):
+        for o in outs:
+            o.zero_()
+        torch.cuda.synchronize()
+        plan.forward_graph()
+        torch.cuda.synchronize()
+        assert all(torch.equal(o, r) for o, r in zip(outs, ref)), f"replay {rep}"
+    plan.close()
