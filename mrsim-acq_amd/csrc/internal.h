@@ -20,7 +20,7 @@ struct LineRec {
     float tb0;     // s of fieldmap phase accrued (t, or t + TE/1000 for gradient echo): d phase/d fmap
     int32_t ky;    // output row, -1 for padding
     int32_t table; // x-stage table 0 (+ghost) or 1 (-ghost), src/kspace.rs:987, 1109
-    int32_t pad;
+    int32_t cls;   // side of the echo: 0 for t >= 0, 1 for t < 0 (selects the shared decay ratios)
 };
 static_assert(sizeof(LineRec) == 48, "LineRec layout");
 
@@ -35,6 +35,8 @@ struct DevSlice {
     uint32_t* s_hi;    // [x][c][y] packed bf16 (re | im<<16) of signal_scale*exp(log_amp)*S_c
     uint32_t* s_lo;    // residual of the same after s_hi, bf16
     float2* s32;       // [c][x][y] the same in FP32 (plain path)
+    float2* rot;       // [x][y] chain phase rotor cis(2 pi (rate dt + dkyn yy)) when the group shares one
+    float* rq;         // [cls][q][x][y] chain decay ratios exp2(dtrf nr2 + dtau_cls nri)
     // user inputs (device)
     const float* fmap;
     const double* phase0;
@@ -56,6 +58,9 @@ struct DevGroup {
     int32_t C, Q, RB;          // coils, compartments, 16-row blocks of 2C
     int32_t Ly;                // acquired lines
     int32_t rotor_passes;      // leading 64-line passes whose lanes' 4 lines are arithmetic progressions
+    int32_t rot_shared;        // every chain steps by the same (dt, dkyn, dtrf) and, per echo side, dtau
+    double rot_dt, rot_dk;     // the shared steps (s, cycles per voxel of yy)
+    float rot_dtrf, rot_dtau[2];
     int32_t LyP;               // padded lines (multiple of 16)
     int32_t cplx;              // complex images
     int32_t echo, do_relax, do_dist;

@@ -195,10 +195,22 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
         const size_t o = (size_t)x * snyP + y;
         gst(sl.base)[o] = sd[0][xl][wy];
         gst(sl.poly)[o] = sd[1][xl][wy];
-        gst(sl.rate)[o] = sf[0][xl][wy];
+        const float rate = sf[0][xl][wy];
+        gst(sl.rate)[o] = rate;
         for (int q = 0; q < Q; q++) {
             gst(sl.nr2)[q * PP + o] = sf[1 + 2 * q][xl][wy];
             gst(sl.nri)[q * PP + o] = sf[2 + 2 * q][xl][wy];
+        }
+        if (gr.rot_shared) {
+            // chain rotors shared by every lane of a chain pass (FP64 angle, FP64 reduction)
+            const double yy = (double)y - gr.sys_yoff;
+            float rs, rc;
+            sincos_turns(__builtin_fma(gr.rot_dk, yy, (double)rate * gr.rot_dt), rs, rc);
+            reinterpret_cast<GAS f32x2*>(gst(sl.rot))[o] = f32x2{rc, rs};
+            for (int c = 0; c < 2; c++)
+                for (int q = 0; q < Q; q++)
+                    gst(sl.rq)[(c * Q + q) * PP + o] = __builtin_amdgcn_exp2f(
+                        __builtin_fmaf(gr.rot_dtrf, sf[1 + 2 * q][xl][wy], gr.rot_dtau[c] * sf[2 + 2 * q][xl][wy]));
         }
     }
     for (int c = 0; c < C; c++) {
@@ -363,28 +375,34 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                 const double yy0 = (double)y0 - sys_yoff;
                 if constexpr (GP == 4 && !EDDY) {
                     if ((g0 >> 2) < gr.rotor_passes) {
-                        // The lane's four lines form an arithmetic progression in (t, kyn, trf, tau):
-                        // evaluate line 0 directly, then advance the phase by the rotor
-                        // cis(2 pi (rate dt + dkyn yy)) and each decay by exp2(dtrf nr2 + dtau nri).
+                        // The lane's four lines form an arithmetic progression in (t, kyn, trf, tau) with
+                        // the group's shared steps: evaluate line 0 directly, then advance the phase by
+                        // the voxel's rotor cis(2 pi (rate dt + dkyn yy)) and each decay by the voxel's
+                        // exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.
                         const LineRec& L0 = lines_s[g0 * 16 + ln];
-                        const LineRec& L1 = lines_s[(g0 + 1) * 16 + ln];
-                        const double dt = L1.t_s - L0.t_s, dk = L1.kyn - L0.kyn;
-                        const float dtrf = L1.trf - L0.trf, dtau = L1.tau - L0.tau;
+                        // the chain rotor and this echo side's decay ratios, precomputed per voxel
+                        const size_t ro = (size_t)x * snyP + y0;
+                        const GAS f32x4* rp = reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro);
+                        const f32x4 srot0 = rp[0], srot1 = rp[1];
+                        f32x4 srq[Q];
+#pragma unroll
+                        for (int q = 0; q < Q; q++)
+                            srq[q] = *reinterpret_cast<const GAS f32x4*>(gld(sl.rq) + (L0.cls * Q + q) * PP + ro);
                         u32x4 bh[4], bl[4];
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
                             const double yy = yy0 + (double)j;
                             double c0 = __builtin_fma(L0.kyn, yy, v.base[j]);
                             c0 = __builtin_fma((double)v.rate[j], L0.t_s, c0);
-                            const double dc = __builtin_fma(dk, yy, (double)v.rate[j] * dt);
-                            float ps, pc, rs, rc;
+                            float ps, pc;
                             sincos_turns(c0, ps, pc);
-                            sincos_turns(dc, rs, rc);
+                            const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
+                            const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
                             float dq[Q], rq[Q];
 #pragma unroll
                             for (int q = 0; q < Q; q++) {
                                 dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(L0.trf, v.nr2[q][j], L0.tau * v.nri[q][j]));
-                                rq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(dtrf, v.nr2[q][j], dtau * v.nri[q][j]));
+                                rq[q] = srq[q][j];
                             }
 #pragma unroll
                             for (int g = 0; g < 4; g++) {

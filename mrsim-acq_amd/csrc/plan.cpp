@@ -386,6 +386,7 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
         L.tau = a.echo == MRS_ECHO_SPIN ? (float)(std::fabs(L.t_s) * 1000.0) : (float)trf[k];
         L.tb0 = a.echo == MRS_ECHO_SPIN ? (float)L.t_s : (float)(L.t_s + a.t_echo / 1000.0);
         L.ky = (int32_t)k;
+        L.cls = L.t_s >= 0.0 ? 0 : 1;
         // src/kspace.rs:987 and :1109: table 1 iff the line's ghost shift is negative
         const double shift = pol[k] < 0 ? -a.ghost_offset : a.ghost_offset;
         L.table = (a.ghost_offset != 0.0 && shift < 0.0) ? 1 : 0;
@@ -457,6 +458,33 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     dg.Ly = Ly;
     dg.LyP = (int)g.lines.size();
     dg.rotor_passes = rotor_passes;
+    // one shared rotor step for every chain of the chain passes?
+    {
+        bool ok = rotor_passes > 0;
+        bool have[2] = {false, false};
+        double dt = 0, dk = 0;
+        float dtrf = 0, dtau[2] = {0, 0};
+        for (int pss = 0; pss < rotor_passes && ok; pss++)
+            for (int ln = 0; ln < 16 && ok; ln++) {
+                const LineRec& a = g.lines[pss * 64 + ln];
+                const LineRec& b = g.lines[pss * 64 + 16 + ln];
+                const double sdt = b.t_s - a.t_s, sdk = b.kyn - a.kyn;
+                const float strf = b.trf - a.trf, stau = b.tau - a.tau;
+                if (pss == 0 && ln == 0) { dt = sdt; dk = sdk; dtrf = strf; }
+                const double tol = 1e-9 * std::max(std::fabs(dt), 1e-12);
+                ok = std::fabs(sdt - dt) <= tol && std::fabs(sdk - dk) <= 1e-12 && strf == dtrf;
+                if (!ok) break;
+                if (!have[a.cls]) { have[a.cls] = true; dtau[a.cls] = stau; }
+                ok = stau == dtau[a.cls];
+            }
+        dg.rot_shared = ok ? 1 : 0;
+        if (!ok) dg.rotor_passes = 0;  // chain passes run only with shared steps
+        dg.rot_dt = dt;
+        dg.rot_dk = dk;
+        dg.rot_dtrf = dtrf;
+        dg.rot_dtau[0] = dtau[0];
+        dg.rot_dtau[1] = dtau[1];
+    }
     dg.cplx = d.complex_images ? 1 : 0;
     dg.echo = a.echo; dg.do_relax = a.do_relaxation ? 1 : 0; dg.do_dist = a.do_distortions ? 1 : 0;
     dg.sys_yoff = (double)(oy * ys) + yoff;
@@ -601,6 +629,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         if ((rc = plan_alloc(p, &sl.s_hi, PP * dg.C, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.s_lo, PP * dg.C, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.s32, PP * dg.C, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.rot, dg.rot_shared ? PP : 1, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.rq, dg.rot_shared ? 2 * PP * dg.Q : 1, err, errlen))) break;
     }
     // per-unit scratch and work lists
     std::map<std::tuple<int, int, int, int>, size_t> cls;
