@@ -32,7 +32,8 @@ struct DevSlice {
     double* poly;      // P: eddy polynomial (cycles per second of eddy clock), 0 without eddy
     float* nr2;        // Q*P: -log2(e)/T2 (1/ms), 0 without relaxation
     float* nri;        // Q*P: -log2(e)/T2' (1/ms)
-    uint32_t* s_hi;    // [x][c][y] packed bf16 (re | im<<16) of signal_scale*exp(log_amp)*S_c
+    uint32_t* s_hi;    // [x][c][y] packed bf16 (re | im<<16) of signal_scale*exp(log_amp)*S_c, c padded
+                       // with zero coils to RB*8
     uint32_t* s_lo;    // residual of the same after s_hi, bf16
     float2* s32;       // [c][x][y] the same in FP32 (plain path)
     float2* rot;       // [x][y] chain phase rotor cis(2 pi (rate dt + dkyn yy)) when the group shares one

@@ -213,12 +213,13 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
                         __builtin_fmaf(gr.rot_dtrf, sf[1 + 2 * q][xl][wy], gr.rot_dtau[c] * sf[2 + 2 * q][xl][wy]));
         }
     }
-    for (int c = 0; c < C; c++) {
+    const int Cp = gr.RB * 8;   // split maps padded with zero coils to whole 8-coil row blocks
+    for (int c = 0; c < Cp; c++) {
         __syncthreads();
         for (int k = 0; k < 4; k++) {
             const int yl = ty + 8 * k, x = x0 + tx, y = y0 + yl;
             float2 v = make_float2(0.f, 0.f);
-            if (x < snx && y < sny) {
+            if (c < C && x < snx && y < sny) {
                 const f32x2 m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(sl.coil_maps)) +
                                                                     2 * ((size_t)c * P + (size_t)x + (size_t)snx * y));
                 const float a = samp[tx][yl];
@@ -233,10 +234,10 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
             const float2 v = sc[xl][wy];
             uint32_t hi, lo;
             split2(v.x, v.y, hi, lo);
-            const size_t so = ((size_t)x * C + c) * snyP + y;
+            const size_t so = ((size_t)x * Cp + c) * snyP + y;
             gst(sl.s_hi)[so] = hi;
             gst(sl.s_lo)[so] = lo;
-            reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
+            if (c < C) reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
         }
     }
 }
@@ -255,14 +256,14 @@ struct Vox4 {
 
 template <int Q, bool CPLX, bool EDDY>
 __device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice& sl, const void* images, int snx,
-                                          int sny, int x, int y0, int snyP, size_t PP) {
-    const size_t o = (size_t)x * snyP + y0;
-    const GAS f64x2* b2 = reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o);
-    const f64x2 b01 = b2[0], b23 = b2[1];
+                                          int sny, int x, int y0, int snyP, int PP) {
+    const int o = x * snyP + y0;   // 32-bit offsets from scalar bases: global loads with saddr
+    const f64x2 b01 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o);
+    const f64x2 b23 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o + 2);
     v.base[0] = b01.x; v.base[1] = b01.y; v.base[2] = b23.x; v.base[3] = b23.y;
     if constexpr (EDDY) {
-        const GAS f64x2* p2 = reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o);
-        const f64x2 p01 = p2[0], p23 = p2[1];
+        const f64x2 p01 = *reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o);
+        const f64x2 p23 = *reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o + 2);
         v.poly[0] = p01.x; v.poly[1] = p01.y; v.poly[2] = p23.x; v.poly[3] = p23.y;
     }
     const f32x4 r4 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rate) + o);
@@ -274,21 +275,21 @@ __device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice
         v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
         v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
     }
-    const size_t P = (size_t)snx * sny;
+    const int P = snx * sny;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
         const int y = y0 + j;
-        const bool ok = x < snx && y < sny;
-        const size_t i = (size_t)x + (size_t)snx * y;
+        const bool ok = y < sny;                       // x < snx always holds here
+        const int i = x + snx * (ok ? y : sny - 1);    // clamped: every load is in bounds
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             if constexpr (CPLX) {
-                f32x2 m = {0.f, 0.f};
-                if (ok) m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images)) + 2 * (q * P + i));
-                v.cr[q][j] = m.x;
-                v.ci[q][j] = m.y;
+                const f32x2 m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images)) + 2 * (q * P + i));
+                v.cr[q][j] = ok ? m.x : 0.0f;
+                v.ci[q][j] = ok ? m.y : 0.0f;
             } else {
-                v.cr[q][j] = ok ? gld(reinterpret_cast<const float*>(images))[q * P + i] : 0.0f;
+                const float m = gld(reinterpret_cast<const float*>(images))[q * P + i];
+                v.cr[q][j] = ok ? m : 0.0f;
             }
         }
     }
@@ -324,8 +325,12 @@ __device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, doubl
 //   G[(coil, part), line] = sum_{y, part'} A[(coil,part), (y,part')] * B[(y,part'), line]
 // with A the split coil sensitivities and B = E generated in registers, 16 lines per MFMA tile,
 // GP tiles per pass and RB 16-row blocks (8 coils each).
+// Occupancy bound: 4 waves/SIMD (128 VGPRs) where the instantiation fits without spilling, else 3.
+template <int Q, int RB, bool CPLX, bool EDDY>
+constexpr int ystage_waves() { return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : 3; }
+
 template <int Q, int RB, int GP, bool CPLX, bool EDDY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_waves<Q, RB, CPLX, EDDY>())))
 void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
                  const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work, int xt) {
     __shared__ LineRec lines_s[256];
@@ -356,21 +361,16 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
             for (int ch = 0; ch < snyP; ch += 16) {
                 const int y0 = ch + lq * 4;
                 Vox4<Q, CPLX, EDDY> v;
-                load_vox4<Q, CPLX, EDDY>(v, sl, u.images, snx, sny, x, y0, snyP, PP);
+                load_vox4<Q, CPLX, EDDY>(v, sl, u.images, snx, sny, x, y0, snyP, (int)PP);
                 u32x4 ahi[RB], alo[RB];
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++) {
-                    const int c = rb * 8 + (ln & 7);
-                    if (c < C) {
-                        const size_t so = ((size_t)x * C + c) * snyP + y0;
-                        u32x4 h = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_hi) + so);
-                        u32x4 l = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_lo) + so);
-                        ahi[rb] = rows_of(h, part);
-                        alo[rb] = rows_of(l, part);
-                    } else {
-                        ahi[rb] = u32x4{0u, 0u, 0u, 0u};
-                        alo[rb] = u32x4{0u, 0u, 0u, 0u};
-                    }
+                    // split maps are padded to RB*8 coils with zeros: no range check
+                    const int so = (x * (RB * 8) + rb * 8 + (ln & 7)) * snyP + y0;
+                    const u32x4 h = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_hi) + so);
+                    const u32x4 l = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_lo) + so);
+                    ahi[rb] = rows_of(h, part);
+                    alo[rb] = rows_of(l, part);
                 }
                 const double yy0 = (double)y0 - sys_yoff;
                 if constexpr (GP == 4 && !EDDY) {
@@ -381,13 +381,13 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         // exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.
                         const LineRec& L0 = lines_s[g0 * 16 + ln];
                         // the chain rotor and this echo side's decay ratios, precomputed per voxel
-                        const size_t ro = (size_t)x * snyP + y0;
-                        const GAS f32x4* rp = reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro);
-                        const f32x4 srot0 = rp[0], srot1 = rp[1];
+                        const int ro = x * snyP + y0;
+                        const f32x4 srot0 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro);
+                        const f32x4 srot1 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro + 2);
                         f32x4 srq[Q];
 #pragma unroll
                         for (int q = 0; q < Q; q++)
-                            srq[q] = *reinterpret_cast<const GAS f32x4*>(gld(sl.rq) + (L0.cls * Q + q) * PP + ro);
+                            srq[q] = *reinterpret_cast<const GAS f32x4*>(gld(sl.rq) + (L0.cls * Q + q) * (int)PP + ro);
                         u32x4 bh[4], bl[4];
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
@@ -920,26 +920,27 @@ hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, in
     return hipGetLastError();
 }
 
-template <int Q, int RB, bool CPLX, bool EDDY>
-static hipError_t ylaunch_gp(int gp, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
-                             const DevGroup* g, const WorkItem* w, int xt) {
-    switch (gp) {
-    case 1: hipLaunchKernelGGL((ystage_mfma<Q, RB, 1, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
-    case 2: hipLaunchKernelGGL((ystage_mfma<Q, RB, 2, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
-    case 4: hipLaunchKernelGGL((ystage_mfma<Q, RB, 4, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt); break;
-    default: return hipErrorInvalidValue;
-    }
+template <int Q, int RB, int GP, bool CPLX, bool EDDY>
+static hipError_t ylaunch(dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                          const WorkItem* w, int xt) {
+    hipLaunchKernelGGL((ystage_mfma<Q, RB, GP, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt);
     return hipGetLastError();
 }
 
+// Instantiated (RB, GP) pairs: the accumulators RB*GP*4 VGPRs stay within budget.
 template <int Q, bool CPLX, bool EDDY>
 static hipError_t ylaunch_rb(int rb, int gp, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, const WorkItem* w, int xt) {
     switch (rb) {
-    case 1: return ylaunch_gp<Q, 1, CPLX, EDDY>(gp, grid, st, u, s, g, w, xt);
-    case 2: return ylaunch_gp<Q, 2, CPLX, EDDY>(gp > 2 ? 2 : gp, grid, st, u, s, g, w, xt);
-    case 4: return ylaunch_gp<Q, 4, CPLX, EDDY>(1, grid, st, u, s, g, w, xt);
-    case 8: return ylaunch_gp<Q, 8, CPLX, EDDY>(1, grid, st, u, s, g, w, xt);
+    case 1:
+        if (gp >= 4) return ylaunch<Q, 1, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        if (gp == 2) return ylaunch<Q, 1, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        return ylaunch<Q, 1, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+    case 2:
+        if (gp >= 2) return ylaunch<Q, 2, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        return ylaunch<Q, 2, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+    case 4: return ylaunch<Q, 4, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+    case 8: return ylaunch<Q, 8, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
     default: return hipErrorInvalidValue;
     }
 }

@@ -626,8 +626,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         if ((rc = plan_alloc(p, &sl.poly, PP, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.nr2, PP * dg.Q, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.nri, PP * dg.Q, err, errlen))) break;
-        if ((rc = plan_alloc(p, &sl.s_hi, PP * dg.C, err, errlen))) break;
-        if ((rc = plan_alloc(p, &sl.s_lo, PP * dg.C, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.s_hi, PP * dg.RB * 8, err, errlen))) break;
+        if ((rc = plan_alloc(p, &sl.s_lo, PP * dg.RB * 8, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.s32, PP * dg.C, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.rot, dg.rot_shared ? PP : 1, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.rq, dg.rot_shared ? 2 * PP * dg.Q : 1, err, errlen))) break;
