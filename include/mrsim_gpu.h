@@ -217,7 +217,8 @@ int mrs_plan_forward_graph(mrs_plan* plan, void* stream, char* err, size_t errle
 int mrs_plan_vjp(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* grad_images, float* const* grad_fmap,
                  float* const* grad_t_inhom, void* stream, char* err, size_t errlen);
 /* Average duration (ms) of each forward stage over `iters` eager launches, timed with HIP events
- * on `stream`: ms[0] slice prep, ms[1] y-stage, ms[2] x-stage, ms[3] zero rows + spikes/noise. */
+ * on `stream`: ms[0] slice prep, ms[1] y-stage, ms[2] x-stage, ms[3] zero rows + spikes/noise,
+ * ms[4] the attached partition transform (mrs_plan_set_zstage). */
 int mrs_plan_stage_times(mrs_plan* plan, void* stream, int32_t iters, double* ms, char* err, size_t errlen);
 /* Terms of the literal sum per batch (sum of n_coils*nx*Ly*snx*sny) and acquired lines. */
 int mrs_plan_stats(const mrs_plan* plan, double* terms, double* lines, double* units);
@@ -230,6 +231,56 @@ int mrs_plan_tune(mrs_plan* plan, void* stream, int32_t trials, char* report, si
  * (1, 2 or 4). */
 int mrs_plan_set_tiles(mrs_plan* plan, int32_t x_tile, int32_t line_tiles, char* err, size_t errlen);
 int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tiles);
+
+/* ---- 3D gradient-echo train (stack of EPI blocks over nz partitions) ----------------------- */
+
+/* Within-echo timing of a GRASE/3D-GE block on ny lines: t_ms[ky] from the echo centre and the
+ * readout polarity (+1/-1, alternating with the acquisition index within the block).
+ * Replaces readout::grase_block (src/readout.rs:206-221); refuses ny not divisible by
+ * ky_segments with the reference's text. */
+int mrs_grase_block(size_t ny, size_t ky_segments, double t_line_ms, int32_t reverse_phase, double* t_ms,
+                    int8_t* polarity, char* err, size_t errlen);
+
+/* Line table of a 3D gradient-echo train on ny x nz (index p*ny + ky): the shot (sy*kz_segments + sz)
+ * and the excitation within it that reads each line.  kz_order 0 = linear, 1 = centric.  Any output
+ * may be NULL.  Replaces readout::ge3d_lines (src/readout.rs:428-462) with readout::kz_order
+ * (src/readout.rs:170-189). */
+int mrs_ge3d_lines(size_t ny, size_t nz, size_t ky_segments, double t_line_ms, int32_t reverse_phase,
+                   size_t kz_segments, int32_t kz_order, int32_t* shot, int32_t* excitation, size_t* n_shots,
+                   size_t* n_exc, size_t* e_c, char* err, size_t errlen);
+
+/* Timing checks of a 3D gradient-echo train (echo blocks clear of the excitation pulse, of each other,
+ * of the next excitation and of the repetition).  Replaces readout::check_ge3d_timing
+ * (src/readout.rs:465-501); the refusal texts are the reference's. */
+int mrs_check_ge3d_timing(const double* echo_times_ms, size_t n_echoes, double exc_spacing_ms,
+                          double excitation_time_ms, size_t ny, size_t ky_segments, double t_line_ms,
+                          int32_t reverse_phase, size_t n_exc, double t_exc_ms, double tr_ms, char* err,
+                          size_t errlen);
+
+/* One set of the batched partition transform: nz partitions of n_coils*ny*nx complex64 samples,
+ * layout [p][coil][ky][kx] (device pointers).
+ *   forward: out[p] (+)= sum_c w(p, ky % ky_segments, c) sum_z zk[p][z] in_c[z]
+ *   adjoint: out_c[z] (+)= sum_p conj(zk[p][z]) w(p, ky % ky_segments, c) in[p]
+ * zk[p][z] = exp(+i 2 pi (p - nz/2)(z - nz/2)/nz); in_c (forward) or out_c (adjoint) is compartment c's
+ * block, comp_stride complex elements apart; weights[(p*ky_segments + sy)*n_comp + c] (NULL = 1).
+ * Replaces the z-DFT and weights of kspace3d::GePlan::{compartments,kspace}
+ * (src/kspace3d.rs:677-692, 716-722) with zkernel (src/kspace3d.rs:90-100).  nz <= 64. */
+typedef struct mrs_zset {
+    const mrs_c32* in;
+    mrs_c32* out;
+    const float* weights;
+    size_t comp_stride;
+    uint32_t nz, n_comp, n_coils, ny, nx, ky_segments;
+    int32_t accumulate;
+    int32_t reserved;
+} mrs_zset;
+
+/* Run sets on `stream` and wait for them (the call allocates and frees its descriptor copy). */
+int mrs_zstage(const mrs_zset* sets, size_t nsets, int32_t adjoint, void* stream, char* err, size_t errlen);
+
+/* Attach forward sets to a plan: they run after the 2D forward inside mrs_plan_forward and its graph.
+ * nsets = 0 detaches.  Sets that accumulate into the same output must not be attached together. */
+int mrs_plan_set_zstage(mrs_plan* plan, const mrs_zset* sets, size_t nsets, char* err, size_t errlen);
 
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);

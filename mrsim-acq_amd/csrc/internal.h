@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/mrsim_gpu.h"
 
 namespace mrs {
@@ -97,6 +99,26 @@ struct WorkItem {
     int32_t unit;
     int32_t tile;   // x tile (y-stage) or column tile (x-stage)
 };
+
+// One z-stage set (the batched 3D partition transform): nz partitions of ncols = coils * ny * nx
+// complex samples, layout [partition][coil][ky][kx]; n_comp compartment blocks cstride apart on the
+// side that carries compartments (the input of the forward, the output of the adjoint).
+struct ZSet {
+    const float2* in;
+    float2* out;
+    const float* w;       // [nz][kseg][nq] per-excitation weights, or null (all 1)
+    int64_t cstride;      // complex elements between compartment blocks
+    int32_t nz, ncols, ny, nx, kseg, nq, accumulate, mt;
+};
+
+struct ZBlock {
+    int32_t set;
+    int32_t col0;
+};
+
+int build_zsets(const mrs_zset* in, size_t n, bool adjoint, std::vector<ZSet>& sets, std::vector<ZBlock>& blocks,
+                int off[4], char* err, size_t errlen);
+hipError_t enqueue_zsets(const ZSet* d_sets, const ZBlock* d_blocks, const int off[4], bool adjoint, hipStream_t st);
 
 struct VjpUnit {
     const float2* gout;   // cotangent C x ny x nx

@@ -814,6 +814,174 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
     }
 }
 
+// ----------------------------------------------------------------------------- z-stage (3D partitions)
+// Centred z-DFT across partitions with per-excitation weights, kspace3d.rs:90-100, 677-692, 716-722:
+//   forward  out[p][n] (+)= sum_c w(p, sy(n), c) sum_z zk[p][z] in_c[z][n]
+//   adjoint  out_c[z][n] (+)= sum_p conj(zk[p][z]) w(p, sy(n), c) in[p][n]
+// zk[p][z] = e^{+i 2 pi (p - nz/2)(z - nz/2)/nz}, symmetric in (p, z); column n = (coil, ky, kx),
+// sy(n) = ky % ky_segments, w(p, sy, c) = weights[(p * ky_segments + sy) * n_comp + c].
+// A block of 256 threads takes one set and ZCOLS columns; the table (split hi/lo, zero-padded to
+// MT*16 square) is built once per block in LDS from an exactly reduced integer angle; each wave
+// contracts 16-column tiles with the complex structure on the B operand, as the x-stage does.
+static constexpr int kZCols = 256;
+
+template <int MT, bool ADJ>
+__global__ __launch_bounds__(256) void zstage(const ZSet* __restrict__ sets, const ZBlock* __restrict__ blocks) {
+    constexpr int NP = MT * 16;
+    __shared__ uint32_t zh[NP][NP + 4], zl[NP][NP + 4];
+    const ZBlock blk = blocks[blockIdx.x];
+    const ZSet S = sets[blk.set];
+    const int nz = S.nz, h = nz / 2;
+    for (int i = threadIdx.x; i < NP * NP; i += 256) {
+        const int r = i / NP, k = i % NP;
+        float re = 0.0f, im = 0.0f;
+        if (r < nz && k < nz) {
+            int m = ((r - h) * (k - h)) % nz;   // the angle is 2 pi m / nz exactly
+            if (m < 0) m += nz;
+            double sv, cv;
+            sincos(kTau * (double)m / (double)nz, &sv, &cv);
+            re = (float)cv;
+            im = ADJ ? -(float)sv : (float)sv;
+        }
+        uint32_t hi, lo;
+        split2(re, im, hi, lo);
+        zh[r][k] = hi;
+        zl[r][k] = lo;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane >> 4, ln = lane & 15;
+    const int ncols = S.ncols;
+    for (int it = 0; it < kZCols / 64; it++) {
+        const int n0 = blk.col0 + (it * 4 + wave) * 16;
+        if (n0 >= ncols) break;
+        const int n = n0 + ln;
+        const bool col_ok = n < ncols;
+        const int nc = col_ok ? n : ncols - 1;
+        const int sy = ((nc / S.nx) % S.ny) % S.kseg;
+        if (!ADJ) {
+            f32x4 outr[MT], outi[MT];
+#pragma unroll
+            for (int m = 0; m < MT; m++) outr[m] = outi[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int c = 0; c < S.nq; c++) {
+                const GAS f32x2* __restrict__ in =
+                    reinterpret_cast<const GAS f32x2*>(gld(S.in)) + (size_t)c * S.cstride;
+                f32x4 ar[MT], ai[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) ar[m] = ai[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kc = 0; kc < MT; kc++) {
+                    u32x4 bhi, blo;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int z = kc * 16 + lq * 4 + j;
+                        f32x2 v = {0.f, 0.f};
+                        if (z < nz) v = in[(size_t)z * ncols + nc];
+                        uint32_t hi, lo;
+                        split2(v.x, v.y, hi, lo);
+                        bhi[j] = hi;
+                        blo[j] = lo;
+                    }
+                    const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+                    const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+#pragma unroll
+                    for (int m = 0; m < MT; m++) {
+                        const int r = m * 16 + ln, k0 = kc * 16 + lq * 4;
+                        const u32x4 th = *reinterpret_cast<const u32x4*>(&zh[r][k0]);
+                        const u32x4 tl = *reinterpret_cast<const u32x4*>(&zl[r][k0]);
+                        ar[m] = mfma3(th, tl, brh, brl, ar[m]);
+                        ai[m] = mfma3(th, tl, bih, bil, ai[m]);
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int p = m * 16 + lq * 4 + r;
+                        const float wv = (S.w && p < nz) ? gld(S.w)[(p * S.kseg + sy) * S.nq + c] : 1.0f;
+                        outr[m][r] = __builtin_fmaf(wv, ar[m][r], outr[m][r]);
+                        outi[m][r] = __builtin_fmaf(wv, ai[m][r], outi[m][r]);
+                    }
+                }
+            }
+            if (!col_ok) continue;
+            GAS f32x2* out = reinterpret_cast<GAS f32x2*>(gst(S.out));
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int p = m * 16 + lq * 4 + r;
+                    if (p >= nz) continue;
+                    const size_t o = (size_t)p * ncols + n;
+                    f32x2 v = {outr[m][r], outi[m][r]};
+                    if (S.accumulate) {
+                        const f32x2 old = out[o];
+                        v.x += old.x;
+                        v.y += old.y;
+                    }
+                    out[o] = v;
+                }
+            }
+        } else {
+            const GAS f32x2* __restrict__ in = reinterpret_cast<const GAS f32x2*>(gld(S.in));
+            for (int c = 0; c < S.nq; c++) {
+                f32x4 ar[MT], ai[MT];
+#pragma unroll
+                for (int m = 0; m < MT; m++) ar[m] = ai[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kc = 0; kc < MT; kc++) {
+                    u32x4 bhi, blo;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int p = kc * 16 + lq * 4 + j;
+                        f32x2 v = {0.f, 0.f};
+                        if (p < nz) {
+                            v = in[(size_t)p * ncols + nc];
+                            if (S.w) {
+                                const float wv = gld(S.w)[(p * S.kseg + sy) * S.nq + c];
+                                v.x *= wv;
+                                v.y *= wv;
+                            }
+                        }
+                        uint32_t hi, lo;
+                        split2(v.x, v.y, hi, lo);
+                        bhi[j] = hi;
+                        blo[j] = lo;
+                    }
+                    const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+                    const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+#pragma unroll
+                    for (int m = 0; m < MT; m++) {
+                        const int r = m * 16 + ln, k0 = kc * 16 + lq * 4;
+                        const u32x4 th = *reinterpret_cast<const u32x4*>(&zh[r][k0]);
+                        const u32x4 tl = *reinterpret_cast<const u32x4*>(&zl[r][k0]);
+                        ar[m] = mfma3(th, tl, brh, brl, ar[m]);
+                        ai[m] = mfma3(th, tl, bih, bil, ai[m]);
+                    }
+                }
+                if (!col_ok) continue;
+                GAS f32x2* out = reinterpret_cast<GAS f32x2*>(gst(S.out)) + (size_t)c * S.cstride;
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int z = m * 16 + lq * 4 + r;
+                        if (z >= nz) continue;
+                        const size_t o = (size_t)z * ncols + n;
+                        f32x2 v = {ar[m][r], ai[m][r]};
+                        if (S.accumulate) {
+                            const f32x2 old = out[o];
+                            v.x += old.x;
+                            v.y += old.y;
+                        }
+                        out[o] = v;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------- epilogues
 // Unacquired rows are exactly zero (src/kspace.rs:839-843).  grid.y = unit.
 __global__ void zero_rows(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups, int unit0) {
@@ -1050,6 +1218,29 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, 
     dim3 grid((snx * sny + 255) / 256, n);
     return cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0) : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0);
 }
+
+
+template <int MT>
+static void zlaunch(bool adjoint, int nblocks, hipStream_t st, const ZSet* sets, const ZBlock* blocks) {
+    if (adjoint)
+        hipLaunchKernelGGL((zstage<MT, true>), dim3(nblocks), dim3(256), 0, st, sets, blocks);
+    else
+        hipLaunchKernelGGL((zstage<MT, false>), dim3(nblocks), dim3(256), 0, st, sets, blocks);
+}
+
+// Blocks are grouped by the table size MT (nz <= 16, 32, 64): one launch per MT present.
+hipError_t launch_zstage(int mt, bool adjoint, const ZSet* sets, const ZBlock* blocks, int nblocks, hipStream_t st) {
+    if (nblocks <= 0) return hipSuccess;
+    switch (mt) {
+    case 1: zlaunch<1>(adjoint, nblocks, st, sets, blocks); break;
+    case 2: zlaunch<2>(adjoint, nblocks, st, sets, blocks); break;
+    case 4: zlaunch<4>(adjoint, nblocks, st, sets, blocks); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+int zstage_columns_per_block() { return kZCols; }
 
 hipError_t launch_probe_sincos(const double* cyc, float* s, float* c, int n, int mode, hipStream_t st) {
     hipLaunchKernelGGL(probe_sincos, dim3((n + 255) / 256), dim3(256), 0, st, cyc, s, c, n, mode);

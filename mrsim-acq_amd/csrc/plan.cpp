@@ -244,6 +244,12 @@ struct mrs_plan {
     std::vector<VjpUnit> vjp_units;
     VjpUnit* d_vjp_units = nullptr;
     bool vjp_fresh = true;
+    // partition transform run after the 2D forward (3D trains)
+    std::vector<ZSet> zsets;
+    void* d_zmem = nullptr;
+    ZSet* d_zsets = nullptr;
+    ZBlock* d_zblocks = nullptr;
+    int zoff[4] = {0, 0, 0, 0};
 };
 
 static void free_plan_memory(mrs_plan* p) {
@@ -257,6 +263,8 @@ static void free_plan_memory(mrs_plan* p) {
     p->ev_prep = nullptr;
     for (void* a : p->allocations) hipFree(a);
     p->allocations.clear();
+    if (p->d_zmem) hipFree(p->d_zmem);
+    p->d_zmem = nullptr;
 }
 
 template <class T>
@@ -844,6 +852,33 @@ static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen
     if (p->any_zero_rows) HIP_TRY(launch_zero_rows(0, (int)p->units.size(), p->d_units, p->d_groups, st));
     if (p->any_epilogue)
         HIP_TRY(launch_spikes_noise(0, (int)p->units.size(), p->maxC, p->d_units, p->d_groups, p->d_rows, st));
+    if (!p->zsets.empty()) HIP_TRY(enqueue_zsets(p->d_zsets, p->d_zblocks, p->zoff, false, st));
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_set_zstage(mrs_plan* p, const mrs_zset* sets, size_t nsets, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    std::vector<ZSet> zs;
+    std::vector<ZBlock> blocks;
+    int off[4];
+    int rc = build_zsets(sets, nsets, false, zs, blocks, off, err, errlen);
+    if (rc) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+    if (p->d_zmem) hipFree(p->d_zmem);
+    p->d_zmem = nullptr;
+    p->zsets = zs;
+    std::copy(off, off + 4, p->zoff);
+    if (zs.empty()) return MRS_OK;
+    const size_t sb = zs.size() * sizeof(ZSet), bb = blocks.size() * sizeof(ZBlock);
+    HIP_TRY(hipMalloc(&p->d_zmem, sb + bb));
+    p->d_zsets = (ZSet*)p->d_zmem;
+    p->d_zblocks = (ZBlock*)((char*)p->d_zmem + sb);
+    HIP_TRY(hipMemcpy(p->d_zsets, zs.data(), sb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_zblocks, blocks.data(), bb, hipMemcpyHostToDevice));
     return MRS_OK;
 }
 
@@ -929,9 +964,9 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
     if (!p || !ms) return fail(err, errlen, MRS_E_INVALID, "null argument");
     hipStream_t st = (hipStream_t)stream;
     if (iters < 1) iters = 1;
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    double acc[4] = {0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     int rc = MRS_OK;
     for (int it = 0; it < iters && rc == MRS_OK; it++) {
         HIP_TRY(hipEventRecord(ev[0], st));
@@ -962,15 +997,17 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
         if (p->any_epilogue)
             HIP_TRY(launch_spikes_noise(0, (int)p->units.size(), p->maxC, p->d_units, p->d_groups, p->d_rows, st));
         HIP_TRY(hipEventRecord(ev[4], st));
-        HIP_TRY(hipEventSynchronize(ev[4]));
-        for (int k = 0; k < 4; k++) {
+        if (!p->zsets.empty()) HIP_TRY(enqueue_zsets(p->d_zsets, p->d_zblocks, p->zoff, false, st));
+        HIP_TRY(hipEventRecord(ev[5], st));
+        HIP_TRY(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; k++) {
             float t = 0.f;
             HIP_TRY(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
             acc[k] += t;
         }
     }
     for (auto& e : ev) hipEventDestroy(e);
-    for (int k = 0; k < 4; k++) ms[k] = acc[k] / iters;
+    for (int k = 0; k < 5; k++) ms[k] = acc[k] / iters;
     return rc;
 }
 
@@ -1122,6 +1159,7 @@ extern "C" size_t mrs_sizeof(const char* name) {
     if (n == "mrs_group_desc") return sizeof(mrs_group_desc);
     if (n == "mrs_slice_desc") return sizeof(mrs_slice_desc);
     if (n == "mrs_unit_desc") return sizeof(mrs_unit_desc);
+    if (n == "mrs_zset") return sizeof(mrs_zset);
     return 0;
 }
 

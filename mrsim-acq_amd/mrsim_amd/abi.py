@@ -157,6 +157,24 @@ class UnitDesc(ct.Structure):
     ]
 
 
+class ZSetDesc(ct.Structure):
+    """mrs_zset: one set of the batched partition transform (device pointers)."""
+    _fields_ = [
+        ("inp", ct.c_void_p),
+        ("out", ct.c_void_p),
+        ("weights", ct.c_void_p),
+        ("comp_stride", ct.c_size_t),
+        ("nz", ct.c_uint32),
+        ("n_comp", ct.c_uint32),
+        ("n_coils", ct.c_uint32),
+        ("ny", ct.c_uint32),
+        ("nx", ct.c_uint32),
+        ("ky_segments", ct.c_uint32),
+        ("accumulate", ct.c_int32),
+        ("reserved", ct.c_int32),
+    ]
+
+
 STRUCT_SIZES = {
     "mrs_acquisition": Acquisition,
     "mrs_t2_slice": T2Slice,
@@ -165,6 +183,7 @@ STRUCT_SIZES = {
     "mrs_group_desc": GroupDesc,
     "mrs_slice_desc": SliceDesc,
     "mrs_unit_desc": UnitDesc,
+    "mrs_zset": ZSetDesc,
 }
 
 

@@ -1,0 +1,182 @@
+"""The 3D gradient-echo train's forward on the GPU: per-partition 2D forward, then the partition
+transform with per-excitation weights, for several volumes (geometries) and echoes in one plan.
+
+Mirrors the reference's `simulate_acquisition_3d_ge_complex` up to the acquired 3D k-space
+(src/kspace3d.rs:732-760 for the refusals, GePlan::{compartments,kspace} at src/kspace3d.rs:643-726
+for the arithmetic): for echo e at TE_e, compartment c and partition z, the 2D forward runs with
+gradient-echo formation at TE_e and the within-echo timing of the GRASE block
+(t, trf = TE_e + t, tread = t, polarity; src/readout.rs:206-221), its decay evaluated per voxel;
+then K3[p] = sum_c w(p, ky % ky_segments, c) sum_z zk[p][z] K2_c[z] (src/kspace3d.rs:90-100, 716-722).
+Where every relaxation input is uniform the reference applies the decay as a line weight after the
+z-DFT instead; the two agree exactly in real arithmetic (exp(-(TE+t)(1/T2 + 1/T2'))).
+
+Output layout per (volume, echo): [p][coil][ky][kx] complex64 (the reference's per-coil
+[p][ky][kx] with the coil axis moved outside ky).  Noise, the inverse z-DFT and reconstruction
+(partitions_to_slices, src/kspace3d.rs:235-279) are not part of this forward.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+from . import abi
+from .gpu import Group, MrsError, Plan, Slice, Unit, ZSet, grase_block
+
+
+@dataclass
+class Ge3dVolume:
+    """One volume of a 3D gradient-echo train (torch device tensors, planes of P = snx*sny).
+
+    images[c]: (nz*P,) float32 or None (a compartment this volume does not use; its weights must be 0)
+    t2[c], t_inhom[c]: float (uniform, ms) or (nz*P,) float32 maps; t_inhom None = acq.t_inhom
+    fmap: (nz*P,) float32 Hz; phase0: (nz*P,) float64 rad or None
+    coil_maps: (nz, C, P) complex64 (sensitivities may vary with the partition)
+    weights: (nz*ky_segments*ncomp,) float32, w[(p*ky_segments + sy)*ncomp + c]
+    """
+    snx: int
+    sny: int
+    nx: int
+    ny: int
+    nz: int
+    images: Sequence[object]
+    t2: Sequence[object]
+    fmap: object
+    coil_maps: object
+    weights: object
+    t_inhom: Optional[Sequence[object]] = None
+    phase0: object = None
+
+
+def _refuse(msg):
+    raise ValueError(msg)
+
+
+def check_volume(v: Ge3dVolume, acq: abi.Acquisition, ky_segments: int):
+    """The reference's assertions on the 3D gradient-echo inputs (src/kspace3d.rs:741-760, 787-797)."""
+    import torch
+    if v.snx % v.nx or v.sny % v.ny:
+        _refuse("sim grid must be an integer multiple of the acquired matrix")
+    if v.snx // v.nx != v.sny // v.ny:
+        _refuse("oversampling must match on both axes")
+    ncomp = len(v.t2)
+    P = v.snx * v.sny
+    nvox = P * v.nz
+    if v.fmap.numel() != nvox:
+        _refuse("fieldmap is not on the simulation grid")
+    if v.t_inhom is not None and len(v.t_inhom) != ncomp:
+        _refuse(f"t_inhom has {len(v.t_inhom)} entries for {ncomp} compartments")
+    if acq.echo != abi.ECHO_GRADIENT:
+        _refuse("the 3D gradient-echo train is a gradient-echo acquisition")
+    if acq.eddy_strength != 0.0 or acq.eddy_quad != 0.0 or acq.eddy_phase != 0.0:
+        _refuse("the eddy model is not available in 3D (an excitation-dependent eddy evolution breaks the z "
+                "factorization)")
+    if acq.partial_fourier < 1.0:
+        _refuse("partial Fourier is not available on the 3D gradient-echo train")
+    if v.nz < 2:
+        _refuse(f"a 3D gradient-echo train on {v.nz} partition is a 2D readout")
+    if len(v.images) != ncomp:
+        _refuse(f"{len(v.images)} compartment images for {ncomp} compartments")
+    for c, im in enumerate(v.images):
+        if im is not None and im.numel() != nvox:
+            _refuse(f"compartment {c} is not on the simulation grid")
+    if v.weights.numel() != v.nz * ky_segments * ncomp:
+        _refuse("weights are not (partition, ky segment, compartment)")
+    w = v.weights.reshape(v.nz * ky_segments, ncomp)
+    for c, im in enumerate(v.images):
+        if im is None and bool(torch.any(w[:, c] != 0)):
+            _refuse(f"compartment {c} has no image but a nonzero weight")
+    if tuple(v.coil_maps.shape[:1]) != (v.nz,) or v.coil_maps.shape[2] != P:
+        _refuse("coil maps are not (partition, coil, sim-grid plane)")
+
+
+class Ge3dForward:
+    """The batched forward of several 3D gradient-echo volumes at several echo times.
+
+    One plan holds a group per (volume, echo), a slice and a unit per (volume, echo, used
+    compartment, partition), and one partition-transform set per (volume, echo) that runs after the
+    2D stages inside the same launch sequence (and the same HIP graph).
+    """
+
+    def __init__(self, volumes: Sequence[Ge3dVolume], acq: abi.Acquisition, echo_times_ms: Sequence[float],
+                 ky_segments: int, t_line_ms: float, precision: int = abi.PREC_SPLIT_BF16):
+        import torch
+        if len(echo_times_ms) == 0:
+            _refuse("a 3D gradient-echo train needs at least one echo time")
+        self.volumes = list(volumes)
+        self.tes = [float(t) for t in echo_times_ms]
+        self.ky_segments = ky_segments
+        groups: List[Group] = []
+        slices: List[Slice] = []
+        units: List[Unit] = []
+        self.k2: List[List[object]] = []
+        self.out: List[List[object]] = []
+        zsets: List[ZSet] = []
+        self._keep = []
+        for vi, v in enumerate(self.volumes):
+            check_volume(v, acq, ky_segments)
+            dev = v.fmap.device
+            P = v.snx * v.sny
+            C = v.coil_maps.shape[1]
+            ncomp = len(v.t2)
+            used = [c for c in range(ncomp) if v.images[c] is not None]
+            t, pol = grase_block(v.ny, ky_segments, t_line_ms, bool(acq.reverse_phase))
+            w_all = v.weights.reshape(v.nz * ky_segments, ncomp)
+            w_used = w_all[:, used].contiguous().reshape(-1) if used else None
+            self._keep.append(w_used)
+            fm = v.fmap.reshape(v.nz, P)
+            ph = None if v.phase0 is None else v.phase0.reshape(v.nz, P)
+            k2_v, out_v = [], []
+            for e, te in enumerate(self.tes):
+                a = acq.copy(echo=abi.ECHO_GRADIENT, t_echo=te, noise_variance=0.0, n_spikes=0)
+                gi = len(groups)
+                groups.append(Group(v.snx, v.sny, v.nx, v.ny, C, 1, a, timing=(t, te + t, t.copy(), pol),
+                                    complex_images=False))
+                k2 = torch.zeros((max(len(used), 1), v.nz, C, v.ny, v.nx), dtype=torch.complex64, device=dev)
+                out = torch.zeros((v.nz, C, v.ny, v.nx), dtype=torch.complex64, device=dev)
+                for ci, c in enumerate(used):
+                    img = v.images[c].reshape(v.nz, P)
+                    t2c, tic = v.t2[c], None if v.t_inhom is None else v.t_inhom[c]
+                    t2m = None if isinstance(t2c, (int, float)) else t2c.reshape(v.nz, P)
+                    tim = None if (tic is None or isinstance(tic, (int, float))) else tic.reshape(v.nz, P)
+                    for z in range(v.nz):
+                        si = len(slices)
+                        slices.append(Slice(gi, fm[z], v.coil_maps[z], phase0=None if ph is None else ph[z],
+                                            t2=None if t2m is None else t2m[z],
+                                            t2_uniform=(float(t2c) if t2m is None else 100.0,) * abi.MRS_MAX_COMP,
+                                            t_inhom=None if tim is None else tim[z],
+                                            t_inhom_uniform=None if (tic is None or tim is not None)
+                                            else (float(tic),) * abi.MRS_MAX_COMP,
+                                            z=z, nz=v.nz))
+                        units.append(Unit(si, img[z], k2[ci, z]))
+                if used:
+                    zsets.append(ZSet(inp=k2, out=out, nz=v.nz, n_coils=C, ny=v.ny, nx=v.nx, n_comp=len(used),
+                                      ky_segments=ky_segments, weights=w_used))
+                k2_v.append(k2)
+                out_v.append(out)
+            self.k2.append(k2_v)
+            self.out.append(out_v)
+        if not units:
+            _refuse("no compartment image in any volume")
+        self.plan = Plan(groups, slices, units, precision)
+        self.plan.set_zstage(zsets)
+        self.zsets = zsets
+
+    def forward(self, stream=None):
+        self.plan.forward(stream)
+        return self.out
+
+    def forward_graph(self, stream=None):
+        self.plan.forward_graph(stream)
+        return self.out
+
+    def tune(self, trials=3):
+        return self.plan.tune(trials)
+
+    def stage_times(self, iters=5):
+        return self.plan.stage_times(iters)
+
+    def close(self):
+        self.plan.close()
+
+
+__all__ = ["Ge3dVolume", "Ge3dForward", "check_volume", "MrsError"]

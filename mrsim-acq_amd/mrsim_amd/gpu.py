@@ -68,6 +68,15 @@ def lib():
     L.mrs_sizeof.argtypes = [ct.c_char_p]
     L.mrs_sizeof.restype = ct.c_size_t
     L.mrs_probe_sincos.argtypes = [P(ct.c_double), P(ct.c_float), P(ct.c_float), ct.c_size_t, ct.c_int32] + E
+    L.mrs_grase_block.argtypes = [ct.c_size_t, ct.c_size_t, ct.c_double, ct.c_int32, P(ct.c_double), P(ct.c_int8)] + E
+    L.mrs_ge3d_lines.argtypes = [ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_double, ct.c_int32, ct.c_size_t,
+                                 ct.c_int32, P(ct.c_int32), P(ct.c_int32), P(ct.c_size_t), P(ct.c_size_t),
+                                 P(ct.c_size_t)] + E
+    L.mrs_check_ge3d_timing.argtypes = [P(ct.c_double), ct.c_size_t, ct.c_double, ct.c_double, ct.c_size_t,
+                                        ct.c_size_t, ct.c_double, ct.c_int32, ct.c_size_t, ct.c_double,
+                                        ct.c_double] + E
+    L.mrs_zstage.argtypes = [P(abi.ZSetDesc), ct.c_size_t, ct.c_int32, ct.c_void_p] + E
+    L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
     _lib = L
     return L
 
@@ -167,6 +176,94 @@ def probe_sincos(cyc: np.ndarray, mode: int = 0):
 
 
 # ----------------------------------------------------------------------------- batched plan
+
+def grase_block(ny: int, ky_segments: int, t_line_ms: float, reverse_phase: bool = False):
+    """mrs_grase_block: (t_ms[ny], polarity[ny]) of a GRASE / 3D-GE block."""
+    t = np.zeros(ny)
+    pol = np.zeros(ny, dtype=np.int8)
+    err = _errbuf()
+    _check(lib().mrs_grase_block(ny, ky_segments, t_line_ms, 1 if reverse_phase else 0, abi.dptr(t),
+                                 pol.ctypes.data_as(ct.POINTER(ct.c_int8)), err, len(err)), err)
+    return t, pol
+
+
+def ge3d_lines(ny: int, nz: int, ky_segments: int, t_line_ms: float, reverse_phase: bool, kz_segments: int,
+               centric: bool):
+    """mrs_ge3d_lines: dict of shot[p*ny+ky], excitation[p*ny+ky], n_shots, n_exc, e_c."""
+    shot = np.zeros(ny * nz, dtype=np.int32)
+    exc = np.zeros(ny * nz, dtype=np.int32)
+    ns, ne, ec = ct.c_size_t(), ct.c_size_t(), ct.c_size_t()
+    err = _errbuf()
+    P = ct.POINTER(ct.c_int32)
+    _check(lib().mrs_ge3d_lines(ny, nz, ky_segments, t_line_ms, 1 if reverse_phase else 0, kz_segments,
+                                1 if centric else 0, shot.ctypes.data_as(P), exc.ctypes.data_as(P), ct.byref(ns),
+                                ct.byref(ne), ct.byref(ec), err, len(err)), err)
+    return {"shot": shot, "excitation": exc, "n_shots": ns.value, "n_exc": ne.value, "e_c": ec.value}
+
+
+def check_ge3d_timing(echo_times_ms, exc_spacing_ms, excitation_time_ms, ny, ky_segments, t_line_ms, reverse_phase,
+                      n_exc, t_exc_ms, tr_ms):
+    """mrs_check_ge3d_timing: raises MrsError with the reference's text when the train does not fit."""
+    tes = np.ascontiguousarray(echo_times_ms, dtype=np.float64)
+    err = _errbuf()
+    _check(lib().mrs_check_ge3d_timing(abi.dptr(tes), len(tes), exc_spacing_ms, excitation_time_ms, ny, ky_segments,
+                                       t_line_ms, 1 if reverse_phase else 0, n_exc, t_exc_ms, tr_ms, err, len(err)),
+           err)
+
+
+@dataclass
+class ZSet:
+    """One set of the batched partition transform (torch complex64 / float32 device tensors).
+
+    forward: out[p] (+)= sum_c w(p, sy, c) sum_z zk[p][z] inp[c][z];  adjoint: out[c][z] (+)= ... (see
+    include/mrsim_gpu.h).  ``inp``/``out`` hold [n_comp][nz][n_coils][ny][nx] on the compartment side and
+    [nz][n_coils][ny][nx] on the other; ``weights`` is [nz][ky_segments][n_comp] or None.
+    """
+    inp: object
+    out: object
+    nz: int
+    n_coils: int
+    ny: int
+    nx: int
+    n_comp: int = 1
+    ky_segments: int = 1
+    weights: object = None
+    accumulate: bool = False
+    comp_stride: Optional[int] = None
+
+
+def _zset_array(sets):
+    arr = (abi.ZSetDesc * max(len(sets), 1))()
+    for i, z in enumerate(sets):
+        d = arr[i]
+        d.inp, d.out = _ptr(z.inp), _ptr(z.out)
+        d.weights = _ptr(z.weights)
+        d.comp_stride = z.comp_stride if z.comp_stride is not None else z.nz * z.n_coils * z.ny * z.nx
+        d.nz, d.n_comp, d.n_coils, d.ny, d.nx, d.ky_segments = z.nz, z.n_comp, z.n_coils, z.ny, z.nx, z.ky_segments
+        d.accumulate = 1 if z.accumulate else 0
+    return arr
+
+
+def zstage(sets: Sequence[ZSet], adjoint: bool = False, stream=None):
+    """mrs_zstage: run the sets on `stream` (torch's current stream by default) and wait."""
+    import torch
+    for i, z in enumerate(sets):
+        ncols = z.n_coils * z.ny * z.nx
+        comp_side = z.n_comp * (z.comp_stride or z.nz * ncols)
+        _check_dev(f"zset {i} inp", z.inp, torch.complex64, None)
+        _check_dev(f"zset {i} out", z.out, torch.complex64, None)
+        need_in = comp_side if not adjoint else z.nz * ncols
+        need_out = z.nz * ncols if not adjoint else comp_side
+        if z.inp.numel() < need_in or z.out.numel() < need_out:
+            raise ValueError(f"zset {i}: buffers smaller than the set's shape")
+        if z.weights is not None:
+            _check_dev(f"zset {i} weights", z.weights, torch.float32, z.nz * z.ky_segments * z.n_comp)
+    arr = _zset_array(sets)
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_zstage(arr, len(sets), 1 if adjoint else 0, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
 
 @dataclass
 class Group:
@@ -350,11 +447,18 @@ class Plan:
         lib().mrs_plan_get_tiles(self._h, ct.byref(a), ct.byref(b))
         return a.value, b.value
 
+    def set_zstage(self, sets: Sequence["ZSet"]):
+        """Attach partition transforms that run after the 2D forward (and inside its graph)."""
+        arr = _zset_array(sets)
+        self._zkeep = list(sets)
+        err = _errbuf()
+        _check(lib().mrs_plan_set_zstage(self._h, arr, len(sets), err, len(err)), err)
+
     def stage_times(self, iters=5, stream=None):
-        ms = (ct.c_double * 4)()
+        ms = (ct.c_double * 5)()
         err = _errbuf()
         _check(lib().mrs_plan_stage_times(self._h, self._stream(stream), iters, ms, err, len(err)), err)
-        return {"prep": ms[0], "ystage": ms[1], "xstage": ms[2], "epilogue": ms[3]}
+        return {"prep": ms[0], "ystage": ms[1], "xstage": ms[2], "epilogue": ms[3], "zstage": ms[4]}
 
     def stats(self):
         t, l, u = ct.c_double(), ct.c_double(), ct.c_double()

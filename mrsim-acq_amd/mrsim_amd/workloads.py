@@ -209,3 +209,130 @@ def to_plan(w: Workload, precision=abi.PREC_SPLIT_BF16, device="cuda"):
         units.append(Unit(si, ti, out))
     plan = Plan([g], slices, units, precision)
     return plan, outs, keep
+
+
+# ---------------------------------------------------------------------------- 3D gradient-echo train
+
+@dataclass
+class Ge3dVolumeData:
+    snx: int
+    sny: int
+    nx: int
+    ny: int
+    nz: int
+    images: list            # per compartment (nz, P) float32
+    t2: list                # per compartment (nz, P) float32 map
+    t_inhom: list           # per compartment float (uniform, ms)
+    fmap: np.ndarray        # (nz, P) float32
+    coil_maps: np.ndarray   # (nz, C, P) complex64
+    weights: np.ndarray     # (nz, ky_segments, Q) float64
+
+
+@dataclass
+class Ge3dWorkload:
+    name: str
+    acq: abi.Acquisition
+    echo_times_ms: tuple
+    ky_segments: int
+    t_line_ms: float
+    kz_segments: int
+    centric: bool
+    exc_spacing_ms: float
+    excitation_time_ms: float
+    t_exc_ms: float
+    tr_ms: float
+    n_coils: int
+    volumes: list = field(default_factory=list)
+
+
+def _blobs3d(st: Stream, snx, sny, nz, nblobs=8):
+    """Real positive object: Gaussian blobs in (x, y, z), sigma U(0.06, 0.18) of the field of view."""
+    y, x = np.mgrid[0:sny, 0:snx].astype(np.float64)
+    out = np.zeros((nz, sny * snx))
+    for _ in range(nblobs):
+        a, s = st.uniform(0.3, 1.0), st.uniform(0.06, 0.18)
+        cx, cy, cz = st.uniform(0.25, 0.75) * snx, st.uniform(0.25, 0.75) * sny, st.uniform(0.2, 0.8) * nz
+        sz = st.uniform(0.15, 0.5) * nz
+        plane = np.exp(-((x - cx) ** 2 / (2 * (s * snx) ** 2) + (y - cy) ** 2 / (2 * (s * sny) ** 2))).reshape(-1)
+        zg = np.exp(-((np.arange(nz) - cz) ** 2) / (2 * sz * sz))
+        out += a * zg[:, None] * plane[None, :]
+    return out.astype(np.float32)
+
+
+def _ge3d_volume(st: Stream, snx, sny, nx, ny, nz, C, Q, kseg, lines, exc_spacing, seed_gain):
+    P = snx * sny
+    images = [_blobs3d(st, snx, sny, nz) for _ in range(Q)]
+    t2 = [st.uniform(40, 120, nz * P).reshape(nz, P).astype(np.float32) for _ in range(Q)]
+    fm2 = fieldmap(st, snx, sny).astype(np.float64)
+    fmap = np.stack([fm2 * (1.0 - 0.3 * z / nz) + 2.0 * z for z in range(nz)]).astype(np.float32)
+    # two rings of C/2 coils stacked along z, Gaussian in z around nz/4 and 3nz/4
+    per_ring = max(C // 2, 1)
+    rings = max(C // per_ring, 1)
+    plane = np.concatenate([gaussian_ring_coils(per_ring, snx, sny, 0.45 * snx, 0.7 * snx) for _ in range(rings)])
+    zc = np.array([(r + 0.5) * nz / rings - 0.5 for r in range(rings)])
+    zz = np.arange(nz, dtype=np.float64)
+    gain = np.exp(-((zz[:, None] - zc[None, :]) ** 2) / (2 * (nz / 3.0) ** 2))      # (nz, rings)
+    zgain = np.repeat(gain, per_ring, axis=1)[:, :C]
+    coil_maps = (zgain[:, :, None] * plane[None, :C, :]).astype(np.complex64)
+    # per-excitation weights: approach to the steady state over the shot's excitations,
+    # w = M_ss + (1 - M_ss) exp(-n TR_exc / T1*), times a per-shot gain
+    m_ss = st.uniform(0.2, 0.4)
+    t1s = 1100.0
+    gains = 1.0 + 0.02 * Stream(seed_gain).gauss(int(lines["n_shots"]))
+    w = np.zeros((nz, kseg, Q))
+    for p in range(nz):
+        for sy in range(kseg):
+            n = lines["excitation"][p * ny + sy]
+            sh = lines["shot"][p * ny + sy]
+            for q in range(Q):
+                w[p, sy, q] = (m_ss + (1.0 - m_ss) * np.exp(-n * exc_spacing / t1s)) * gains[sh] * (1.0 - 0.1 * q)
+    return Ge3dVolumeData(snx, sny, nx, ny, nz, images, t2, [45.0] * Q, fmap, coil_maps, w)
+
+
+def make_ge3d(name: str = "ge3d", seed_base=0x5EED3D00) -> Ge3dWorkload:
+    """3D gradient-echo trains.
+
+    ge3d        two volumes, 128^2 -> 64^2 and 192^2 -> 96^2, 24 partitions each, 32 coils (two
+                rings of 16 along z), 1 compartment, echoes at 12/24/36/48 ms, 4 ky segments of
+                0.4 ms lines, 2 kz segments in centric order, 60 ms excitation spacing
+    ge3d_small  the same train on 24^2 -> 12^2 (4 partitions) and 32^2 -> 16^2 (6 partitions),
+                4 coils, 2 compartments, echoes at 12/24 ms, 2 ky segments
+    """
+    from .gpu import check_ge3d_timing, ge3d_lines
+    if name == "ge3d":
+        geoms, C, Q, tes, kseg, kzs = [(128, 64, 24), (192, 96, 24)], 32, 1, (12.0, 24.0, 36.0, 48.0), 4, 2
+    elif name == "ge3d_small":
+        geoms, C, Q, tes, kseg, kzs = [(24, 12, 4), (32, 16, 6)], 4, 2, (12.0, 24.0), 2, 2
+    else:
+        raise KeyError(name)
+    acq = abi.Acquisition.default(echo=abi.ECHO_GRADIENT, t_line=0.4, signal_scale=1.0, n_coils=C, t_inhom=45.0)
+    w = Ge3dWorkload(name, acq, tes, kseg, 0.4, kzs, True, 60.0, 2.0, 10.0, 1000.0, C)
+    for i, (sn, n, nz) in enumerate(geoms):
+        lines = ge3d_lines(n, nz, kseg, w.t_line_ms, False, kzs, w.centric)
+        check_ge3d_timing(list(tes), w.exc_spacing_ms, w.excitation_time_ms, n, kseg, w.t_line_ms, False,
+                          lines["n_exc"], w.t_exc_ms, w.tr_ms)
+        st = Stream(seed_base + 101 * i)
+        w.volumes.append(_ge3d_volume(st, sn, sn, n, n, nz, C, Q, kseg, lines, w.exc_spacing_ms,
+                                      seed_base + 101 * i + 1))
+    return w
+
+
+def to_ge3d(w: Ge3dWorkload, precision=abi.PREC_SPLIT_BF16, device="cuda"):
+    """Upload a 3D gradient-echo workload and build its batched forward."""
+    import torch
+
+    from .ge3d import Ge3dForward, Ge3dVolume
+
+    def dev(a, dt=None):
+        t = torch.from_numpy(np.ascontiguousarray(a))
+        return (t if dt is None else t.to(dt)).to(device)
+
+    vols = []
+    for v in w.volumes:
+        vols.append(Ge3dVolume(v.snx, v.sny, v.nx, v.ny, v.nz,
+                               images=[dev(im.reshape(-1)) for im in v.images],
+                               t2=[dev(t.reshape(-1)) for t in v.t2], fmap=dev(v.fmap.reshape(-1)),
+                               coil_maps=dev(v.coil_maps), weights=dev(v.weights.reshape(-1), torch.float32),
+                               t_inhom=list(v.t_inhom)))
+    fw = Ge3dForward(vols, w.acq, w.echo_times_ms, w.ky_segments, w.t_line_ms, precision)
+    return fw, vols

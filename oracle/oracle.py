@@ -224,6 +224,12 @@ def forward_direct(snx, sny, nx, ny, n_coils, n_comp, acq, images, fmap, coil_ma
         keep.append(a)
         return a.ctypes.data
 
+    P = snx * sny
+    for name, a, n in (("coil_maps", coil_maps, n_coils * P), ("fmap", fmap, P), ("images", images, n_comp * P),
+                       ("phase0", phase0, P), ("t2", t2, n_comp * P), ("t_inhom", t_inhom, n_comp * P),
+                       ("log_amp", log_amp, P)):
+        if a is not None and np.size(a) != n:
+            raise ValueError(f"forward_direct: {name} has {np.size(a)} values, expected {n}")
     g = abi.GroupDesc()
     g.snx, g.sny, g.nx, g.ny, g.n_coils, g.n_compartments = snx, sny, nx, ny, n_coils, n_comp
     g.complex_images = 1 if complex_images else 0

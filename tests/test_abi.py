@@ -78,3 +78,76 @@ def test_refusals_carry_the_reference_messages(L):
         gpu.build_coil_kspace(ok, abi.Acquisition.default(), timing=t)
     with pytest.raises(gpu.MrsError, match="out of range"):
         gpu.build_coil_kspace(ok, abi.Acquisition.default(), coil=4, ncoils=4)
+
+
+# ------------------------------------------------------------------ 3D gradient-echo host tables
+
+def test_grase_block_matches_oracle(L):
+    from oracle import ge3d as G
+    for ny, kseg, tl in [(20, 2, 1.0), (64, 4, 0.2048), (16, 2, 0.5), (96, 4, 0.4), (7, 1, 0.3), (1, 1, 0.0001)]:
+        for rev in (False, True):
+            t, pol = gpu.grase_block(ny, kseg, tl, rev)
+            tw, pw, _ = G.grase_block(ny, kseg, tl, rev)
+            assert np.array_equal(t, tw) and np.array_equal(pol, pw), (ny, kseg, tl, rev)
+    with pytest.raises(gpu.MrsError, match="69 phase-encode lines do not divide into 4 ky segments"):
+        gpu.grase_block(69, 4, 0.2)
+
+
+@pytest.mark.parametrize("centric", [False, True])
+@pytest.mark.parametrize("ny,nz,kseg,kzs", [(16, 8, 2, 1), (16, 8, 2, 2), (16, 8, 2, 8), (64, 24, 4, 1),
+                                            (96, 24, 4, 3), (12, 5, 3, 5)])
+def test_ge3d_lines_match_oracle(L, centric, ny, nz, kseg, kzs):
+    from oracle import ge3d as G
+    a = gpu.ge3d_lines(ny, nz, kseg, 0.4, False, kzs, centric)
+    b = G.ge3d_lines(ny, nz, kseg, 0.4, False, kzs, centric)
+    for k in ("shot", "excitation"):
+        assert np.array_equal(a[k], b[k]), k
+    for k in ("n_shots", "n_exc", "e_c"):
+        assert a[k] == b[k], k
+
+
+def test_ge3d_line_refusals(L):
+    for args, msg in [((16, 8, 2, 0.5, False, 3, False), "8 partitions do not divide into 3 kz segments"),
+                      ((16, 1, 2, 0.5, False, 1, False), "a 3D gradient-echo train on 1 partition is a 2D readout"),
+                      ((16, 8, 3, 0.5, False, 1, False), "16 phase-encode lines do not divide into 3 ky segments")]:
+        with pytest.raises(gpu.MrsError, match=msg):
+            gpu.ge3d_lines(*args)
+
+
+def test_check_ge3d_timing_messages_match_oracle(L):
+    # the reference's timing cases (src/readout.rs:1043-1088) and the config-4 train: the library
+    # and the oracle accept the same trains and refuse the others with the same text
+    from oracle import ge3d as G
+    cases = [([12.0, 22.0], 40.0, 2000.0, 4), ([2.5], 40.0, 2000.0, 4), ([3.0], 40.0, 2000.0, 4),
+             ([12.0, 15.5], 40.0, 2000.0, 4), ([12.0, 16.0], 40.0, 2000.0, 4), ([12.0, 32.0], 34.9, 2000.0, 4),
+             ([12.0, 32.0], 35.0, 2000.0, 4), ([12.0], 40.0, 1133.9, 4), ([12.0], 40.0, 1134.0, 4),
+             ([12.0], 10.0, 2000.0, 1), ([12.0], 10.0, 1013.9, 1), ([], 40.0, 2000.0, 4)]
+    for tes, spacing, tr, n_exc in cases:
+        want = None
+        try:
+            G.check_ge3d_timing(tes, spacing, 2.0, 16, 2, 0.5, False, n_exc, 1000.0, tr)
+        except G.TimingError as e:
+            want = str(e)
+        got = None
+        try:
+            gpu.check_ge3d_timing(tes, spacing, 2.0, 16, 2, 0.5, False, n_exc, 1000.0, tr)
+        except gpu.MrsError as e:
+            got = str(e).split(": ", 1)[1]
+        assert got == want, (tes, spacing, tr)
+    for ny in (64, 96):
+        gpu.check_ge3d_timing([12.0, 24.0, 36.0, 48.0], 60.0, 2.0, ny, 4, 0.4, False, 24, 10.0, 1500.0)
+
+
+def test_zstage_refusals_before_any_device_work(L):
+    import torch
+    z = abi.ZSetDesc()
+    z.inp, z.out = 16, 32          # never dereferenced: validation refuses first
+    z.nz, z.n_comp, z.n_coils, z.ny, z.nx, z.ky_segments = 65, 1, 1, 4, 4, 1
+    err = ct.create_string_buffer(256)
+    assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -4 and b"1..64" in err.value
+    z.nz, z.ky_segments = 4, 3
+    assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -1 and b"ky segments" in err.value
+    z.ky_segments, z.out = 1, 16
+    assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -1 and b"not in place" in err.value
+    z.out, z.n_comp, z.comp_stride = 32, 2, 10
+    assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -1 and b"overlap" in err.value
