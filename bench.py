@@ -121,6 +121,49 @@ def grad_pass(iters: int):
             "ratio": round(t_both / t_fwd, 3), "fit50_ms": round(1e3 * t_fit, 2), "fit50_final_loss": l_end}
 
 
+def ge3d_pass(steps: int, check: bool = True):
+    """The 3D gradient-echo train (two volumes, 128^2 -> 64^2 and 192^2 -> 96^2, 24 partitions each,
+    32 coils, 1 compartment, 4 echoes): one hipGraph replay per step runs every per-partition 2D
+    forward and both volumes' partition transforms with their per-excitation weights."""
+    import numpy as np
+    import torch
+
+    from mrsim_amd import workloads
+    w = workloads.make_ge3d("ge3d")
+    fw, keep = workloads.to_ge3d(w)
+    log(fw.tune(2).strip().splitlines()[-1])
+    log(fw.describe().strip())
+    for _ in range(2):
+        fw.forward_graph()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fw.forward_graph()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = fw.stage_times(iters=3)
+    n_e = len(w.echo_times_ms)
+    parts = sum(v.nz for v in w.volumes) * n_e
+    # partition transform traffic: read K2 and write K3, complex64, every (volume, echo)
+    zbytes = sum(2 * 8 * v.nz * w.n_coils * v.ny * v.nx for v in w.volumes) * n_e
+    err = None
+    if check:
+        from oracle import ge3d as G
+        v = w.volumes[0]
+        want = G.ge3d_kspace(v.snx, v.sny, v.nx, v.ny, v.nz, v.images, v.t2, v.t_inhom, v.fmap, None,
+                             v.coil_maps[:, [7]], w.acq, w.echo_times_ms[0], w.ky_segments, w.t_line_ms,
+                             v.weights.astype(np.float32))[:, 0]
+        got = fw.out[0][0][:, 7].cpu().numpy()
+        err = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    fw.close()
+    return {"workload": "ge3d: 2 volumes (128^2->64^2, 192^2->96^2) x 24 partitions x 4 echoes (TE 12/24/36/48), "
+                        "32 coils in 2 rings along z, 4 ky segments, per-excitation weights",
+            "ms_per_train": round(1e3 * dt, 4), "partition_slices_per_s": round(parts / dt, 1),
+            "volumes_x_echoes_per_s": round(len(w.volumes) * n_e / dt, 2), "stage_ms": st,
+            "zstage_gbs": round(zbytes / (st["zstage"] * 1e-3) / 1e9, 1) if st["zstage"] > 0 else None,
+            "rel_l2_vs_fp64_coil7_vol0_echo0": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +174,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-ge3d", action="store_true")
+    ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
     ap.add_argument("--check-units", type=int, default=2)
     ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles (skips the autotuner)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -154,6 +199,9 @@ def main():
 
     from mrsim_amd import abi, workloads
     torch.cuda.set_device(0)
+    if args.only_ge3d:
+        print(json.dumps({"ge3d": ge3d_pass(args.steps, check=args.check_units > 0)}), flush=True)
+        return
     t_gen = time.perf_counter()
     w = workloads.make("batch_128", n_slices=args.slices, n_volumes=args.volumes)
     log(f"workload generated in {time.perf_counter() - t_gen:.1f} s: {len(w.units)} units")
@@ -213,6 +261,9 @@ def main():
     grad = None if args.no_grad else grad_pass(max(5, args.steps))
     if grad:
         log(f"gradient pass: {grad}")
+    ge3d = None if args.no_ge3d else ge3d_pass(max(3, args.steps // 4), check=args.check_units > 0)
+    if ge3d:
+        log(f"3D gradient-echo train: {ge3d}")
 
     stats = plan.stats()
     terms = stats["terms"]
@@ -266,6 +317,7 @@ def main():
         "stage_ms": stage_ms,
         "tiles": plan.tiles(),
         "grad_pass": grad,
+        "ge3d": ge3d,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -222,6 +222,9 @@ int mrs_plan_vjp(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* gr
 int mrs_plan_stage_times(mrs_plan* plan, void* stream, int32_t iters, double* ms, char* err, size_t errlen);
 /* Terms of the literal sum per batch (sum of n_coils*nx*Ly*snx*sny) and acquired lines. */
 int mrs_plan_stats(const mrs_plan* plan, double* terms, double* lines, double* units);
+/* Human-readable layout of a plan (groups, lines, chain passes, coil-plane sharing, tiles).  No
+ * counterpart in the reference: an inspection aid for callers tuning batches. */
+int mrs_plan_describe(const mrs_plan* plan, char* out, size_t outlen);
 int mrs_plan_destroy(mrs_plan* plan);
 
 /* Kernel tile parameters chosen by the autotuner (see mrs_plan_tune). */

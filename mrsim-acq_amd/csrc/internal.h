@@ -54,7 +54,11 @@ struct DevSlice {
     int32_t z, nz;
     int32_t has_eddy;
     double eddy[3];
+    int32_t s_owner;   // this slice writes the coil planes; otherwise they alias an earlier slice's
 };
+
+// Slice prep outputs: the split coil planes (MFMA forward) and/or the FP32 ones (plain, adjoint).
+enum { PREP_SPLIT = 1, PREP_S32 = 2 };
 
 struct DevGroup {
     int32_t snx, sny, nx, ny;

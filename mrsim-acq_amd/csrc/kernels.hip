@@ -129,7 +129,7 @@ __device__ __forceinline__ void sincos_turns(double cyc, float& s, float& c) {
 // slice: user planes (x fastest) are read coalesced along x, staged in LDS, and written in the
 // kernels' [x][y] order coalesced along y.  grid = (tiles_x * tiles_y, slices).
 __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices, const DevGroup* __restrict__ groups,
-                                                   int nslices) {
+                                                   int nslices, int mode) {
     __shared__ double sd[2][32][33];
     __shared__ float sf[1 + 2 * MRS_MAX_COMP][32][33];
     __shared__ float2 sc[32][33];
@@ -213,7 +213,10 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
                         __builtin_fmaf(gr.rot_dtrf, sf[1 + 2 * q][xl][wy], gr.rot_dtau[c] * sf[2 + 2 * q][xl][wy]));
         }
     }
-    const int Cp = gr.RB * 8;   // split maps padded with zero coils to whole 8-coil row blocks
+    // coil planes: written once per distinct (maps, amplitude) set, only in the forms asked for
+    if (!sl.s_owner) return;
+    const bool w_split = mode & PREP_SPLIT, w_s32 = mode & PREP_S32;
+    const int Cp = w_split ? gr.RB * 8 : (w_s32 ? C : 0);   // split maps padded with zero coils to 8-coil row blocks
     for (int c = 0; c < Cp; c++) {
         __syncthreads();
         for (int k = 0; k < 4; k++) {
@@ -232,12 +235,15 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
             const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
             if (x >= snxP || y >= snyP) continue;
             const float2 v = sc[xl][wy];
-            uint32_t hi, lo;
-            split2(v.x, v.y, hi, lo);
-            const size_t so = ((size_t)x * Cp + c) * snyP + y;
-            gst(sl.s_hi)[so] = hi;
-            gst(sl.s_lo)[so] = lo;
-            if (c < C) reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
+            if (w_split) {
+                uint32_t hi, lo;
+                split2(v.x, v.y, hi, lo);
+                const size_t so = ((size_t)x * Cp + c) * snyP + y;
+                gst(sl.s_hi)[so] = hi;
+                gst(sl.s_lo)[so] = lo;
+            }
+            if (w_s32 && c < C)
+                reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
         }
     }
 }
@@ -325,12 +331,15 @@ __device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, doubl
 //   G[(coil, part), line] = sum_{y, part'} A[(coil,part), (y,part')] * B[(y,part'), line]
 // with A the split coil sensitivities and B = E generated in registers, 16 lines per MFMA tile,
 // GP tiles per pass and RB 16-row blocks (8 coils each).
-// Occupancy bound: 4 waves/SIMD (128 VGPRs) where the instantiation fits without spilling, else 3.
-template <int Q, int RB, bool CPLX, bool EDDY>
-constexpr int ystage_waves() { return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : 3; }
+// Occupancy bound: 4 waves/SIMD (128 VGPRs) where the instantiation fits without spilling, 2 for the
+// 16-accumulator tiles of many coils over 64 lines, else 3.
+template <int Q, int RB, int GP, bool CPLX, bool EDDY>
+constexpr int ystage_waves() {
+    return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : (RB * GP >= 8 ? 2 : 3);
+}
 
 template <int Q, int RB, int GP, bool CPLX, bool EDDY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_waves<Q, RB, CPLX, EDDY>())))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_waves<Q, RB, GP, CPLX, EDDY>())))
 void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
                  const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work, int xt) {
     __shared__ LineRec lines_s[256];
@@ -1082,9 +1091,9 @@ __global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__
 namespace mrs {
 
 // `maxtiles`: the largest number of 32 x 32 tiles of any slice's padded grid.
-hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxtiles, hipStream_t st) {
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxtiles, int mode, hipStream_t st) {
     dim3 grid(maxtiles, nslices);
-    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, nslices);
+    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, nslices, mode);
     return hipGetLastError();
 }
 
@@ -1105,9 +1114,15 @@ static hipError_t ylaunch_rb(int rb, int gp, dim3 grid, hipStream_t st, const De
         if (gp == 2) return ylaunch<Q, 1, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
         return ylaunch<Q, 1, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
     case 2:
+        if (gp >= 4) return ylaunch<Q, 2, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
         if (gp >= 2) return ylaunch<Q, 2, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
         return ylaunch<Q, 2, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-    case 4: return ylaunch<Q, 4, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+    case 4:
+        // 16 accumulator tiles fit without spilling for up to two compartments
+        if constexpr (Q <= 2) {
+            if (gp >= 4) return ylaunch<Q, 4, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        }
+        return ylaunch<Q, 4, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
     case 8: return ylaunch<Q, 8, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
     default: return hipErrorInvalidValue;
     }

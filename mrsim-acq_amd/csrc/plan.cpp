@@ -16,7 +16,7 @@
 #include "internal.h"
 
 namespace mrs {
-hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, hipStream_t st);
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, int mode, hipStream_t st);
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st);
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
@@ -244,6 +244,9 @@ struct mrs_plan {
     std::vector<VjpUnit> vjp_units;
     VjpUnit* d_vjp_units = nullptr;
     bool vjp_fresh = true;
+    // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
+    std::vector<std::array<void*, 3>> s_alloc;
+    std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
     // partition transform run after the 2D forward (3D trains)
     std::vector<ZSet> zsets;
     void* d_zmem = nullptr;
@@ -277,6 +280,9 @@ static int plan_alloc(mrs_plan* p, T** ptr, size_t n, char* err, size_t errlen) 
 // Reorder work so that the hardware's round-robin dispatch of consecutive blocks over the 8 XCDs
 // gives each XCD a contiguous run of items (units of one slice share that XCD's L2).
 static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w);
+static std::vector<size_t> unit_order(const mrs_plan* p);
+static std::vector<std::vector<WorkItem>> ylist(const mrs_plan* p, const std::vector<size_t>& order, int Q, int RB,
+                                                int cplx, int eddy, int xt);
 
 // Concatenate per-chunk item lists, each XCD-ordered; off[k] is chunk k's first item.
 static std::vector<WorkItem> chunked(const std::vector<std::vector<WorkItem>>& per, std::vector<size_t>& off) {
@@ -316,36 +322,50 @@ static bool is_chain(const LineRec* c) {
 }
 
 // Order one x-stage table segment: full passes of 16 chains first (line 16*(4p+e)+ln is element e
-// of chain 16p+ln), then the remaining lines by ky.  Chains come from runs of same-parity lines on
-// one side of the echo.  Returns the number of chain passes.
+// of chain 16p+ln), then the remaining lines by ky.  A chain is four lines ky, ky+s, ky+2s, ky+3s on
+// one side of the echo whose (t, kyn, d, trf, tau) are arithmetic progressions.  The ky stride s is
+// the one that puts the most lines into chains (2 first: EPI lines of one readout polarity; 1 for
+// blocks whose interleaved lines share a time; wider strides for segmented blocks).  Returns the
+// number of chain passes.
 static int order_segment(std::vector<LineRec>& seg) {
-    std::map<std::pair<int, int>, std::vector<LineRec>> cls;
-    for (const LineRec& L : seg) cls[std::make_pair(L.t_s >= 0.0 ? 0 : 1, L.ky & 1)].push_back(L);
-    std::vector<std::array<LineRec, 4>> chains;
-    std::vector<LineRec> rest;
-    for (auto& kv : cls) {
-        std::vector<LineRec>& v = kv.second;
-        size_t i = 0;
-        while (i < v.size()) {
-            size_t j = i + 1;
-            while (j < v.size() && v[j].ky == v[j - 1].ky + 2) j++;
-            size_t k = i;
-            for (; k + 4 <= j; k += 4) {
-                std::array<LineRec, 4> c{v[k], v[k + 1], v[k + 2], v[k + 3]};
-                if (is_chain(c.data())) chains.push_back(c);
-                else rest.insert(rest.end(), c.begin(), c.end());
+    std::vector<std::array<LineRec, 4>> best_chains;
+    std::vector<LineRec> best_rest = seg;
+    for (int stride : {2, 1, 4, 3, 8}) {
+        std::map<std::pair<int, int>, std::vector<LineRec>> cls;
+        for (const LineRec& L : seg) cls[std::make_pair(L.t_s >= 0.0 ? 0 : 1, L.ky % stride)].push_back(L);
+        std::vector<std::array<LineRec, 4>> chains;
+        std::vector<LineRec> rest;
+        for (auto& kv : cls) {
+            std::vector<LineRec>& v = kv.second;
+            std::sort(v.begin(), v.end(), [](const LineRec& x, const LineRec& y) { return x.ky < y.ky; });
+            size_t i = 0;
+            while (i < v.size()) {
+                size_t j = i + 1;
+                while (j < v.size() && v[j].ky == v[j - 1].ky + stride) j++;
+                size_t k = i;
+                for (; k + 4 <= j; k += 4) {
+                    std::array<LineRec, 4> c{v[k], v[k + 1], v[k + 2], v[k + 3]};
+                    if (is_chain(c.data())) chains.push_back(c);
+                    else rest.insert(rest.end(), c.begin(), c.end());
+                }
+                for (; k < j; k++) rest.push_back(v[k]);
+                i = j;
             }
-            for (; k < j; k++) rest.push_back(v[k]);
-            i = j;
+        }
+        if (chains.size() / 16 > best_chains.size() / 16) {
+            best_chains = chains;
+            best_rest = rest;
         }
     }
-    const size_t npass = chains.size() / 16;
-    for (size_t c = npass * 16; c < chains.size(); c++) rest.insert(rest.end(), chains[c].begin(), chains[c].end());
+    const size_t npass = best_chains.size() / 16;
+    std::vector<LineRec> rest = best_rest;
+    for (size_t c = npass * 16; c < best_chains.size(); c++)
+        rest.insert(rest.end(), best_chains[c].begin(), best_chains[c].end());
     std::sort(rest.begin(), rest.end(), [](const LineRec& a, const LineRec& b) { return a.ky < b.ky; });
     seg.clear();
     for (size_t pss = 0; pss < npass; pss++)
         for (int e = 0; e < 4; e++)
-            for (int ln = 0; ln < 16; ln++) seg.push_back(chains[pss * 16 + ln][e]);
+            for (int ln = 0; ln < 16; ln++) seg.push_back(best_chains[pss * 16 + ln][e]);
     seg.insert(seg.end(), rest.begin(), rest.end());
     return (int)npass;
 }
@@ -533,7 +553,31 @@ static void fill_slice_inputs(DevSlice& s, const mrs_slice_desc& d) {
     for (int i = 0; i < 3; i++) s.eddy[i] = d.eddy_drive[i];
 }
 
+static int fwd_prep_mode(const mrs_plan* p) { return p->precision == MRS_PREC_SPLIT_BF16 ? PREP_SPLIT : PREP_S32; }
+
+// Slices whose coil planes would be identical (same maps and log-amplitude buffers, same amplitude
+// scale, geometry and coil count) share the first such slice's planes, so the prep writes them once.
+static void share_coil_planes(mrs_plan* p) {
+    std::map<std::tuple<const void*, const void*, double, int, int, int, int>, size_t> owner;
+    for (size_t s = 0; s < p->slices.size(); s++) {
+        DevSlice& sl = p->slices[s];
+        const DevGroup& dg = p->groups[sl.group].dev;
+        const auto key = std::make_tuple((const void*)sl.coil_maps, (const void*)sl.log_amp, dg.signal_scale, dg.snx,
+                                         dg.sny, dg.C, dg.RB);
+        auto it = owner.find(key);
+        const size_t o = it == owner.end() ? s : it->second;
+        if (it == owner.end()) owner[key] = s;
+        sl.s_owner = o == s;
+        p->s_owner.resize(p->slices.size());
+        p->s_owner[s] = o;
+        sl.s_hi = (uint32_t*)p->s_alloc[o][0];
+        sl.s_lo = (uint32_t*)p->s_alloc[o][1];
+        sl.s32 = (float2*)p->s_alloc[o][2];
+    }
+}
+
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
+    share_coil_planes(p);
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -637,6 +681,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         if ((rc = plan_alloc(p, &sl.s_hi, PP * dg.RB * 8, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.s_lo, PP * dg.RB * 8, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.s32, PP * dg.C, err, errlen))) break;
+        p->s_alloc.push_back({(void*)sl.s_hi, (void*)sl.s_lo, (void*)sl.s32});
         if ((rc = plan_alloc(p, &sl.rot, dg.rot_shared ? PP : 1, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.rq, dg.rot_shared ? 2 * PP * dg.Q : 1, err, errlen))) break;
     }
@@ -689,11 +734,9 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
     }
     if (rc == MRS_OK) rc = upload_descs(p, err, errlen);
     if (rc == MRS_OK) {
-        // work lists: units sorted by slice so an XCD's run covers the volumes of few slices
-        std::vector<size_t> order(nunits);
-        for (size_t u = 0; u < nunits; u++) order[u] = u;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](size_t a, size_t b) { return units[a].slice < units[b].slice; });
+        // work lists: units grouped by shared coil planes, then slice, so an XCD's run covers the
+        // volumes and echoes of few slices
+        const std::vector<size_t> order = unit_order(p);
         p->nchunks = nunits >= 64 ? 4 : 1;
         p->unit_chunk.assign(nunits, 0);
         for (size_t k = 0; k < nunits; k++) p->unit_chunk[order[k]] = (int)(k * p->nchunks / nunits);
@@ -732,8 +775,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const DevGroup& dg = p->groups[gi].dev;
             const size_t c = cls[std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0)];
             const int ch = p->unit_chunk[u];
-            const int ntile = (dg.snx + p->xt - 1) / p->xt;
-            for (int t = 0; t < ntile; t++) yw[c][ch].push_back({(int32_t)u, t});
+            (void)c;
             const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi]);
             if (!xcls.count(xkey)) {
                 xcls[xkey] = p->xclasses.size();
@@ -746,6 +788,10 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             for (int t = 0; t < nkxb * g_np[gi]; t++) xw[xi][ch].push_back({(int32_t)u, t});
         }
         for (size_t c = 0; c < p->xclasses.size(); c++) p->xclasses[c].items = chunked(xw[c], p->xclasses[c].off);
+        for (size_t c = 0; c < p->yclasses.size(); c++) {
+            const auto& yc = p->yclasses[c];
+            yw[c] = ylist(p, order, yc.Q, yc.RB, yc.cplx, yc.eddy, p->xt);
+        }
         for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
             p->yclasses[c].items = chunked(yw[c], p->yclasses[c].off);
             rc = plan_alloc(p, &p->yclasses[c].d_items, p->yclasses[c].items.size(), err, errlen);
@@ -778,22 +824,51 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
 }
 
 // Rebuild the y-stage work lists for a new x tile (autotuner).
+// Units in work order: grouped by the coil planes they read, then by slice.  The y-stage items of
+// units sharing coil planes are interleaved per x tile, so blocks reading the same plane columns run
+// side by side on one XCD and share its L2.
+static std::vector<size_t> unit_order(const mrs_plan* p) {
+    std::vector<size_t> order(p->units.size());
+    for (size_t u = 0; u < order.size(); u++) order[u] = u;
+    auto own = [&](size_t u) { return p->s_owner[p->units[u].slice]; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return std::make_pair(own(a), p->units[a].slice) < std::make_pair(own(b), p->units[b].slice);
+    });
+    return order;
+}
+
+static std::vector<std::vector<WorkItem>> ylist(const mrs_plan* p, const std::vector<size_t>& order, int Q, int RB,
+                                                int cplx, int eddy, int xt) {
+    std::vector<std::vector<WorkItem>> per(p->nchunks);
+    size_t i = 0;
+    while (i < order.size()) {
+        size_t j = i;
+        const size_t own = p->s_owner[p->units[order[i]].slice];
+        const int ch = p->unit_chunk[order[i]];
+        while (j < order.size() && p->s_owner[p->units[order[j]].slice] == own && p->unit_chunk[order[j]] == ch) j++;
+        int ntile = 0;
+        for (size_t k = i; k < j; k++) {
+            const DevGroup& dg = p->groups[p->units[order[k]].group].dev;
+            ntile = std::max(ntile, (dg.snx + xt - 1) / xt);
+        }
+        for (int t = 0; t < ntile; t++)
+            for (size_t k = i; k < j; k++) {
+                const size_t u = order[k];
+                const DevGroup& dg = p->groups[p->units[u].group].dev;
+                if (dg.Q != Q || dg.RB != RB || dg.cplx != cplx || (dg.eddy_strength != 0.0 ? 1 : 0) != eddy) continue;
+                if (t < (dg.snx + xt - 1) / xt) per[ch].push_back({(int32_t)u, t});
+            }
+        i = j;
+    }
+    return per;
+}
+
 static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
     p->xt = xt;
     p->gp = gp;
+    const std::vector<size_t> order = unit_order(p);
     for (auto& c : p->yclasses) {
-        std::vector<std::vector<WorkItem>> per(p->nchunks);
-        std::vector<size_t> order(p->units.size());
-        for (size_t u = 0; u < order.size(); u++) order[u] = u;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](size_t a, size_t b) { return p->units[a].slice < p->units[b].slice; });
-        for (size_t u : order) {
-            const DevGroup& dg = p->groups[p->units[u].group].dev;
-            if (dg.Q != c.Q || dg.RB != c.RB || dg.cplx != c.cplx || (dg.eddy_strength != 0.0) != (c.eddy != 0))
-                continue;
-            const int ntile = (dg.snx + xt - 1) / xt;
-            for (int t = 0; t < ntile; t++) per[p->unit_chunk[u]].push_back({(int32_t)u, t});
-        }
+        std::vector<std::vector<WorkItem>> per = ylist(p, order, c.Q, c.RB, c.cplx, c.eddy, xt);
         std::vector<size_t> off;
         std::vector<WorkItem> items = chunked(per, off);
         if (items.size() > c.items.size()) {
@@ -828,7 +903,7 @@ static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t e
 }
 
 static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen) {
-    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, fwd_prep_mode(p), st));
     if (p->precision == MRS_PREC_SPLIT_BF16) {
         if (p->nchunks == 1) {
             int rc = enqueue_chunk(p, 0, st, err, errlen);
@@ -945,7 +1020,7 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
         HIP_TRY(hipMemcpy(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
         p->vjp_fresh = false;
     }
-    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, PREP_S32, st));
     size_t u0 = 0;
     while (u0 < n) {
         size_t u1 = u0 + 1;
@@ -970,7 +1045,7 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
     int rc = MRS_OK;
     for (int it = 0; it < iters && rc == MRS_OK; it++) {
         HIP_TRY(hipEventRecord(ev[0], st));
-        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, st));
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, fwd_prep_mode(p), st));
         HIP_TRY(hipEventRecord(ev[1], st));
         if (p->precision == MRS_PREC_SPLIT_BF16) {
             // every chunk's y-stage, then every chunk's x-stage, serially on `stream`
@@ -1011,6 +1086,31 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
     return rc;
 }
 
+extern "C" int mrs_plan_describe(const mrs_plan* p, char* out, size_t outlen) {
+    if (!p || !out || !outlen) return MRS_E_INVALID;
+    std::string r;
+    char line[256];
+    snprintf(line, sizeof line, "plan: %zu groups, %zu slices, %zu units, %d chunks, x_tile %d, line_tiles %d, %s\n",
+             p->groups.size(), p->slices.size(), p->units.size(), p->nchunks, p->xt, p->gp,
+             p->precision == MRS_PREC_SPLIT_BF16 ? "split-bf16" : "fp32");
+    r += line;
+    size_t owners = 0;
+    for (const DevSlice& s : p->slices) owners += s.s_owner ? 1 : 0;
+    snprintf(line, sizeof line, "coil planes: %zu distinct sets\n", owners);
+    r += line;
+    for (size_t g = 0; g < p->groups.size(); g++) {
+        const DevGroup& d = p->groups[g].dev;
+        snprintf(line, sizeof line,
+                 "group %zu: %dx%d -> %dx%d, %d coils (%d row blocks), %d compartments, %d lines (%d padded), "
+                 "%d chain passes, shared steps %s\n",
+                 g, d.snx, d.sny, d.nx, d.ny, d.C, d.RB, d.Q, d.Ly, d.LyP, d.rotor_passes, d.rot_shared ? "yes" : "no");
+        r += line;
+    }
+    std::strncpy(out, r.c_str(), outlen - 1);
+    out[outlen - 1] = 0;
+    return MRS_OK;
+}
+
 extern "C" int mrs_plan_stats(const mrs_plan* p, double* terms, double* lines, double* units) {
     if (!p) return MRS_E_INVALID;
     if (terms) *terms = p->terms;
@@ -1038,10 +1138,9 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
     int maxRB = 1;
     for (auto& c : p->yclasses) maxRB = std::max(maxRB, c.RB);
     const int xts[] = {4, 8, 16};
-    std::vector<int> gps;
-    if (maxRB == 1) gps = {1, 2, 4};
-    else if (maxRB == 2) gps = {1, 2};
-    else gps = {1};
+    // line tiles per pass; classes without that instantiation fall back to a smaller one
+    std::vector<int> gps = {1, 2, 4};
+    (void)maxRB;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

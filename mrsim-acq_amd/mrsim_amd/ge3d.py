@@ -172,6 +172,9 @@ class Ge3dForward:
     def tune(self, trials=3):
         return self.plan.tune(trials)
 
+    def describe(self):
+        return self.plan.describe()
+
     def stage_times(self, iters=5):
         return self.plan.stage_times(iters)
 

@@ -60,6 +60,7 @@ def lib():
                                ct.c_void_p] + E
     L.mrs_plan_stage_times.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, P(ct.c_double)] + E
     L.mrs_plan_stats.argtypes = [ct.c_void_p, P(ct.c_double), P(ct.c_double), P(ct.c_double)]
+    L.mrs_plan_describe.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_size_t]
     L.mrs_plan_destroy.argtypes = [ct.c_void_p]
     L.mrs_plan_tune.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_char_p, ct.c_size_t]
     L.mrs_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32] + E
@@ -459,6 +460,11 @@ class Plan:
         err = _errbuf()
         _check(lib().mrs_plan_stage_times(self._h, self._stream(stream), iters, ms, err, len(err)), err)
         return {"prep": ms[0], "ystage": ms[1], "xstage": ms[2], "epilogue": ms[3], "zstage": ms[4]}
+
+    def describe(self) -> str:
+        buf = ct.create_string_buffer(1 << 16)
+        lib().mrs_plan_describe(self._h, buf, len(buf))
+        return buf.value.decode()
 
     def stats(self):
         t, l, u = ct.c_double(), ct.c_double(), ct.c_double()
