@@ -177,7 +177,7 @@ def main():
     ap.add_argument("--no-ge3d", action="store_true")
     ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
     ap.add_argument("--check-units", type=int, default=2)
-    ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles (skips the autotuner)")
+    ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles[,units_per_item] (skips the autotuner)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run 5 device copies of this many bytes (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -208,8 +208,7 @@ def main():
     plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
     st = torch.cuda.current_stream()
     if args.tiles:
-        xt, gp = (int(v) for v in args.tiles.split(","))
-        plan.set_tiles(xt, gp)
+        plan.set_tiles(*(int(v) for v in args.tiles.split(",")))
     else:
         log(plan.tune(trials=2).strip())
     nunits = len(w.units)

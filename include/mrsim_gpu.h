@@ -230,10 +230,13 @@ int mrs_plan_destroy(mrs_plan* plan);
 /* Kernel tile parameters chosen by the autotuner (see mrs_plan_tune). */
 int mrs_plan_tune(mrs_plan* plan, void* stream, int32_t trials, char* report, size_t reportlen);
 
-/* Set / read the y-stage tiles directly: x columns per block (1..64) and 16-line tiles per pass
- * (1, 2 or 4). */
-int mrs_plan_set_tiles(mrs_plan* plan, int32_t x_tile, int32_t line_tiles, char* err, size_t errlen);
-int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tiles);
+/* Set / read the y-stage tiles directly: x columns per block (1..64), 16-line tiles per pass
+ * (1, 2 or 4) and units per work item (1, 2 or 4: units of one slice evaluated together, sharing
+ * the phase and decay; above 1 only with 4 line tiles, and used by classes of up to 8 coils
+ * without the eddy term). */
+int mrs_plan_set_tiles(mrs_plan* plan, int32_t x_tile, int32_t line_tiles, int32_t units_per_item, char* err,
+                       size_t errlen);
+int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tiles, int32_t* units_per_item);
 
 /* ---- 3D gradient-echo train (stack of EPI blocks over nz partitions) ----------------------- */
 

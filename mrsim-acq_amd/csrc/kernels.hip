@@ -249,20 +249,21 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
 }
 
 // ----------------------------------------------------------------------------- E generation
-// The four voxels (x, y0..y0+3) a lane evaluates for every line it owns.
-template <int Q, bool CPLX, bool EDDY>
+// The four voxels (x, y0..y0+3) a lane evaluates for every line it owns, with the images of the V
+// units (volumes of one slice) that share every other per-voxel quantity.
+template <int Q, bool CPLX, bool EDDY, int V>
 struct Vox4 {
     double base[4];
     double poly[EDDY ? 4 : 1];
     float rate[4];
     float nr2[Q][4], nri[Q][4];
-    float cr[Q][4];
-    float ci[CPLX ? Q : 1][4];
+    float cr[V][Q][4];
+    float ci[V][CPLX ? Q : 1][4];
 };
 
-template <int Q, bool CPLX, bool EDDY>
-__device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice& sl, const void* images, int snx,
-                                          int sny, int x, int y0, int snyP, int PP) {
+template <int Q, bool CPLX, bool EDDY, int V>
+__device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY, V>& v, const DevSlice& sl, const void* const (&images)[V],
+                                          int snx, int sny, int x, int y0, int snyP, int PP) {
     const int o = x * snyP + y0;   // 32-bit offsets from scalar bases: global loads with saddr
     const f64x2 b01 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o);
     const f64x2 b23 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o + 2);
@@ -283,27 +284,44 @@ __device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY>& v, const DevSlice
     }
     const int P = snx * sny;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int y = y0 + j;
-        const bool ok = y < sny;                       // x < snx always holds here
-        const int i = x + snx * (ok ? y : sny - 1);    // clamped: every load is in bounds
+    for (int vv = 0; vv < V; vv++) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) {
-            if constexpr (CPLX) {
-                const f32x2 m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images)) + 2 * (q * P + i));
-                v.cr[q][j] = ok ? m.x : 0.0f;
-                v.ci[q][j] = ok ? m.y : 0.0f;
-            } else {
-                const float m = gld(reinterpret_cast<const float*>(images))[q * P + i];
-                v.cr[q][j] = ok ? m : 0.0f;
+        for (int j = 0; j < 4; j++) {
+            const int y = y0 + j;
+            const bool ok = y < sny;                       // x < snx always holds here
+            const int i = x + snx * (ok ? y : sny - 1);    // clamped: every load is in bounds
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if constexpr (CPLX) {
+                    const f32x2 m =
+                        *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images[vv])) + 2 * (q * P + i));
+                    v.cr[vv][q][j] = ok ? m.x : 0.0f;
+                    v.ci[vv][q][j] = ok ? m.y : 0.0f;
+                } else {
+                    const float m = gld(reinterpret_cast<const float*>(images[vv]))[q * P + i];
+                    v.cr[vv][q][j] = ok ? m : 0.0f;
+                }
             }
         }
     }
 }
 
-// E(x, y0+j, line) for one j: yy = y0 + j - (sys + yoff)
+// Phase (sin, cos of 2 pi cyc) and per-compartment decay of voxel (x, y0+j) at one line:
+// yy = y0 + j - (sys + yoff).  Shared by every unit of the slice.
+template <int Q, bool CPLX, bool EDDY, int V>
+__device__ __forceinline__ void phase_decay(const Vox4<Q, CPLX, EDDY, V>& v, int j, double yy, double t_s, double kyn,
+                                            double d, float trf, float tau, float& sn, float& cs, float (&dq)[Q]) {
+    double cyc = __builtin_fma(kyn, yy, v.base[j]);
+    cyc = __builtin_fma((double)v.rate[j], t_s, cyc);
+    if constexpr (EDDY) cyc = __builtin_fma(v.poly[j], d, cyc);
+    sincos_turns(cyc, sn, cs);
+#pragma unroll
+    for (int q = 0; q < Q; q++) dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
+}
+
+// E of a single unit at voxel j, the decay folded into the compartment sum as it is evaluated.
 template <int Q, bool CPLX, bool EDDY>
-__device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, double yy, double t_s, double kyn,
+__device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY, 1>& v, int j, double yy, double t_s, double kyn,
                                       double d, float trf, float tau, float& er, float& ei) {
     double cyc = __builtin_fma(kyn, yy, v.base[j]);
     cyc = __builtin_fma((double)v.rate[j], t_s, cyc);
@@ -313,9 +331,28 @@ __device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, doubl
     float wr = 0.0f, wi = 0.0f;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
-        wr = __builtin_fmaf(v.cr[q][j], dq, wr);
-        if constexpr (CPLX) wi = __builtin_fmaf(v.ci[q][j], dq, wi);
+        const float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
+        wr = __builtin_fmaf(v.cr[0][q][j], dq, wr);
+        if constexpr (CPLX) wi = __builtin_fmaf(v.ci[0][q][j], dq, wi);
+    }
+    if constexpr (CPLX) {
+        er = wr * cs - wi * sn;
+        ei = wr * sn + wi * cs;
+    } else {
+        er = wr * cs;
+        ei = wr * sn;
+    }
+}
+
+// E of unit vv at voxel j: (sum_q m_q dq_q) (cs + i sn).
+template <int Q, bool CPLX, bool EDDY, int V>
+__device__ __forceinline__ void unit_e(const Vox4<Q, CPLX, EDDY, V>& v, int vv, int j, float sn, float cs,
+                                       const float (&dq)[Q], float& er, float& ei) {
+    float wr = 0.0f, wi = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        wr = __builtin_fmaf(v.cr[vv][q][j], dq[q], wr);
+        if constexpr (CPLX) wi = __builtin_fmaf(v.ci[vv][q][j], dq[q], wi);
     }
     if constexpr (CPLX) {
         er = wr * cs - wi * sn;
@@ -327,26 +364,43 @@ __device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY>& v, int j, doubl
 }
 
 // ----------------------------------------------------------------------------- y-stage, MFMA
-// Block = 4 waves = one (unit, x tile of `xt` columns).  Per column x a wave accumulates
+// Block = 4 waves = one (unit set, x tile of `xt` columns).  Per column x a wave accumulates
 //   G[(coil, part), line] = sum_{y, part'} A[(coil,part), (y,part')] * B[(y,part'), line]
 // with A the split coil sensitivities and B = E generated in registers, 16 lines per MFMA tile,
-// GP tiles per pass and RB 16-row blocks (8 coils each).
+// GP tiles per pass and RB 16-row blocks (8 coils each).  A set holds V units of one slice (its
+// volumes): the phase, the decay and A are evaluated once per voxel and line for all of them.
 // Occupancy bound: 4 waves/SIMD (128 VGPRs) where the instantiation fits without spilling, 2 for the
-// 16-accumulator tiles of many coils over 64 lines, else 3.
-template <int Q, int RB, int GP, bool CPLX, bool EDDY>
+// 16-accumulator tiles of many coils over 64 lines and for unit sets, else 3.
+template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
 constexpr int ystage_waves() {
+    if constexpr (V > 1) return (V == 2 && (CPLX ? 2 * Q : Q) <= 2) ? 3 : 2;
     return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : (RB * GP >= 8 ? 2 : 3);
 }
 
-template <int Q, int RB, int GP, bool CPLX, bool EDDY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_waves<Q, RB, GP, CPLX, EDDY>())))
+template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_waves<Q, RB, GP, CPLX, EDDY, V>())))
 void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
-                 const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work, int xt) {
+                 const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work,
+                 const int32_t* __restrict__ usets, int xt) {
     __shared__ LineRec lines_s[256];
     const WorkItem wi = work[blockIdx.x];
-    const DevUnit& u = units[wi.unit];
+    int uid[V];
+    if constexpr (V == 1) {
+        uid[0] = wi.unit;
+    } else {
+#pragma unroll
+        for (int vv = 0; vv < V; vv++) uid[vv] = usets[wi.unit * V + vv];
+    }
+    const DevUnit& u = units[uid[0]];
     const DevGroup& gr = groups[u.group];
     const DevSlice& sl = slices[u.slice];
+    const void* img[V];
+    float2* gbuf[V];
+#pragma unroll
+    for (int vv = 0; vv < V; vv++) {
+        img[vv] = units[uid[vv]].images;
+        gbuf[vv] = units[uid[vv]].gbuf;
+    }
     for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -362,15 +416,17 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
         const int x = wi.tile * xt + xi;
         if (x >= snx) break;
         for (int g0 = 0; g0 < G; g0 += GP) {
-            f32x4 acc[RB][GP];
+            f32x4 acc[V][RB][GP];
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++)
+            for (int vv = 0; vv < V; vv++)
 #pragma unroll
-                for (int g = 0; g < GP; g++) acc[rb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                    for (int g = 0; g < GP; g++) acc[vv][rb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int ch = 0; ch < snyP; ch += 16) {
                 const int y0 = ch + lq * 4;
-                Vox4<Q, CPLX, EDDY> v;
-                load_vox4<Q, CPLX, EDDY>(v, sl, u.images, snx, sny, x, y0, snyP, (int)PP);
+                Vox4<Q, CPLX, EDDY, V> v;
+                load_vox4<Q, CPLX, EDDY, V>(v, sl, img, snx, sny, x, y0, snyP, (int)PP);
                 u32x4 ahi[RB], alo[RB];
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++) {
@@ -389,7 +445,6 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         // the voxel's rotor cis(2 pi (rate dt + dkyn yy)) and each decay by the voxel's
                         // exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.
                         const LineRec& L0 = lines_s[g0 * 16 + ln];
-                        // the chain rotor and this echo side's decay ratios, precomputed per voxel
                         const int ro = x * snyP + y0;
                         const f32x4 srot0 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro);
                         const f32x4 srot1 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro + 2);
@@ -397,55 +452,85 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 #pragma unroll
                         for (int q = 0; q < Q; q++)
                             srq[q] = *reinterpret_cast<const GAS f32x4*>(gld(sl.rq) + (L0.cls * Q + q) * (int)PP + ro);
-                        u32x4 bh[4], bl[4];
+                        if constexpr (V == 1) {
+                            u32x4 bh[4], bl[4];
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            const double yy = yy0 + (double)j;
-                            double c0 = __builtin_fma(L0.kyn, yy, v.base[j]);
-                            c0 = __builtin_fma((double)v.rate[j], L0.t_s, c0);
-                            float ps, pc;
-                            sincos_turns(c0, ps, pc);
-                            const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
-                            const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
-                            float dq[Q], rq[Q];
-#pragma unroll
-                            for (int q = 0; q < Q; q++) {
-                                dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(L0.trf, v.nr2[q][j], L0.tau * v.nri[q][j]));
-                                rq[q] = srq[q][j];
-                            }
-#pragma unroll
-                            for (int g = 0; g < 4; g++) {
-                                if (g) {
-                                    const float npc = pc * rc - ps * rs;
-                                    ps = __builtin_fmaf(pc, rs, ps * rc);
-                                    pc = npc;
-#pragma unroll
-                                    for (int q = 0; q < Q; q++) dq[q] *= rq[q];
-                                }
-                                float wr = 0.0f, wi = 0.0f;
+                            for (int j = 0; j < 4; j++) {
+                                const double yy = yy0 + (double)j;
+                                double c0 = __builtin_fma(L0.kyn, yy, v.base[j]);
+                                c0 = __builtin_fma((double)v.rate[j], L0.t_s, c0);
+                                float ps, pc;
+                                sincos_turns(c0, ps, pc);
+                                const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
+                                const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
+                                float dq[Q], rq[Q];
 #pragma unroll
                                 for (int q = 0; q < Q; q++) {
-                                    wr = __builtin_fmaf(v.cr[q][j], dq[q], wr);
-                                    if constexpr (CPLX) wi = __builtin_fmaf(v.ci[q][j], dq[q], wi);
+                                    dq[q] = __builtin_amdgcn_exp2f(
+                                        __builtin_fmaf(L0.trf, v.nr2[q][j], L0.tau * v.nri[q][j]));
+                                    rq[q] = srq[q][j];
                                 }
-                                float er, ei;
-                                if constexpr (CPLX) {
-                                    er = wr * pc - wi * ps;
-                                    ei = wr * ps + wi * pc;
-                                } else {
-                                    er = wr * pc;
-                                    ei = wr * ps;
+#pragma unroll
+                                for (int g = 0; g < 4; g++) {
+                                    if (g) {
+                                        const float npc = pc * rc - ps * rs;
+                                        ps = __builtin_fmaf(pc, rs, ps * rc);
+                                        pc = npc;
+#pragma unroll
+                                        for (int q = 0; q < Q; q++) dq[q] *= rq[q];
+                                    }
+                                    float er, ei;
+                                    unit_e(v, 0, j, ps, pc, dq, er, ei);
+                                    uint32_t h, l;
+                                    split2(er, ei, h, l);
+                                    bh[g][j] = h;
+                                    bl[g][j] = l;
                                 }
-                                uint32_t h, l;
-                                split2(er, ei, h, l);
-                                bh[g][j] = h;
-                                bl[g][j] = l;
+                            }
+#pragma unroll
+                            for (int g = 0; g < 4; g++)
+#pragma unroll
+                                for (int rb = 0; rb < RB; rb++)
+                                    acc[0][rb][g] = mfma3(ahi[rb], alo[rb], bh[g], bl[g], acc[0][rb][g]);
+                        } else {
+                            // line by line across the lane's four voxels, so only one line's B
+                            // operands per unit are live at a time
+                            float ps[4], pc[4], dq[4][Q];
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                phase_decay(v, j, yy0 + (double)j, L0.t_s, L0.kyn, 0.0, L0.trf, L0.tau, ps[j], pc[j],
+                                            dq[j]);
+#pragma unroll
+                            for (int g = 0; g < 4; g++) {
+                                u32x4 bh[V], bl[V];
+#pragma unroll
+                                for (int j = 0; j < 4; j++) {
+                                    if (g) {
+                                        const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
+                                        const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
+                                        const float npc = pc[j] * rc - ps[j] * rs;
+                                        ps[j] = __builtin_fmaf(pc[j], rs, ps[j] * rc);
+                                        pc[j] = npc;
+#pragma unroll
+                                        for (int q = 0; q < Q; q++) dq[j][q] *= srq[q][j];
+                                    }
+#pragma unroll
+                                    for (int vv = 0; vv < V; vv++) {
+                                        float er, ei;
+                                        unit_e(v, vv, j, ps[j], pc[j], dq[j], er, ei);
+                                        uint32_t h, l;
+                                        split2(er, ei, h, l);
+                                        bh[vv][j] = h;
+                                        bl[vv][j] = l;
+                                    }
+                                }
+#pragma unroll
+                                for (int vv = 0; vv < V; vv++)
+#pragma unroll
+                                    for (int rb = 0; rb < RB; rb++)
+                                        acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
                             }
                         }
-#pragma unroll
-                        for (int g = 0; g < 4; g++)
-#pragma unroll
-                            for (int rb = 0; rb < RB; rb++) acc[rb][g] = mfma3(ahi[rb], alo[rb], bh[g], bl[g], acc[rb][g]);
                         continue;
                     }
                 }
@@ -456,37 +541,57 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         const bool ok = L.ky >= 0;
                         const double t_s = L.t_s, kyn = L.kyn, d = L.d;
                         const float trf = L.trf, tau = L.tau;
-                        u32x4 bhi, blo;
+                        u32x4 bhi[V], blo[V];
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
-                            float er, ei;
-                            gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
-                            uint32_t h, l;
-                            split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
-                            bhi[j] = h;
-                            blo[j] = l;
+                            if constexpr (V == 1) {
+                                float er, ei;
+                                gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
+                                uint32_t h, l;
+                                split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
+                                bhi[0][j] = h;
+                                blo[0][j] = l;
+                            } else {
+                                float sn, cs, dq[Q];
+                                phase_decay(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, sn, cs, dq);
+#pragma unroll
+                                for (int vv = 0; vv < V; vv++) {
+                                    float er, ei;
+                                    unit_e(v, vv, j, sn, cs, dq, er, ei);
+                                    uint32_t h, l;
+                                    split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
+                                    bhi[vv][j] = h;
+                                    blo[vv][j] = l;
+                                }
+                            }
                         }
 #pragma unroll
-                        for (int rb = 0; rb < RB; rb++) acc[rb][g] = mfma3(ahi[rb], alo[rb], bhi, blo, acc[rb][g]);
+                        for (int vv = 0; vv < V; vv++)
+#pragma unroll
+                            for (int rb = 0; rb < RB; rb++)
+                                acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bhi[vv], blo[vv], acc[vv][rb][g]);
                     }
                 }
             }
-            // acc[rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
+            // acc[vv][rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
 #pragma unroll
             for (int g = 0; g < GP; g++) {
                 if (g0 + g < G) {
                     const int lp = (g0 + g) * 16 + ln;
 #pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
+                    for (int vv = 0; vv < V; vv++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int c = rb * 8 + 4 * (lq & 1) + r;
-                            if (c < C) {
-                                GAS float* dst = reinterpret_cast<GAS float*>(gst(u.gbuf) + ((size_t)x * LyP + lp) * C + c);
-                                dst[lq >> 1] = acc[rb][g][r];
+                        for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int c = rb * 8 + 4 * (lq & 1) + r;
+                                if (c < C) {
+                                    GAS float* dst =
+                                        reinterpret_cast<GAS float*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C + c);
+                                    dst[lq >> 1] = acc[vv][rb][g][r];
+                                }
                             }
                         }
-                    }
                 }
             }
         }
@@ -1097,57 +1202,72 @@ hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, in
     return hipGetLastError();
 }
 
-template <int Q, int RB, int GP, bool CPLX, bool EDDY>
+template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V = 1>
 static hipError_t ylaunch(dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
-                          const WorkItem* w, int xt) {
-    hipLaunchKernelGGL((ystage_mfma<Q, RB, GP, CPLX, EDDY>), grid, dim3(256), 0, st, u, s, g, w, xt);
+                          const WorkItem* w, const int32_t* usets, int xt) {
+    hipLaunchKernelGGL((ystage_mfma<Q, RB, GP, CPLX, EDDY, V>), grid, dim3(256), 0, st, u, s, g, w, usets, xt);
     return hipGetLastError();
 }
 
-// Instantiated (RB, GP) pairs: the accumulators RB*GP*4 VGPRs stay within budget.
+// Instantiated (RB, GP) pairs: the accumulators RB*GP*4 VGPRs stay within budget.  Unit sets (V > 1)
+// exist for one row block (up to 8 coils), four line tiles and no eddy term.
 template <int Q, bool CPLX, bool EDDY>
-static hipError_t ylaunch_rb(int rb, int gp, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
-                             const DevGroup* g, const WorkItem* w, int xt) {
+static hipError_t ylaunch_rb(int rb, int gp, int nv, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                             const DevGroup* g, const WorkItem* w, const int32_t* usets, int xt) {
+    if (nv > 1) {
+        if constexpr (!EDDY) {
+            if (rb == 1 && gp >= 4) {
+                if (nv == 2) return ylaunch<Q, 1, 4, CPLX, EDDY, 2>(grid, st, u, s, g, w, usets, xt);
+                // four units' images and accumulators fit without spilling for up to two image planes
+                if constexpr ((CPLX ? 2 * Q : Q) <= 2) {
+                    if (nv == 4) return ylaunch<Q, 1, 4, CPLX, EDDY, 4>(grid, st, u, s, g, w, usets, xt);
+                }
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     switch (rb) {
     case 1:
-        if (gp >= 4) return ylaunch<Q, 1, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-        if (gp == 2) return ylaunch<Q, 1, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-        return ylaunch<Q, 1, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        if (gp >= 4) return ylaunch<Q, 1, 4, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
+        if (gp == 2) return ylaunch<Q, 1, 2, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
+        return ylaunch<Q, 1, 1, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
     case 2:
-        if (gp >= 4) return ylaunch<Q, 2, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-        if (gp >= 2) return ylaunch<Q, 2, 2, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-        return ylaunch<Q, 2, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        if (gp >= 4) return ylaunch<Q, 2, 4, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
+        if (gp >= 2) return ylaunch<Q, 2, 2, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
+        return ylaunch<Q, 2, 1, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
     case 4:
         // 16 accumulator tiles fit without spilling for up to two compartments
         if constexpr (Q <= 2) {
-            if (gp >= 4) return ylaunch<Q, 4, 4, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+            if (gp >= 4) return ylaunch<Q, 4, 4, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
         }
-        return ylaunch<Q, 4, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
-    case 8: return ylaunch<Q, 8, 1, CPLX, EDDY>(grid, st, u, s, g, w, xt);
+        return ylaunch<Q, 4, 1, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
+    case 8: return ylaunch<Q, 8, 1, CPLX, EDDY>(grid, st, u, s, g, w, usets, xt);
     default: return hipErrorInvalidValue;
     }
 }
 
 template <bool CPLX, bool EDDY>
-static hipError_t ylaunch_q(int Q, int RB, int GP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
-                            const DevGroup* g, const WorkItem* w, int xt) {
+static hipError_t ylaunch_q(int Q, int RB, int GP, int nv, dim3 grid, hipStream_t st, const DevUnit* u,
+                            const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets, int xt) {
     switch (Q) {
-    case 1: return ylaunch_rb<1, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
-    case 2: return ylaunch_rb<2, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
-    case 3: return ylaunch_rb<3, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
-    case 4: return ylaunch_rb<4, CPLX, EDDY>(RB, GP, grid, st, u, s, g, w, xt);
+    case 1: return ylaunch_rb<1, CPLX, EDDY>(RB, GP, nv, grid, st, u, s, g, w, usets, xt);
+    case 2: return ylaunch_rb<2, CPLX, EDDY>(RB, GP, nv, grid, st, u, s, g, w, usets, xt);
+    case 3: return ylaunch_rb<3, CPLX, EDDY>(RB, GP, nv, grid, st, u, s, g, w, usets, xt);
+    case 4: return ylaunch_rb<4, CPLX, EDDY>(RB, GP, nv, grid, st, u, s, g, w, usets, xt);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
-                              const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st) {
+// nv = units per work item (1, or 2 / 4 for sets of one slice's units listed in `usets`).
+hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
+                              const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
+                              hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
     dim3 grid(nwork);
-    if (cplx) return eddy ? ylaunch_q<true, true>(Q, RB, GP, grid, st, u, s, g, w, xt)
-                          : ylaunch_q<true, false>(Q, RB, GP, grid, st, u, s, g, w, xt);
-    return eddy ? ylaunch_q<false, true>(Q, RB, GP, grid, st, u, s, g, w, xt)
-                : ylaunch_q<false, false>(Q, RB, GP, grid, st, u, s, g, w, xt);
+    if (cplx) return eddy ? ylaunch_q<true, true>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt)
+                          : ylaunch_q<true, false>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt);
+    return eddy ? ylaunch_q<false, true>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt)
+                : ylaunch_q<false, false>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt);
 }
 
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,

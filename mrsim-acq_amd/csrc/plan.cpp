@@ -17,8 +17,9 @@
 
 namespace mrs {
 hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, int mode, hipStream_t st);
-hipError_t launch_ystage_mfma(int Q, int RB, int GP, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
-                              const DevSlice* s, const DevGroup* g, const WorkItem* w, hipStream_t st);
+hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
+                              const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
+                              hipStream_t st);
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
                               const WorkItem* w, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
@@ -215,6 +216,14 @@ struct mrs_plan {
         std::vector<WorkItem> items;  // per-chunk runs, each XCD-ordered; chunk k is [off[k], off[k+1])
         WorkItem* d_items = nullptr;
         std::vector<size_t> off;
+        // items over sets of `nv` units of one slice (unit = set index into usets)
+        int nv = 1;
+        std::vector<WorkItem> vitems;
+        WorkItem* d_vitems = nullptr;
+        std::vector<size_t> voff;
+        std::vector<int32_t> usets;
+        int32_t* d_usets = nullptr;
+        size_t cap_items = 0, cap_vitems = 0, cap_usets = 0;
     };
     std::vector<YClass> yclasses;
     struct XClass {
@@ -232,7 +241,7 @@ struct mrs_plan {
     std::vector<hipEvent_t> ev_chunk;
     hipEvent_t ev_prep = nullptr;
     std::vector<XClass> xclasses;
-    int xt = 8, gp = 4;
+    int xt = 8, gp = 4, nv = 1;
     int maxvox = 0;
     int maxC = 1;
     bool any_zero_rows = false, any_epilogue = false;
@@ -281,8 +290,8 @@ static int plan_alloc(mrs_plan* p, T** ptr, size_t n, char* err, size_t errlen) 
 // gives each XCD a contiguous run of items (units of one slice share that XCD's L2).
 static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w);
 static std::vector<size_t> unit_order(const mrs_plan* p);
-static std::vector<std::vector<WorkItem>> ylist(const mrs_plan* p, const std::vector<size_t>& order, int Q, int RB,
-                                                int cplx, int eddy, int xt);
+static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<size_t>& order, char* err,
+                         size_t errlen);
 
 // Concatenate per-chunk item lists, each XCD-ordered; off[k] is chunk k's first item.
 static std::vector<WorkItem> chunked(const std::vector<std::vector<WorkItem>>& per, std::vector<size_t>& off) {
@@ -740,8 +749,6 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         p->nchunks = nunits >= 64 ? 4 : 1;
         p->unit_chunk.assign(nunits, 0);
         for (size_t k = 0; k < nunits; k++) p->unit_chunk[order[k]] = (int)(k * p->nchunks / nunits);
-        std::vector<std::vector<std::vector<WorkItem>>> yw(p->yclasses.size(),
-                                                           std::vector<std::vector<WorkItem>>(p->nchunks));
         std::vector<std::vector<std::vector<WorkItem>>> xw;
         std::map<std::tuple<int, int, size_t>, size_t> xcls;
         // x-stage tiling per group: 8*MT kx rows of twiddles staged in LDS (<= 80 KiB so two blocks
@@ -788,17 +795,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             for (int t = 0; t < nkxb * g_np[gi]; t++) xw[xi][ch].push_back({(int32_t)u, t});
         }
         for (size_t c = 0; c < p->xclasses.size(); c++) p->xclasses[c].items = chunked(xw[c], p->xclasses[c].off);
-        for (size_t c = 0; c < p->yclasses.size(); c++) {
-            const auto& yc = p->yclasses[c];
-            yw[c] = ylist(p, order, yc.Q, yc.RB, yc.cplx, yc.eddy, p->xt);
-        }
-        for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++) {
-            p->yclasses[c].items = chunked(yw[c], p->yclasses[c].off);
-            rc = plan_alloc(p, &p->yclasses[c].d_items, p->yclasses[c].items.size(), err, errlen);
-            if (!rc && hipMemcpy(p->yclasses[c].d_items, p->yclasses[c].items.data(),
-                                 p->yclasses[c].items.size() * sizeof(WorkItem), hipMemcpyHostToDevice) != hipSuccess)
-                rc = fail(err, errlen, MRS_E_DEVICE, "upload of work items failed");
-        }
+        for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++)
+            rc = upload_ylists(p, p->yclasses[c], order, err, errlen);
         for (auto& xc : p->xclasses) {
             if (rc) break;
             rc = plan_alloc(p, &xc.d_items, xc.items.size(), err, errlen);
@@ -837,47 +835,111 @@ static std::vector<size_t> unit_order(const mrs_plan* p) {
     return order;
 }
 
-static std::vector<std::vector<WorkItem>> ylist(const mrs_plan* p, const std::vector<size_t>& order, int Q, int RB,
-                                                int cplx, int eddy, int xt) {
-    std::vector<std::vector<WorkItem>> per(p->nchunks);
+// Unit sets of `nv` units are formed within one slice (the units share the group, the phase and the
+// decay); a class gets them only where its kernel has the variant (one row block, four line tiles,
+// no eddy term).  Leftover units run one per item.
+static bool set_eligible(const mrs_plan* p, const mrs_plan::YClass& c, int gp) {
+    const int planes = c.cplx ? 2 * c.Q : c.Q;   // sets of four: up to two image planes per unit
+    return p->nv > 1 && c.RB == 1 && gp >= 4 && !c.eddy && (p->nv == 2 || planes <= 2);
+}
+
+static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs_plan::YClass& c, int xt, int gp,
+                  std::vector<std::vector<WorkItem>>& single, std::vector<std::vector<WorkItem>>& multi,
+                  std::vector<int32_t>& usets) {
+    single.assign(p->nchunks, {});
+    multi.assign(p->nchunks, {});
+    usets.clear();
+    const int nv = set_eligible(p, c, gp) ? p->nv : 1;
     size_t i = 0;
     while (i < order.size()) {
         size_t j = i;
         const size_t own = p->s_owner[p->units[order[i]].slice];
         const int ch = p->unit_chunk[order[i]];
         while (j < order.size() && p->s_owner[p->units[order[j]].slice] == own && p->unit_chunk[order[j]] == ch) j++;
-        int ntile = 0;
+        // this run's units of the class, by slice; sets of nv, then the leftovers
+        std::vector<size_t> mine;
         for (size_t k = i; k < j; k++) {
             const DevGroup& dg = p->groups[p->units[order[k]].group].dev;
-            ntile = std::max(ntile, (dg.snx + xt - 1) / xt);
+            if (dg.Q == c.Q && dg.RB == c.RB && dg.cplx == c.cplx && (dg.eddy_strength != 0.0 ? 1 : 0) == c.eddy)
+                mine.push_back(order[k]);
         }
-        for (int t = 0; t < ntile; t++)
-            for (size_t k = i; k < j; k++) {
-                const size_t u = order[k];
-                const DevGroup& dg = p->groups[p->units[u].group].dev;
-                if (dg.Q != Q || dg.RB != RB || dg.cplx != cplx || (dg.eddy_strength != 0.0 ? 1 : 0) != eddy) continue;
-                if (t < (dg.snx + xt - 1) / xt) per[ch].push_back({(int32_t)u, t});
+        std::vector<int32_t> sets;     // first usets index of each set
+        std::vector<size_t> rest;
+        size_t a = 0;
+        while (a < mine.size()) {
+            size_t b = a;
+            while (b < mine.size() && p->units[mine[b]].slice == p->units[mine[a]].slice) b++;
+            size_t k = a;
+            for (; nv > 1 && k + nv <= b; k += nv) {
+                sets.push_back((int32_t)(usets.size() / nv));
+                for (int v = 0; v < nv; v++) usets.push_back((int32_t)mine[k + v]);
             }
+            for (; k < b; k++) rest.push_back(mine[k]);
+            a = b;
+        }
+        int ntile = 0;
+        for (size_t u : mine) ntile = std::max(ntile, (p->groups[p->units[u].group].dev.snx + xt - 1) / xt);
+        for (int t = 0; t < ntile; t++) {
+            for (int32_t sidx : sets) {
+                const DevGroup& dg = p->groups[p->units[usets[(size_t)sidx * nv]].group].dev;
+                if (t < (dg.snx + xt - 1) / xt) multi[ch].push_back({sidx, t});
+            }
+            for (size_t u : rest) {
+                const DevGroup& dg = p->groups[p->units[u].group].dev;
+                if (t < (dg.snx + xt - 1) / xt) single[ch].push_back({(int32_t)u, t});
+            }
+        }
         i = j;
     }
-    return per;
 }
 
-static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
+// Rebuilds and uploads a class's work lists for the current tiles.
+static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<size_t>& order, char* err,
+                         size_t errlen) {
+    std::vector<std::vector<WorkItem>> single, multi;
+    std::vector<int32_t> usets;
+    ylist(p, order, c, p->xt, p->gp, single, multi, usets);
+    c.nv = set_eligible(p, c, p->gp) ? p->nv : 1;
+    c.items = chunked(single, c.off);
+    c.vitems = chunked(multi, c.voff);
+    c.usets = usets;
+    auto put = [&](auto*& dptr, size_t& cap, const auto& host) -> int {
+        using T = typename std::decay_t<decltype(host)>::value_type;
+        const size_t n = std::max<size_t>(host.size(), 1);
+        if (n > cap) {
+            int rc = plan_alloc(p, &dptr, n, err, errlen);
+            if (rc) return rc;
+            cap = n;
+        }
+        if (!host.empty()) HIP_TRY(hipMemcpy(dptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+        return MRS_OK;
+    };
+    int rc = put(c.d_items, c.cap_items, c.items);
+    if (!rc) rc = put(c.d_vitems, c.cap_vitems, c.vitems);
+    if (!rc) rc = put(c.d_usets, c.cap_usets, c.usets);
+    return rc;
+}
+
+static hipError_t launch_yclass(const mrs_plan* p, const mrs_plan::YClass& c, int k, hipStream_t st) {
+    hipError_t e = hipSuccess;
+    const size_t va = c.voff[k], vb = c.voff[k + 1];
+    if (vb > va)
+        e = launch_ystage_mfma(c.Q, c.RB, p->gp, c.nv, c.cplx != 0, c.eddy != 0, (int)(vb - va), p->xt, p->d_units,
+                               p->d_slices, p->d_groups, c.d_vitems + va, c.d_usets, st);
+    if (e != hipSuccess) return e;
+    const size_t a = c.off[k], b = c.off[k + 1];
+    return launch_ystage_mfma(c.Q, c.RB, p->gp, 1, c.cplx != 0, c.eddy != 0, (int)(b - a), p->xt, p->d_units,
+                              p->d_slices, p->d_groups, c.d_items + a, c.d_usets, st);
+}
+
+static int set_tiles(mrs_plan* p, int xt, int gp, int nv, char* err, size_t errlen) {
     p->xt = xt;
     p->gp = gp;
+    p->nv = nv;
     const std::vector<size_t> order = unit_order(p);
     for (auto& c : p->yclasses) {
-        std::vector<std::vector<WorkItem>> per = ylist(p, order, c.Q, c.RB, c.cplx, c.eddy, xt);
-        std::vector<size_t> off;
-        std::vector<WorkItem> items = chunked(per, off);
-        if (items.size() > c.items.size()) {
-            int rc = plan_alloc(p, &c.d_items, items.size(), err, errlen);
-            if (rc) return rc;
-        }
-        c.items = items;
-        c.off = off;
-        HIP_TRY(hipMemcpy(c.d_items, items.data(), items.size() * sizeof(WorkItem), hipMemcpyHostToDevice));
+        int rc = upload_ylists(p, c, order, err, errlen);
+        if (rc) return rc;
     }
     if (p->graph_exec) {
         hipGraphExecDestroy(p->graph_exec);
@@ -889,11 +951,7 @@ static int set_tiles(mrs_plan* p, int xt, int gp, char* err, size_t errlen) {
 }
 
 static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t errlen) {
-    for (auto& c : p->yclasses) {
-        const size_t a = c.off[k], b = c.off[k + 1];
-        HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0, (int)(b - a), p->xt, p->d_units,
-                                   p->d_slices, p->d_groups, c.d_items + a, st));
-    }
+    for (auto& c : p->yclasses) HIP_TRY(launch_yclass(p, c, k, st));
     for (auto& xc : p->xclasses) {
         const size_t a = xc.off[k], b = xc.off[k + 1];
         HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(b - a), p->d_units, p->d_groups, xc.d_items + a,
@@ -1050,10 +1108,7 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
         if (p->precision == MRS_PREC_SPLIT_BF16) {
             // every chunk's y-stage, then every chunk's x-stage, serially on `stream`
             for (auto& c : p->yclasses)
-                for (int k = 0; k < p->nchunks; k++)
-                    HIP_TRY(launch_ystage_mfma(c.Q, c.RB, p->gp, c.cplx != 0, c.eddy != 0,
-                                               (int)(c.off[k + 1] - c.off[k]), p->xt, p->d_units, p->d_slices,
-                                               p->d_groups, c.d_items + c.off[k], st));
+                for (int k = 0; k < p->nchunks; k++) HIP_TRY(launch_yclass(p, c, k, st));
             HIP_TRY(hipEventRecord(ev[2], st));
             for (auto& xc : p->xclasses)
                 for (int k = 0; k < p->nchunks; k++)
@@ -1090,8 +1145,9 @@ extern "C" int mrs_plan_describe(const mrs_plan* p, char* out, size_t outlen) {
     if (!p || !out || !outlen) return MRS_E_INVALID;
     std::string r;
     char line[256];
-    snprintf(line, sizeof line, "plan: %zu groups, %zu slices, %zu units, %d chunks, x_tile %d, line_tiles %d, %s\n",
-             p->groups.size(), p->slices.size(), p->units.size(), p->nchunks, p->xt, p->gp,
+    snprintf(line, sizeof line,
+             "plan: %zu groups, %zu slices, %zu units, %d chunks, x_tile %d, line_tiles %d, units per item %d, %s\n",
+             p->groups.size(), p->slices.size(), p->units.size(), p->nchunks, p->xt, p->gp, p->nv,
              p->precision == MRS_PREC_SPLIT_BF16 ? "split-bf16" : "fp32");
     r += line;
     size_t owners = 0;
@@ -1135,42 +1191,55 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
     if (p->precision != MRS_PREC_SPLIT_BF16) return MRS_OK;
     hipStream_t st = (hipStream_t)stream;
     if (trials < 1) trials = 3;
-    int maxRB = 1;
-    for (auto& c : p->yclasses) maxRB = std::max(maxRB, c.RB);
     const int xts[] = {4, 8, 16};
-    // line tiles per pass; classes without that instantiation fall back to a smaller one
-    std::vector<int> gps = {1, 2, 4};
-    (void)maxRB;
+    // line tiles per pass (classes without that instantiation fall back to a smaller one) and units
+    // per item (sets only where some class can form them)
+    const int gps[] = {1, 2, 4};
+    std::vector<int> nvs = {1};
+    {
+        std::map<size_t, int> per_slice;
+        for (const DevUnit& u : p->units) per_slice[u.slice]++;
+        int most = 0;
+        for (auto& kv : per_slice) most = std::max(most, kv.second);
+        bool any2 = false, any4 = false;
+        for (auto& c : p->yclasses) {
+            any2 |= c.RB == 1 && !c.eddy;
+            any4 |= c.RB == 1 && !c.eddy && (c.cplx ? 2 * c.Q : c.Q) <= 2;
+        }
+        if (any2 && most >= 2) nvs.push_back(2);
+        if (any4 && most >= 4) nvs.push_back(4);
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bxt = p->xt, bgp = p->gp;
+    int bxt = p->xt, bgp = p->gp, bnv = p->nv;
     std::string rep;
-    for (int xt : xts) {
-        for (int gp : gps) {
-            int rc = set_tiles(p, xt, gp, err, errlen);
-            if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
-            rc = enqueue_forward(p, st, err, errlen);  // warm-up
-            if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
-            HIP_TRY(hipEventRecord(e0, st));
-            for (int t = 0; t < trials; t++) enqueue_forward(p, st, err, errlen);
-            HIP_TRY(hipEventRecord(e1, st));
-            HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            ms /= (float)trials;
-            char line[96];
-            snprintf(line, sizeof line, "xt=%d gp=%d %.4f ms\n", xt, gp, ms);
-            rep += line;
-            if (ms < best) { best = ms; bxt = xt; bgp = gp; }
-        }
-    }
+    for (int xt : xts)
+        for (int gp : gps)
+            for (int nv : nvs) {
+                if (nv > 1 && gp != 4) continue;
+                int rc = set_tiles(p, xt, gp, nv, err, errlen);
+                if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+                rc = enqueue_forward(p, st, err, errlen);  // warm-up
+                if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
+                HIP_TRY(hipEventRecord(e0, st));
+                for (int t = 0; t < trials; t++) enqueue_forward(p, st, err, errlen);
+                HIP_TRY(hipEventRecord(e1, st));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms /= (float)trials;
+                char line[96];
+                snprintf(line, sizeof line, "xt=%d gp=%d nv=%d %.4f ms\n", xt, gp, nv, ms);
+                rep += line;
+                if (ms < best) { best = ms; bxt = xt; bgp = gp; bnv = nv; }
+            }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    int rc = set_tiles(p, bxt, bgp, err, errlen);
+    int rc = set_tiles(p, bxt, bgp, bnv, err, errlen);
     char line[96];
-    snprintf(line, sizeof line, "best xt=%d gp=%d %.4f ms\n", bxt, bgp, best);
+    snprintf(line, sizeof line, "best xt=%d gp=%d nv=%d %.4f ms\n", bxt, bgp, bnv, best);
     rep += line;
     if (report && reportlen) {
         std::strncpy(report, rep.c_str(), reportlen - 1);
@@ -1179,17 +1248,23 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
     return rc;
 }
 
-extern "C" int mrs_plan_set_tiles(mrs_plan* p, int32_t xt, int32_t gp, char* err, size_t errlen) {
+extern "C" int mrs_plan_set_tiles(mrs_plan* p, int32_t xt, int32_t gp, int32_t units_per_item, char* err,
+                                  size_t errlen) {
     if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
     if (xt < 1 || xt > 64) return fail(err, errlen, MRS_E_INVALID, "x tile must be 1..64");
     if (gp != 1 && gp != 2 && gp != 4) return fail(err, errlen, MRS_E_INVALID, "lines tiles per pass: 1, 2 or 4");
-    return set_tiles(p, xt, gp, err, errlen);
+    if (units_per_item != 1 && units_per_item != 2 && units_per_item != 4)
+        return fail(err, errlen, MRS_E_INVALID, "units per item: 1, 2 or 4");
+    if (units_per_item > 1 && gp != 4)
+        return fail(err, errlen, MRS_E_INVALID, "units per item above 1 need 4 line tiles per pass");
+    return set_tiles(p, xt, gp, units_per_item, err, errlen);
 }
 
-extern "C" int mrs_plan_get_tiles(const mrs_plan* p, int32_t* xt, int32_t* gp) {
+extern "C" int mrs_plan_get_tiles(const mrs_plan* p, int32_t* xt, int32_t* gp, int32_t* units_per_item) {
     if (!p) return MRS_E_INVALID;
     if (xt) *xt = p->xt;
     if (gp) *gp = p->gp;
+    if (units_per_item) *units_per_item = p->nv;
     return MRS_OK;
 }
 

@@ -63,8 +63,8 @@ def lib():
     L.mrs_plan_describe.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_size_t]
     L.mrs_plan_destroy.argtypes = [ct.c_void_p]
     L.mrs_plan_tune.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_char_p, ct.c_size_t]
-    L.mrs_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32] + E
-    L.mrs_plan_get_tiles.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32)]
+    L.mrs_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32] + E
+    L.mrs_plan_get_tiles.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32), P(ct.c_int32)]
     L.mrs_version.restype = ct.c_char_p
     L.mrs_sizeof.argtypes = [ct.c_char_p]
     L.mrs_sizeof.restype = ct.c_size_t
@@ -439,14 +439,14 @@ class Plan:
             raise MrsError(rc, "tuning failed")
         return rep.value.decode()
 
-    def set_tiles(self, x_tile, line_tiles):
+    def set_tiles(self, x_tile, line_tiles, units_per_item=1):
         err = _errbuf()
-        _check(lib().mrs_plan_set_tiles(self._h, x_tile, line_tiles, err, len(err)), err)
+        _check(lib().mrs_plan_set_tiles(self._h, x_tile, line_tiles, units_per_item, err, len(err)), err)
 
     def tiles(self):
-        a, b = ct.c_int32(), ct.c_int32()
-        lib().mrs_plan_get_tiles(self._h, ct.byref(a), ct.byref(b))
-        return a.value, b.value
+        a, b, c = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        lib().mrs_plan_get_tiles(self._h, ct.byref(a), ct.byref(b), ct.byref(c))
+        return a.value, b.value, c.value
 
     def set_zstage(self, sets: Sequence["ZSet"]):
         """Attach partition transforms that run after the 2D forward (and inside its graph)."""

@@ -193,9 +193,57 @@ def test_graph_replay_tuning_and_determinism():
     print(report, flush=True)
     plan.forward()
     torch.cuda.synchronize()
+    # the x tile only redistributes work (bitwise equal); line tiles and unit sets change the
+    # evaluation path, so the tuned result agrees to round-off
     for o, r in zip(outs, ref):
-        assert torch.equal(o, r), "tile choice changed the result"
+        assert rel(o.cpu().numpy(), r.cpu().numpy()) <= 1e-6, "tile choice changed the result"
     plan.close()
+
+
+def _set_plan_outputs(w, tiles):
+    import torch
+    plan, outs, keep = workloads.to_plan(w)
+    plan.set_tiles(*tiles)
+    assert plan.tiles() == tuple(tiles)
+    plan.forward()
+    torch.cuda.synchronize()
+    res = [o.cpu().numpy() for o in outs]
+    desc = plan.describe()
+    plan.close()
+    return res, desc
+
+
+@pytest.mark.parametrize("name,kw,nv", [("batch_128", dict(n_slices=3, n_volumes=6), 2),
+                                        ("unitsets_q1", dict(), 4), ("unitsets_q1", dict(), 2),
+                                        ("unitsets_cplx", dict(), 4)])
+def test_unit_sets_match_single_units_and_oracle(name, kw, nv):
+    # Units of one slice evaluated together in sets of nv (the phase and decay shared): every unit,
+    # in full sets and leftovers, equals its single-unit evaluation to round-off and the FP64 oracle.
+    if name == "unitsets_q1":
+        w = workloads.make("batch_128", n_slices=2, n_volumes=7)
+        w.n_comp = 1
+        w.units = [(si, img[:1].copy()) for si, img in w.units]
+        for sd in w.slices:
+            sd.t2, sd.t_inhom = sd.t2[:1].copy(), sd.t_inhom[:1].copy()
+    elif name == "unitsets_cplx":
+        w = workloads.make("batch_128", n_slices=2, n_volumes=5)
+        w.n_comp, w.complex_images = 1, True
+        w.units = [(si, (img[:1] * np.exp(1j * w.slices[si].phase0)[None, :]).astype(np.complex64))
+                   for si, img in w.units]
+        for sd in w.slices:
+            sd.t2, sd.t_inhom = sd.t2[:1].copy(), sd.t_inhom[:1].copy()
+    else:
+        w = workloads.make(name, **kw)
+    single, _ = _set_plan_outputs(w, (16, 4, 1))
+    sets, desc = _set_plan_outputs(w, (16, 4, nv))
+    assert f"units per item {nv}" in desc
+    worst = max(rel(a, b) for a, b in zip(sets, single))
+    print(f"{name} sets of {nv}: max rel diff to single units {worst:.2e}", flush=True)
+    assert worst <= 1e-6
+    for u in (0, len(w.units) // 2, len(w.units) - 1):
+        e = rel(sets[u], O.forward_workload_unit(w, u))
+        print(f"  unit {u}: rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e <= TOL
 
 
 def test_chunked_streams_batch_matches_oracle_and_replays():
