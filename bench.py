@@ -104,6 +104,18 @@ def grad_pass(iters: int):
     torch.cuda.synchronize()
     t_both = (time.perf_counter() - t0) / iters
     target = model.forward_raw(img, fmap, ti).detach()
+
+    def fit(f, n):
+        opt = torch.optim.Adam([f], lr=0.5)
+        for _ in range(n):
+            opt.zero_grad()
+            loss = (model(img, f, ti) - target).abs().pow(2).sum()
+            loss.backward()
+            opt.step()
+        return loss
+
+    # untimed: the first iterations load torch's loss and optimizer kernels
+    fit((fmap + 5.0).clone().requires_grad_(True), 2)
     f = (fmap + 5.0).clone().requires_grad_(True)
     opt = torch.optim.Adam([f], lr=0.5)
     torch.cuda.synchronize()
@@ -121,7 +133,7 @@ def grad_pass(iters: int):
             "ratio": round(t_both / t_fwd, 3), "fit50_ms": round(1e3 * t_fit, 2), "fit50_final_loss": l_end}
 
 
-def ge3d_pass(steps: int, check: bool = True):
+def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool = False):
     """The 3D gradient-echo train (two volumes, 128^2 -> 64^2 and 192^2 -> 96^2, 24 partitions each,
     32 coils, 1 compartment, 4 echoes): one hipGraph replay per step runs every per-partition 2D
     forward and both volumes' partition transforms with their per-excitation weights."""
@@ -131,8 +143,15 @@ def ge3d_pass(steps: int, check: bool = True):
     from mrsim_amd import workloads
     w = workloads.make_ge3d("ge3d")
     fw, keep = workloads.to_ge3d(w)
-    log(fw.tune(2).strip().splitlines()[-1])
+    if tiles:
+        fw.plan.set_tiles(*(int(v) for v in tiles.split(",")))
+    else:
+        log(fw.tune(2).strip().splitlines()[-1])
     log(fw.describe().strip())
+    if stages_only:
+        st = fw.stage_times(iters=steps)
+        fw.close()
+        return {"stage_ms": st, "tiles": tiles}
     for _ in range(2):
         fw.forward_graph()
     torch.cuda.synchronize()
@@ -164,6 +183,61 @@ def ge3d_pass(steps: int, check: bool = True):
             "rel_l2_vs_fp64_coil7_vol0_echo0": err}
 
 
+def _pmc_per_dispatch(outdir, counter, match):
+    """Average per dispatch of `counter` over kernels whose name contains `match` (rocprofv3 csv)."""
+    import csv
+    import glob
+    tot, ids = 0.0, set()
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        for r in csv.DictReader(open(path)):
+            if match in r["Kernel_Name"] and r["Counter_Name"].startswith(counter):
+                tot += float(r["Counter_Value"])
+                ids.add(r["Dispatch_Id"])
+    return (tot / len(ids), len(ids)) if ids else (None, 0)
+
+
+def measure_traffic(tiles: str, kernel: str, timeout: float = 300.0):
+    """HBM bytes per dispatch of `kernel` in the batch forward: rocprofv3 --pmc FETCH_SIZE, then
+    WRITE_SIZE, each run on its own in a child process (before this process touches the GPU), each
+    scaled by what the same counter reads for a device copy of 256 MiB.  None when rocprofv3 is
+    missing or a pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    if not shutil.which("rocprofv3"):
+        return None
+    me = [sys.executable, "-u", os.path.abspath(__file__)]
+    child = me + ["--steps", "2", "--warmup", "1", "--no-cpu", "--no-plain", "--no-grad", "--no-ge3d",
+                  "--check-units", "0", "--no-traffic", "--tiles", tiles]
+    nbytes = 256 << 20
+    copy = me + ["--calibrate-copy", str(nbytes)]
+    env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
+    got = {}
+    with tempfile.TemporaryDirectory() as d:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            for name, cmd, match in (("bench", child, kernel), ("copy", copy, "copyBuffer")):
+                out = os.path.join(d, f"{name}_{counter}")
+                log(f"traffic pass: {counter} on {name}")
+                try:
+                    r = subprocess.run(["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", out, "-o",
+                                        "r", "--"] + cmd, capture_output=True, text=True, timeout=timeout, env=env)
+                except subprocess.TimeoutExpired:
+                    log(f"traffic pass {counter}/{name} timed out")
+                    return None
+                v, n = _pmc_per_dispatch(out, counter, match)
+                if v is None:
+                    log(f"traffic pass {counter}/{name} gave no counters (rc {r.returncode}): {r.stderr[-400:]}")
+                    return None
+                got[(counter, name)] = (v, n)
+    # counters in KiB per dispatch; the copy moves nbytes each way
+    f_scale = (nbytes / 1024) / got[("FETCH_SIZE", "copy")][0]
+    w_scale = (nbytes / 1024) / got[("WRITE_SIZE", "copy")][0]
+    fetch = got[("FETCH_SIZE", "bench")][0] * f_scale * 1024
+    write = got[("WRITE_SIZE", "bench")][0] * w_scale * 1024
+    return {"fetch_bytes": fetch, "write_bytes": write, "fetch_scale": round(f_scale, 4),
+            "write_scale": round(w_scale, 4), "dispatches": got[("FETCH_SIZE", "bench")][1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,7 +249,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-ge3d", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--traffic-tiles", type=str, default="8,4,2", help="tiles of the traffic passes")
     ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
+    ap.add_argument("--ge3d-tiles", type=str, default="", help="tiles of the 3D train (skips its autotuner)")
+    ap.add_argument("--stages-only", action="store_true",
+                    help="only the per-stage serial launches (profiling: one dispatch per chunk at a time)")
     ap.add_argument("--check-units", type=int, default=2)
     ap.add_argument("--tiles", type=str, default="", help="x_tile,line_tiles[,units_per_item] (skips the autotuner)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -194,13 +273,20 @@ def main():
         print("bench.py measures one GPU: the forward has no cross-GPU step, and multi-GPU runs are not "
               "part of this build (each GPU would take its own share of units).", file=sys.stderr)
         sys.exit(2)
+    # HBM traffic of the y-stage from counter passes in child processes, before this process
+    # initialises the GPU
+    traffic = None
+    if not args.no_traffic and not args.only_ge3d:
+        traffic = measure_traffic(args.traffic_tiles, "ystage_mfma")
+        log(f"y-stage traffic per dispatch: {traffic}")
     import numpy as np
     import torch
 
     from mrsim_amd import abi, workloads
     torch.cuda.set_device(0)
     if args.only_ge3d:
-        print(json.dumps({"ge3d": ge3d_pass(args.steps, check=args.check_units > 0)}), flush=True)
+        print(json.dumps({"ge3d": ge3d_pass(args.steps, check=args.check_units > 0, tiles=args.ge3d_tiles,
+                                            stages_only=args.stages_only)}), flush=True)
         return
     t_gen = time.perf_counter()
     w = workloads.make("batch_128", n_slices=args.slices, n_volumes=args.volumes)
@@ -212,6 +298,11 @@ def main():
     else:
         log(plan.tune(trials=2).strip())
     nunits = len(w.units)
+    if args.stages_only:
+        st_ms = plan.stage_times(iters=args.steps)
+        print(json.dumps({"stage_ms": st_ms, "tiles": plan.tiles(), "describe": plan.describe()}), flush=True)
+        plan.close()
+        return
 
     # warm-up, then K graph replays bracketed by synchronisation
     for _ in range(args.warmup):
@@ -260,7 +351,8 @@ def main():
     grad = None if args.no_grad else grad_pass(max(5, args.steps))
     if grad:
         log(f"gradient pass: {grad}")
-    ge3d = None if args.no_ge3d else ge3d_pass(max(3, args.steps // 4), check=args.check_units > 0)
+    ge3d = None if args.no_ge3d else ge3d_pass(max(3, args.steps // 4), check=args.check_units > 0,
+                                               tiles=args.ge3d_tiles)
     if ge3d:
         log(f"3D gradient-echo train: {ge3d}")
 
@@ -273,22 +365,36 @@ def main():
     trans = lines * P * (Q + 2)
     y_ms = stage_ms["ystage"]
     achieved = trans / (y_ms * 1e-3) / 1e12
+    # HBM bytes of one y-stage pass over the batch (its chunk dispatches), from the counter passes
+    import re
+    nchunks = int(re.search(r"(\d+) chunks", plan.describe()).group(1))
+    traffic_bytes = None
+    if traffic:
+        traffic_bytes = round((traffic["fetch_bytes"] + traffic["write_bytes"]) * nchunks)
+    # algorithmic bytes of the y-stage pass: images, coil maps and static maps in, G out
+    alg_bytes = (nunits * Q * P * 4 + len(w.slices) * P * (4 + 8 + 8 * Q + 8 * C) +
+                 nunits * C * w.snx * (lines / nunits) * 8)
     roofline = {
         "bound": "trans",
         "achieved": round(achieved, 4),
         "peak": round(PEAK_TRANS_TS, 3),
         "unit": "Ttrans/s",
         "frac": round(achieved / PEAK_TRANS_TS, 4),
-        "traffic": None,
+        "traffic": traffic_bytes,
         "kernel": "ystage_mfma",
         "kernel_ms": round(y_ms, 4),
-        "per_unit": f"Ly*P*(Q+2) = {int(lines / nunits)}*{P}*{Q + 2} transcendentals",
-        "secondary": {
-            "valu_fp32_tflops": round(8.0 * C * P * lines / (y_ms * 1e-3) / 1e12, 2),
-            "hbm_gbs": round((nunits * (Q * P * 4 + C * w.nx * lines / nunits * 8) +
-                              len(w.slices) * P * (4 + 8 + 8 * Q + 8 * C)) / (y_ms * 1e-3) / 1e9, 1),
-            "xstage_mfma_tflops": round(3 * 8.0 * C * lines * w.nx * w.snx / (stage_ms["xstage"] * 1e-3) / 1e12, 2),
+        "per_unit": f"Ly*P*(Q+2) = {int(lines / nunits)}*{P}*{Q + 2} transcendentals per slice unit, "
+                    f"{nunits} units per pass",
+        "traffic_detail": traffic,
+        "contract_view": {
+            "hbm": {"achieved": round(alg_bytes / (y_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(alg_bytes / (y_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                    "algorithmic_bytes": round(alg_bytes)},
+            "mfma": {"achieved": round(3 * 8.0 * C * P * lines / (y_ms * 1e-3) / 1e12, 2), "peak": PEAK_BF16_MFMA_TF,
+                     "unit": "TFLOP/s", "frac": round(3 * 8.0 * C * P * lines / (y_ms * 1e-3) / 1e12 / PEAK_BF16_MFMA_TF, 4),
+                     "note": "three split-BF16 products of F_y = 8*C*P*Ly per unit"},
         },
+        "xstage_mfma_tflops": round(3 * 8.0 * C * lines * w.nx * w.snx / (stage_ms["xstage"] * 1e-3) / 1e12, 2),
     }
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
