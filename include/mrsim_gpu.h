@@ -288,6 +288,34 @@ int mrs_zstage(const mrs_zset* sets, size_t nsets, int32_t adjoint, void* stream
  * nsets = 0 detaches.  Sets that accumulate into the same output must not be attached together. */
 int mrs_plan_set_zstage(mrs_plan* plan, const mrs_zset* sets, size_t nsets, char* err, size_t errlen);
 
+/* ---- reconstruction (the observation point of the reference's golden fixtures) -------------- */
+
+/* Batched reconstruction of one geometry: per unit, reconstruct_coils_full (src/kspace.rs:1301-1378)
+ * without GRAPPA -- the window (KspaceWindow::at, :140-161) on every sample, the centred inverse 2D
+ * DFT of each coil (inverse_2d, :1428-1453), the phase-preserving Roemer combine
+ * sum_c s_c img_c / max(sum_c s_c^2, 1e-12) with the real sensitivities `sensitivities`
+ * (n_coils x ny x nx, x + nx*y; NULL = coil_sensitivity on the acquired grid, :1353-1355).
+ * n_coils must be max(acq->n_coils, 1) ("one k-space per coil"); accel > 1 is refused (GRAPPA).
+ * nx, ny <= 128.  The contractions run as split-BF16 MFMA with FP32 accumulation. */
+typedef struct mrs_recon mrs_recon;
+int mrs_recon_create(size_t nx, size_t ny, size_t n_coils, const mrs_acquisition* acq, const float* sensitivities,
+                     mrs_recon** recon, char* err, size_t errlen);
+/* kspace[u]: n_coils x ny x nx complex64 (kx + nx*ky per coil); image[u]: ny x nx complex64.
+ * Device pointers; runs on `stream` and waits for it. */
+int mrs_recon_run(mrs_recon* recon, const mrs_c32* const* kspace, mrs_c32* const* image, size_t n, void* stream,
+                  char* err, size_t errlen);
+/* The same in two steps: upload the unit list once, then launch it asynchronously on `stream`
+ * (no allocation or copy: a HIP graph can record the launch). */
+int mrs_recon_set_units(mrs_recon* recon, const mrs_c32* const* kspace, mrs_c32* const* image, size_t n, char* err,
+                        size_t errlen);
+int mrs_recon_launch(mrs_recon* recon, void* stream, char* err, size_t errlen);
+int mrs_recon_destroy(mrs_recon* recon);
+
+/* reconstruct_coils (src/kspace.rs:1290-1298) for host buffers: coil_kspace n_coils x (kx + nx*ky)
+ * complex f64 -> out (x + nx*y) complex f64, computed on the GPU as mrs_recon_run. */
+int mrs_reconstruct_coils(const mrs_c64* coil_kspace, size_t n_coils, size_t nx, size_t ny,
+                          const mrs_acquisition* acq, mrs_c64* out, char* err, size_t errlen);
+
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);
 

@@ -120,6 +120,26 @@ struct ZBlock {
     int32_t col0;
 };
 
+// Reconstruction of one geometry (reconstruct_coils_full, src/kspace.rs:1301-1378): tables on the
+// device, matrices zero-padded to multiples of 16.
+struct ReconGeom {
+    int32_t nx, ny, nxP, nyP, C;
+    const uint32_t* fy_hi;   // [nyP][nyP] packed bf16 of Fy[y][ky] = cis(-2 pi (ky-ys)(y-ys)/ny)
+    const uint32_t* fy_lo;
+    const uint32_t* fx_hi;   // [nxP][nxP] packed bf16 of Fx[x][kx] = cis(-2 pi (kx-xs)(x-xs)/nx)
+    const uint32_t* fx_lo;
+    const float* window;     // [ny][nx] reconstruction window, or null
+    const float* sens;       // [C][nyP][nxP] Roemer sensitivities (real)
+    const float* rssum;      // [nyP][nxP] 1 / max(sum_c sens^2, 1e-12)
+};
+
+struct ReconItem {
+    const float2* kspace;    // C x ny x nx complex64
+    float2* image;           // ny x nx complex64 (x + nx*y)
+};
+
+double ring_sensitivity(size_t coil, size_t n_coils, double x, double y, size_t nx, size_t ny);
+
 int build_zsets(const mrs_zset* in, size_t n, bool adjoint, std::vector<ZSet>& sets, std::vector<ZBlock>& blocks,
                 int off[4], char* err, size_t errlen);
 hipError_t enqueue_zsets(const ZSet* d_sets, const ZBlock* d_blocks, const int off[4], bool adjoint, hipStream_t st);

@@ -1096,6 +1096,187 @@ __global__ __launch_bounds__(256) void zstage(const ZSet* __restrict__ sets, con
     }
 }
 
+// ----------------------------------------------------------------------------- reconstruction
+// reconstruct_coils_full (src/kspace.rs:1301-1378) without GRAPPA: per coil the window, the centred
+// inverse 2D DFT img_c = Fy . (W o K_c) . Fx (inverse_2d, :1428-1453), then the Roemer combine
+// sum_c s_c img_c / max(sum_c s_c^2, 1e-12).
+// A block owns (unit, 16-column x tile); each of its four waves a quarter of the coils.  The block
+// stages Fy (all of it) and its x tile's columns of Fx in LDS, split hi/lo, once.  Stage 1,
+// D1 = K . Fx for every 16-row ky tile of the x tile: K rows as the A operand (four consecutive kx
+// per lane, straight from HBM/L2), Fx columns as B with the complex structure.  An MFMA result tile
+// (rows 4*lq + r, column ln) is laid out exactly as a B operand (k = 4*lq + j, column ln), so
+// stage 2, img = Fy . D1, takes D1 from the wave's own registers -- split to hi/lo once per ky tile
+// -- with Fy rows as A.  Each coil's image tile is weighted by its sensitivity into the wave's
+// running combine; the four coil groups' combines are summed in LDS in a fixed order
+// (deterministic), reusing the Fy area.
+// NT = the 16-row tiles of ny (1..8, ny <= 128), which size the register arrays.
+template <int NT>
+__global__ __launch_bounds__(256) void recon_units(const ReconGeom* __restrict__ gp, const ReconItem* __restrict__ items,
+                                                   int blocks_per_unit) {
+    extern __shared__ uint32_t rlds[];
+    const ReconGeom G = *gp;
+    const int item = blockIdx.x / blocks_per_unit, t = blockIdx.x % blocks_per_unit;
+    const ReconItem it = items[item];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int lq = lane >> 4, ln = lane & 15;
+    const int ntx = G.nxP >> 4;
+    const int nx = G.nx, ny = G.ny, nyP = NT * 16, nxP = G.nxP;
+    const int fys = nyP + 4, fxs = nxP + 4;                 // LDS row strides (words)
+    uint32_t* fy_h = rlds;                                    // [nyP][fys]
+    uint32_t* fy_l = rlds + nyP * fys;
+    const int fy_words = max(2 * nyP * fys, 4 * nyP * 16 * 2);
+    uint32_t* fx_h = rlds + fy_words;                         // [16][fxs]: this x tile's rows of Fx^T
+    uint32_t* fx_l = fx_h + 16 * fxs;
+    for (int i = threadIdx.x; i < nyP * nyP; i += 256) {
+        const int r = i / nyP, k = i % nyP;
+        fy_h[r * fys + k] = gld(G.fy_hi)[i];
+        fy_l[r * fys + k] = gld(G.fy_lo)[i];
+    }
+    for (int i = threadIdx.x; i < 16 * nxP; i += 256) {
+        const int r = i / nxP, k = i % nxP;
+        fx_h[r * fxs + k] = gld(G.fx_hi)[(t * 16 + r) * nxP + k];
+        fx_l[r * fxs + k] = gld(G.fx_lo)[(t * 16 + r) * nxP + k];
+    }
+    __syncthreads();
+    const int x = t * 16 + ln;
+    f32x4 cr[NT], ci[NT];
+#pragma unroll
+    for (int i = 0; i < NT; i++) cr[i] = ci[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const GAS f32x2* __restrict__ kall = reinterpret_cast<const GAS f32x2*>(gld(it.kspace));
+    const bool vec = (nx & 1) == 0;   // rows of K start 16-byte aligned
+    for (int c = grp; c < G.C; c += 4) {
+        const GAS f32x2* __restrict__ K = kall + (size_t)c * ny * nx;
+        u32x4 dh[NT], dl[NT];
+#pragma unroll
+        for (int kt = 0; kt < NT; kt++) {
+            // keep each tile's loads in its own iteration: hoisting all of them costs occupancy
+            __asm__ volatile("" ::: "memory");
+            f32x4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+            const int ky = kt * 16 + ln;
+            for (int kb = 0; kb < ntx; kb++) {
+                const int kx0 = kb * 16 + lq * 4;
+                f32x2 v[4];
+                if (ky < ny && vec && kx0 + 3 < nx) {
+                    const f32x4 p0 = *reinterpret_cast<const GAS f32x4*>(K + ky * nx + kx0);
+                    const f32x4 p1 = *reinterpret_cast<const GAS f32x4*>(K + ky * nx + kx0 + 2);
+                    v[0] = f32x2{p0.x, p0.y}; v[1] = f32x2{p0.z, p0.w};
+                    v[2] = f32x2{p1.x, p1.y}; v[3] = f32x2{p1.z, p1.w};
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        v[j] = (ky < ny && kx0 + j < nx) ? K[ky * nx + kx0 + j] : f32x2{0.f, 0.f};
+                }
+                u32x4 ah, al;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (G.window && ky < ny && kx0 + j < nx) {
+                        const float w = gld(G.window)[ky * nx + kx0 + j];
+                        v[j].x *= w;
+                        v[j].y *= w;
+                    }
+                    uint32_t h, l;
+                    split2(v[j].x, v[j].y, h, l);
+                    ah[j] = h;
+                    al[j] = l;
+                }
+                const u32x4 bh = *reinterpret_cast<const u32x4*>(fx_h + ln * fxs + kx0);
+                const u32x4 bl = *reinterpret_cast<const u32x4*>(fx_l + ln * fxs + kx0);
+                ar = mfma3(ah, al, rows_of(bh, 0), rows_of(bl, 0), ar);
+                ai = mfma3(ah, al, rows_of(bh, 1), rows_of(bl, 1), ai);
+            }
+            // D1 tile (ky = kt*16 + 4*lq + r, x) is this lane's B operand of stage 2
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                uint32_t h, l;
+                split2(ar[r], ai[r], h, l);
+                dh[kt][r] = h;
+                dl[kt][r] = l;
+            }
+        }
+#pragma unroll
+        for (int yt = 0; yt < NT; yt++) {
+            __asm__ volatile("" ::: "memory");
+            f32x4 ir = {0.f, 0.f, 0.f, 0.f}, ii = {0.f, 0.f, 0.f, 0.f};
+            const int ao = (yt * 16 + ln) * fys + lq * 4;
+#pragma unroll
+            for (int kb = 0; kb < NT; kb++) {
+                const u32x4 ah = *reinterpret_cast<const u32x4*>(fy_h + ao + kb * 16);
+                const u32x4 al = *reinterpret_cast<const u32x4*>(fy_l + ao + kb * 16);
+                ir = mfma3(ah, al, rows_of(dh[kb], 0), rows_of(dl[kb], 0), ir);
+                ii = mfma3(ah, al, rows_of(dh[kb], 1), rows_of(dl[kb], 1), ii);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int y = yt * 16 + lq * 4 + r;
+                const float sv = gld(G.sens)[((size_t)c * nyP + y) * nxP + x];
+                cr[yt][r] = __builtin_fmaf(sv, ir[r], cr[yt][r]);
+                ci[yt][r] = __builtin_fmaf(sv, ii[r], ci[yt][r]);
+            }
+        }
+    }
+    // the coil groups' combines, summed in group order in the Fy area
+    __syncthreads();
+    float* red = reinterpret_cast<float*>(rlds);   // [4][nyP][16][2]
+#pragma unroll
+    for (int yt = 0; yt < NT; yt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int o = ((grp * nyP + yt * 16 + lq * 4 + r) * 16 + ln) * 2;
+            red[o] = cr[yt][r];
+            red[o + 1] = ci[yt][r];
+        }
+    __syncthreads();
+    if (grp != 0) return;
+    GAS f32x2* out = reinterpret_cast<GAS f32x2*>(gst(it.image));
+#pragma unroll
+    for (int yt = 0; yt < NT; yt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int y = yt * 16 + lq * 4 + r;
+            float sr = cr[yt][r], si = ci[yt][r];
+            for (int g = 1; g < 4; g++) {
+                const int o = ((g * nyP + y) * 16 + ln) * 2;
+                sr += red[o];
+                si += red[o + 1];
+            }
+            if (x < nx && y < ny) {
+                const float rs = gld(G.rssum)[y * nxP + x];
+                out[y * nx + x] = f32x2{sr * rs, si * rs};
+            }
+        }
+}
+
+size_t recon_lds_bytes(int nxP, int nyP) {
+    const size_t fy_words = std::max((size_t)2 * nyP * (nyP + 4), (size_t)4 * nyP * 16 * 2);
+    return (fy_words + 2 * 16 * (size_t)(nxP + 4)) * 4;
+}
+
+hipError_t set_recon_lds_limit(size_t bytes) {
+    hipError_t e;
+#define RL(N) if ((e = hipFuncSetAttribute((const void*)recon_units<N>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
+    RL(1) RL(2) RL(3) RL(4) RL(5) RL(6) RL(7) RL(8)
+#undef RL
+    return hipSuccess;
+}
+
+hipError_t launch_recon(const ReconGeom* g, const ReconItem* items, int nitems, int blocks_per_unit, int nty,
+                        size_t lds, hipStream_t st) {
+    if (nitems <= 0) return hipSuccess;
+    const dim3 grid(nitems * blocks_per_unit), block(256);
+    switch (nty) {
+    case 1: hipLaunchKernelGGL((recon_units<1>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 2: hipLaunchKernelGGL((recon_units<2>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 3: hipLaunchKernelGGL((recon_units<3>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 4: hipLaunchKernelGGL((recon_units<4>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 5: hipLaunchKernelGGL((recon_units<5>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 6: hipLaunchKernelGGL((recon_units<6>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 7: hipLaunchKernelGGL((recon_units<7>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 8: hipLaunchKernelGGL((recon_units<8>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------- epilogues
 // Unacquired rows are exactly zero (src/kspace.rs:839-843).  grid.y = unit.
 __global__ void zero_rows(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups, int unit0) {

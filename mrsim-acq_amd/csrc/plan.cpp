@@ -137,17 +137,6 @@ std::vector<uint8_t> acquired_rows(size_t ny, const mrs_acquisition& a) {
     return rows;
 }
 
-double ring_sensitivity(size_t coil, size_t n_coils, double x, double y, size_t nx, size_t ny) {
-    if (n_coils <= 1) return 1.0;
-    const double cx = (double)nx / 2.0, cy = (double)ny / 2.0;
-    const double m = (double)std::max(nx, ny);
-    const double r = 0.6 * m;
-    const double ang = kTau * (double)coil / (double)n_coils;
-    const double px = cx + r * std::cos(ang), py = cy + r * std::sin(ang);
-    const double dx = x - px, dy = y - py;
-    const double sigma = 0.9 * m;
-    return std::exp(-(dx * dx + dy * dy) / (2.0 * sigma * sigma)) + 0.15;
-}
 
 uint16_t bf16_rne(float f) {
     uint32_t u;
@@ -182,6 +171,21 @@ int dev_alloc(T** p, size_t n, char* err, size_t errlen) {
 }
 
 }  // namespace
+
+namespace mrs {
+// coil_sensitivity (src/kspace.rs:226-237): the real ring Gaussian plus 0.15, in acquired-voxel units.
+double ring_sensitivity(size_t coil, size_t n_coils, double x, double y, size_t nx, size_t ny) {
+    if (n_coils <= 1) return 1.0;
+    const double cx = (double)nx / 2.0, cy = (double)ny / 2.0;
+    const double m = (double)std::max(nx, ny);
+    const double r = 0.6 * m;
+    const double ang = kTau * (double)coil / (double)n_coils;
+    const double px = cx + r * std::cos(ang), py = cy + r * std::sin(ang);
+    const double dx = x - px, dy = y - py;
+    const double sigma = 0.9 * m;
+    return std::exp(-(dx * dx + dy * dy) / (2.0 * sigma * sigma)) + 0.15;
+}
+}  // namespace mrs
 
 // ------------------------------------------------------------------ plan
 

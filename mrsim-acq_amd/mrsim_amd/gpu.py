@@ -77,6 +77,14 @@ def lib():
                                         ct.c_size_t, ct.c_double, ct.c_int32, ct.c_size_t, ct.c_double,
                                         ct.c_double] + E
     L.mrs_zstage.argtypes = [P(abi.ZSetDesc), ct.c_size_t, ct.c_int32, ct.c_void_p] + E
+    L.mrs_recon_create.argtypes = [ct.c_size_t, ct.c_size_t, ct.c_size_t, P(abi.Acquisition), P(ct.c_float),
+                                   P(ct.c_void_p)] + E
+    L.mrs_recon_run.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), ct.c_size_t, ct.c_void_p] + E
+    L.mrs_recon_destroy.argtypes = [ct.c_void_p]
+    L.mrs_recon_set_units.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), ct.c_size_t] + E
+    L.mrs_recon_launch.argtypes = [ct.c_void_p, ct.c_void_p] + E
+    L.mrs_reconstruct_coils.argtypes = [P(abi.C64), ct.c_size_t, ct.c_size_t, ct.c_size_t, P(abi.Acquisition),
+                                        P(abi.C64)] + E
     L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
     _lib = L
     return L
@@ -210,6 +218,76 @@ def check_ge3d_timing(echo_times_ms, exc_spacing_ms, excitation_time_ms, ny, ky_
     _check(lib().mrs_check_ge3d_timing(abi.dptr(tes), len(tes), exc_spacing_ms, excitation_time_ms, ny, ky_segments,
                                        t_line_ms, 1 if reverse_phase else 0, n_exc, t_exc_ms, tr_ms, err, len(err)),
            err)
+
+
+def reconstruct_coils(coil_kspace: np.ndarray, nx: int, ny: int, acq: abi.Acquisition) -> np.ndarray:
+    """mrs_reconstruct_coils (reconstruct_coils, src/kspace.rs:1290-1298): (C, nx*ny) complex -> (ny*nx,)
+    complex128 combined image, x + nx*y, computed on the GPU."""
+    k = np.ascontiguousarray(coil_kspace, dtype=np.complex128)
+    out = np.zeros(nx * ny, dtype=np.complex128)
+    err = _errbuf()
+    _check(lib().mrs_reconstruct_coils(k.ctypes.data_as(ct.POINTER(abi.C64)), k.shape[0], nx, ny, ct.byref(acq),
+                                       out.ctypes.data_as(ct.POINTER(abi.C64)), err, len(err)), err)
+    return out
+
+
+class Recon:
+    """Batched reconstruction of one geometry (mrs_recon_*): window, centred inverse 2D DFT per coil,
+    Roemer combine.  `run` takes lists of device tensors: k-spaces (C, ny, nx) complex64 -> images
+    (ny, nx) complex64."""
+
+    def __init__(self, nx: int, ny: int, n_coils: int, acq: abi.Acquisition, sensitivities=None):
+        self.nx, self.ny, self.C = nx, ny, n_coils
+        sens = None
+        if sensitivities is not None:
+            sens = np.ascontiguousarray(sensitivities, dtype=np.float32)
+        h = ct.c_void_p()
+        err = _errbuf()
+        _check(lib().mrs_recon_create(nx, ny, n_coils, ct.byref(acq),
+                                      None if sens is None else sens.ctypes.data_as(ct.POINTER(ct.c_float)),
+                                      ct.byref(h), err, len(err)), err)
+        self._h = h
+
+    def bind(self, kspaces, images):
+        """Upload the unit list (k-spaces (C, ny, nx) and images (ny, nx), complex64 device tensors)."""
+        import torch
+        n = len(kspaces)
+        if len(images) != n:
+            raise ValueError("one image per k-space")
+        kp, ip = (ct.c_void_p * max(n, 1))(), (ct.c_void_p * max(n, 1))()
+        for i, (k, im) in enumerate(zip(kspaces, images)):
+            _check_dev(f"kspace {i}", k, torch.complex64, self.C * self.ny * self.nx)
+            _check_dev(f"image {i}", im, torch.complex64, self.ny * self.nx)
+            kp[i], ip[i] = k.data_ptr(), im.data_ptr()
+        self._bound = (list(kspaces), list(images))
+        err = _errbuf()
+        _check(lib().mrs_recon_set_units(self._h, kp, ip, n, err, len(err)), err)
+
+    def launch(self, stream=None):
+        """Run the bound units asynchronously on `stream` (torch's current stream by default)."""
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        err = _errbuf()
+        _check(lib().mrs_recon_launch(self._h, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+    def run(self, kspaces, images, stream=None):
+        """bind + launch, then wait for the stream."""
+        import torch
+        self.bind(kspaces, images)
+        self.launch(stream)
+        (stream or torch.cuda.current_stream()).synchronize()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mrs_recon_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

@@ -151,3 +151,15 @@ def test_zstage_refusals_before_any_device_work(L):
     assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -1 and b"not in place" in err.value
     z.out, z.n_comp, z.comp_stride = 32, 2, 10
     assert L.mrs_zstage(ct.byref(z), 1, 0, None, err, 256) == -1 and b"overlap" in err.value
+
+
+def test_recon_refusals_before_any_device_work(L):
+    acq = abi.Acquisition.default(n_coils=4, accel=2)
+    h = ct.c_void_p()
+    err = ct.create_string_buffer(256)
+    assert L.mrs_recon_create(16, 16, 4, ct.byref(acq), None, ct.byref(h), err, 256) == -4
+    assert b"GRAPPA" in err.value
+    acq = abi.Acquisition.default(n_coils=4)
+    assert L.mrs_recon_create(16, 16, 3, ct.byref(acq), None, ct.byref(h), err, 256) == -1
+    assert b"one k-space per coil" in err.value
+    assert L.mrs_recon_create(200, 16, 4, ct.byref(acq), None, ct.byref(h), err, 256) == -4
