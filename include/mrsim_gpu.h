@@ -295,8 +295,9 @@ int mrs_plan_set_zstage(mrs_plan* plan, const mrs_zset* sets, size_t nsets, char
  * DFT of each coil (inverse_2d, :1428-1453), the phase-preserving Roemer combine
  * sum_c s_c img_c / max(sum_c s_c^2, 1e-12) with the real sensitivities `sensitivities`
  * (n_coils x ny x nx, x + nx*y; NULL = coil_sensitivity on the acquired grid, :1353-1355).
- * n_coils must be max(acq->n_coils, 1) ("one k-space per coil"); accel > 1 is refused (GRAPPA).
- * nx, ny <= 128.  The contractions run as split-BF16 MFMA with FP32 accumulation. */
+ * n_coils must be max(acq->n_coils, 1) ("one k-space per coil").  Under acceleration the caller applies
+ * GRAPPA to the k-spaces first (mrs_grappa_apply).  nx, ny <= 128.  The contractions run as
+ * split-BF16 MFMA with FP32 accumulation. */
 typedef struct mrs_recon mrs_recon;
 int mrs_recon_create(size_t nx, size_t ny, size_t n_coils, const mrs_acquisition* acq, const float* sensitivities,
                      mrs_recon** recon, char* err, size_t errlen);
@@ -311,10 +312,33 @@ int mrs_recon_set_units(mrs_recon* recon, const mrs_c32* const* kspace, mrs_c32*
 int mrs_recon_launch(mrs_recon* recon, void* stream, char* err, size_t errlen);
 int mrs_recon_destroy(mrs_recon* recon);
 
+/* GRAPPA (src/kspace.rs:1498-1633), in FP64 on the GPU.  Weights: (accel-1) x n_coils x 6*n_coils complex
+ * f64, weights[d-1][output coil][feature], feature (line, point, coil) = 2 source lines x readout
+ * offsets {-1, 0, +1} (wrapped) x coils.
+ * mrs_grappa_fit: GrappaKernel::fit (:1540-1598) on a fully sampled reference scan `ref_kspace`
+ * (device, n_coils x ny x nx complex64) over the acs_lines about ny/2; `lines` (host, ny flags) are
+ * the PE lines the reference scan read; `weights` is device memory.  Tikhonov 1e-3 of the mean
+ * diagonal; solve_hermitian (:1457-1491).  Waits for `stream`. */
+int mrs_grappa_fit(const mrs_c32* ref_kspace, size_t n_coils, size_t nx, size_t ny, size_t accel, size_t acs_lines,
+                   const uint8_t* lines, mrs_c64* weights, void* stream, char* err, size_t errlen);
+/* GrappaKernel::apply (:1605-1633): synthesize every missing PE line of n units' undersampled coil
+ * k-spaces (device pointers, n_coils x ny x nx complex64) in place; `acquired` (host, ny flags) are
+ * the lines the units read.  Waits for `stream`. */
+int mrs_grappa_apply(mrs_c32* const* kspace, size_t n, size_t n_coils, size_t nx, size_t ny, size_t accel,
+                     const uint8_t* acquired, const mrs_c64* weights, void* stream, char* err, size_t errlen);
+
 /* reconstruct_coils (src/kspace.rs:1290-1298) for host buffers: coil_kspace n_coils x (kx + nx*ky)
- * complex f64 -> out (x + nx*y) complex f64, computed on the GPU as mrs_recon_run. */
+ * complex f64 -> out (x + nx*y) complex f64, on the GPU.  Under acceleration `grappa_weights`
+ * (host, as mrs_grappa_fit writes them) is required and applied first. */
 int mrs_reconstruct_coils(const mrs_c64* coil_kspace, size_t n_coils, size_t nx, size_t ny,
-                          const mrs_acquisition* acq, mrs_c64* out, char* err, size_t errlen);
+                          const mrs_acquisition* acq, const mrs_c64* grappa_weights, mrs_c64* out, char* err,
+                          size_t errlen);
+
+/* simulate_slice (src/kspace.rs:1229-1235, 1273-1284) entirely on the GPU: every coil's k-space,
+ * under acceleration the GRAPPA kernel fitted on the slice's own reference scan (slice seed salted by
+ * 0x4752_5043_414C, no eddy or preparation drive, no acceleration, no spikes) and applied, then the
+ * reconstruction.  out: nx*ny interleaved (re, im) f32, the reference's Vec<(f32, f32)>. */
+int mrs_simulate_slice(const mrs_slice_input* inp, const mrs_acquisition* acq, float* out, char* err, size_t errlen);
 
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);

@@ -126,9 +126,6 @@ extern "C" int mrs_recon_create(size_t nx, size_t ny, size_t n_coils, const mrs_
     if (nx < 1 || ny < 1) return fail(err, errlen, MRS_E_INVALID, "empty acquired matrix");
     if (nx > 128 || ny > 128)
         return fail(err, errlen, MRS_E_UNSUPPORTED, "reconstruction supports nx, ny <= 128 (got %zux%zu)", nx, ny);
-    if (acq->accel > 1)
-        return fail(err, errlen, MRS_E_UNSUPPORTED,
-                    "an accelerated slice needs its GRAPPA kernel (fitted on a reference scan): not built on the GPU");
     mrs_recon* r = new mrs_recon();
     const size_t nxP = (nx + 15) & ~size_t(15), nyP = (ny + 15) & ~size_t(15);
     ReconGeom& g = r->g;
@@ -247,25 +244,39 @@ extern "C" int mrs_recon_destroy(mrs_recon* r) {
 }
 
 // reconstruct_coils (src/kspace.rs:1290-1298) for host buffers: coil k-spaces (n_coils x nx*ny,
-// kx + nx*ky, complex f64) -> the combined image (x + nx*y, complex f64).
+// kx + nx*ky, complex f64) -> the combined image (x + nx*y, complex f64); GRAPPA first under
+// acceleration.
 extern "C" int mrs_reconstruct_coils(const mrs_c64* coil_kspace, size_t n_coils, size_t nx, size_t ny,
-                                     const mrs_acquisition* acq, mrs_c64* out, char* err, size_t errlen) {
-    if (!coil_kspace || !out) return fail(err, errlen, MRS_E_INVALID, "null argument");
+                                     const mrs_acquisition* acq, const mrs_c64* grappa_weights, mrs_c64* out,
+                                     char* err, size_t errlen) {
+    if (!coil_kspace || !out || !acq) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    if (acq->accel > 1 && !grappa_weights)
+        return fail(err, errlen, MRS_E_INVALID, "an accelerated slice needs its GRAPPA kernel (fitted on a reference scan)");
     mrs_recon* r = nullptr;
     int rc = mrs_recon_create(nx, ny, n_coils, acq, nullptr, &r, err, errlen);
     if (rc) return rc;
     const size_t nk = n_coils * nx * ny;
+    const size_t nw = acq->accel > 1 ? (acq->accel - 1) * n_coils * 6 * n_coils : 0;
     std::vector<float2> hk(nk), hi(nx * ny);
     for (size_t i = 0; i < nk; i++) hk[i] = make_float2((float)coil_kspace[i].re, (float)coil_kspace[i].im);
     float2 *dk = nullptr, *di = nullptr;
+    mrs_c64* dw = nullptr;
     hipError_t e = hipMalloc(&dk, nk * sizeof(float2));
     if (e == hipSuccess) e = hipMalloc(&di, nx * ny * sizeof(float2));
     if (e == hipSuccess) e = hipMemcpy(dk, hk.data(), nk * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nw) e = hipMalloc(&dw, nw * sizeof(mrs_c64));
+    if (e == hipSuccess && nw) e = hipMemcpy(dw, grappa_weights, nw * sizeof(mrs_c64), hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = fail(err, errlen, MRS_E_DEVICE, "device buffers: %s", hipGetErrorString(e));
+    mrs_c32* kp = reinterpret_cast<mrs_c32*>(dk);
+    if (!rc && nw) {
+        std::vector<uint8_t> mask(ny);
+        mrs_sampling_mask(1, ny, acq, mask.data());
+        rc = mrs_grappa_apply(&kp, 1, n_coils, nx, ny, acq->accel, mask.data(), dw, nullptr, err, errlen);
+    }
     if (!rc) {
-        const mrs_c32* kp = reinterpret_cast<const mrs_c32*>(dk);
+        const mrs_c32* kc = kp;
         mrs_c32* ip = reinterpret_cast<mrs_c32*>(di);
-        rc = mrs_recon_run(r, &kp, &ip, 1, nullptr, err, errlen);
+        rc = mrs_recon_run(r, &kc, &ip, 1, nullptr, err, errlen);
     }
     if (!rc && (e = hipMemcpy(hi.data(), di, nx * ny * sizeof(float2), hipMemcpyDeviceToHost)) != hipSuccess)
         rc = fail(err, errlen, MRS_E_DEVICE, "copy back: %s", hipGetErrorString(e));
@@ -273,6 +284,62 @@ extern "C" int mrs_reconstruct_coils(const mrs_c64* coil_kspace, size_t n_coils,
         for (size_t i = 0; i < nx * ny; i++) out[i] = mrs_c64{(double)hi[i].x, (double)hi[i].y};
     hipFree(dk);
     hipFree(di);
+    hipFree(dw);
     mrs_recon_destroy(r);
     return rc;
+}
+
+// simulate_slice (src/kspace.rs:1229-1235, 1273-1284) on the GPU: coil k-spaces from the drop-in
+// forward, the GRAPPA kernel fitted on the slice's own reference scan under acceleration
+// (grappa_reference_kernel, :1256-1262, on the seed salted by SEED_SALT_GRAPPA_CAL), the
+// reconstruction, and the f32 cast of the combined image.
+extern "C" int mrs_simulate_slice(const mrs_slice_input* inp, const mrs_acquisition* acq, float* out, char* err,
+                                  size_t errlen) {
+    if (!inp || !acq || !out) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    const size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    const size_t nc = std::max<uint64_t>(acq->n_coils, 1);
+    const size_t nk = nc * nx * ny;
+    std::vector<mrs_c64> coils(nk), img(nx * ny);
+    std::vector<mrs_c64> w;
+    int rc = MRS_OK;
+    if (acq->accel > 1) {
+        mrs_slice_input ref = *inp;
+        ref.slice_seed = inp->slice_seed ^ 0x47525043414CULL;
+        ref.has_eddy_drive = 0;
+        ref.has_prep_drive = 0;
+        mrs_acquisition acq_ref = *acq;
+        acq_ref.accel = 1;
+        acq_ref.n_spikes = 0;
+        rc = mrs_build_coil_kspaces(&ref, &acq_ref, nullptr, nullptr, coils.data(), err, errlen);
+        if (rc) return rc;
+        std::vector<float2> hk(nk);
+        for (size_t i = 0; i < nk; i++) hk[i] = make_float2((float)coils[i].re, (float)coils[i].im);
+        std::vector<uint8_t> lines(ny);
+        mrs_sampling_mask(1, ny, &acq_ref, lines.data());
+        const size_t nw = (acq->accel - 1) * nc * 6 * nc;
+        w.resize(nw);
+        float2* dk = nullptr;
+        mrs_c64* dw = nullptr;
+        hipError_t e = hipMalloc(&dk, nk * sizeof(float2));
+        if (e == hipSuccess) e = hipMalloc(&dw, nw * sizeof(mrs_c64));
+        if (e == hipSuccess) e = hipMemcpy(dk, hk.data(), nk * sizeof(float2), hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = fail(err, errlen, MRS_E_DEVICE, "device buffers: %s", hipGetErrorString(e));
+        if (!rc)
+            rc = mrs_grappa_fit(reinterpret_cast<const mrs_c32*>(dk), nc, nx, ny, acq->accel, acq->acs_lines,
+                                lines.data(), dw, nullptr, err, errlen);
+        if (!rc && (e = hipMemcpy(w.data(), dw, nw * sizeof(mrs_c64), hipMemcpyDeviceToHost)) != hipSuccess)
+            rc = fail(err, errlen, MRS_E_DEVICE, "copy back: %s", hipGetErrorString(e));
+        hipFree(dk);
+        hipFree(dw);
+        if (rc) return rc;
+    }
+    rc = mrs_build_coil_kspaces(inp, acq, nullptr, nullptr, coils.data(), err, errlen);
+    if (!rc) rc = mrs_reconstruct_coils(coils.data(), nc, nx, ny, acq, w.empty() ? nullptr : w.data(), img.data(), err,
+                                        errlen);
+    if (rc) return rc;
+    for (size_t i = 0; i < nx * ny; i++) {
+        out[2 * i] = (float)img[i].re;
+        out[2 * i + 1] = (float)img[i].im;
+    }
+    return MRS_OK;
 }

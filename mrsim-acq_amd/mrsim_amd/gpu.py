@@ -84,7 +84,12 @@ def lib():
     L.mrs_recon_set_units.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), ct.c_size_t] + E
     L.mrs_recon_launch.argtypes = [ct.c_void_p, ct.c_void_p] + E
     L.mrs_reconstruct_coils.argtypes = [P(abi.C64), ct.c_size_t, ct.c_size_t, ct.c_size_t, P(abi.Acquisition),
-                                        P(abi.C64)] + E
+                                        P(abi.C64), P(abi.C64)] + E
+    L.mrs_grappa_fit.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                                 P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
+    L.mrs_grappa_apply.argtypes = [P(ct.c_void_p), ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                                   P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
+    L.mrs_simulate_slice.argtypes = [P(abi.SliceInput), P(abi.Acquisition), P(ct.c_float)] + E
     L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
     _lib = L
     return L
@@ -220,15 +225,60 @@ def check_ge3d_timing(echo_times_ms, exc_spacing_ms, excitation_time_ms, ny, ky_
            err)
 
 
-def reconstruct_coils(coil_kspace: np.ndarray, nx: int, ny: int, acq: abi.Acquisition) -> np.ndarray:
+def reconstruct_coils(coil_kspace: np.ndarray, nx: int, ny: int, acq: abi.Acquisition,
+                      grappa_weights: Optional[np.ndarray] = None) -> np.ndarray:
     """mrs_reconstruct_coils (reconstruct_coils, src/kspace.rs:1290-1298): (C, nx*ny) complex -> (ny*nx,)
-    complex128 combined image, x + nx*y, computed on the GPU."""
+    complex128 combined image, x + nx*y, computed on the GPU (GRAPPA first under acceleration)."""
     k = np.ascontiguousarray(coil_kspace, dtype=np.complex128)
     out = np.zeros(nx * ny, dtype=np.complex128)
+    w = None if grappa_weights is None else np.ascontiguousarray(grappa_weights, dtype=np.complex128)
     err = _errbuf()
     _check(lib().mrs_reconstruct_coils(k.ctypes.data_as(ct.POINTER(abi.C64)), k.shape[0], nx, ny, ct.byref(acq),
+                                       None if w is None else w.ctypes.data_as(ct.POINTER(abi.C64)),
                                        out.ctypes.data_as(ct.POINTER(abi.C64)), err, len(err)), err)
     return out
+
+
+def grappa_fit(ref_kspace, accel: int, acs_lines: int, lines: np.ndarray, stream=None):
+    """mrs_grappa_fit: ref_kspace (C, ny, nx) complex64 device tensor, lines (ny,) flags of the reference
+    scan -> weights (accel-1, C, 6C) complex128 device tensor."""
+    import torch
+    C, ny, nx = ref_kspace.shape
+    _check_dev("ref_kspace", ref_kspace, torch.complex64, C * ny * nx)
+    w = torch.zeros((max(accel - 1, 1), C, 6 * C), dtype=torch.complex128, device=ref_kspace.device)
+    fl = np.ascontiguousarray(lines, dtype=np.uint8)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_grappa_fit(ref_kspace.data_ptr(), C, nx, ny, accel, acs_lines,
+                                fl.ctypes.data_as(ct.POINTER(ct.c_uint8)), w.data_ptr(),
+                                ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+    return w
+
+
+def grappa_apply(kspaces, accel: int, acquired: np.ndarray, weights, stream=None):
+    """mrs_grappa_apply: synthesize the missing lines of each (C, ny, nx) complex64 device tensor in place."""
+    import torch
+    n = len(kspaces)
+    C, ny, nx = kspaces[0].shape
+    ptrs = (ct.c_void_p * max(n, 1))()
+    for i, k in enumerate(kspaces):
+        _check_dev(f"kspace {i}", k, torch.complex64, C * ny * nx)
+        ptrs[i] = k.data_ptr()
+    _check_dev("weights", weights, torch.complex128, max(accel - 1, 1) * C * 6 * C)
+    fl = np.ascontiguousarray(acquired, dtype=np.uint8)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_grappa_apply(ptrs, n, C, nx, ny, accel, fl.ctypes.data_as(ct.POINTER(ct.c_uint8)),
+                                  weights.data_ptr(), ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+
+def simulate_slice(b: abi.SliceInputBuilder, acq: abi.Acquisition) -> np.ndarray:
+    """mrs_simulate_slice (simulate_slice, src/kspace.rs:1229-1235) on the GPU: (nx*ny,) complex64 image."""
+    nx, ny = b.acq_matrix
+    out = np.zeros(2 * nx * ny, dtype=np.float32)
+    err = _errbuf()
+    _check(lib().mrs_simulate_slice(ct.byref(b.inp), ct.byref(acq), abi.fptr(out), err, len(err)), err)
+    return out[0::2] + 1j * out[1::2]
 
 
 class Recon:
