@@ -157,8 +157,13 @@ def test_recon_refusals_before_any_device_work(L):
     acq = abi.Acquisition.default(n_coils=4, accel=2)
     h = ct.c_void_p()
     err = ct.create_string_buffer(256)
-    assert L.mrs_recon_create(16, 16, 4, ct.byref(acq), None, ct.byref(h), err, 256) == -4
-    assert b"GRAPPA" in err.value
+    k = np.zeros(4 * 16 * 16, dtype=np.complex128)
+    out = np.zeros(16 * 16, dtype=np.complex128)
+    P = ct.POINTER(abi.C64)
+    assert L.mrs_reconstruct_coils(k.ctypes.data_as(P), 4, 16, 16, ct.byref(acq), None, out.ctypes.data_as(P),
+                                   err, 256) == -1
+    assert b"an accelerated slice needs its GRAPPA kernel" in err.value
+    assert L.mrs_grappa_fit(None, 4, 16, 16, 2, 8, None, None, None, err, 256) == -1
     acq = abi.Acquisition.default(n_coils=4)
     assert L.mrs_recon_create(16, 16, 3, ct.byref(acq), None, ct.byref(h), err, 256) == -1
     assert b"one k-space per coil" in err.value
