@@ -284,6 +284,14 @@ typedef struct mrs_zset {
 /* Run sets on `stream` and wait for them (the call allocates and frees its descriptor copy). */
 int mrs_zstage(const mrs_zset* sets, size_t nsets, int32_t adjoint, void* stream, char* err, size_t errlen);
 
+/* Spikes and receiver noise of one volume's acquired 3D k-space (device, nz x n_coils x ny x nx complex64,
+ * in place): partitions_to_slices (src/kspace3d.rs:235-264) before its inverse z-DFT.  `volume` is the
+ * volume index g, `seed` the echo's seed (series seed ^ echo_salt(e)); acq gives n_spikes,
+ * spike_amplitude, noise_variance and the sampling mask.  The inverse z-DFT is mrs_zstage's adjoint
+ * with weights 1/nz.  Waits for `stream`. */
+int mrs_partition_noise(mrs_c32* kspace, size_t nz, size_t n_coils, size_t nx, size_t ny, const mrs_acquisition* acq,
+                        uint64_t volume, uint64_t seed, void* stream, char* err, size_t errlen);
+
 /* Attach forward sets to a plan: they run after the 2D forward inside mrs_plan_forward and its graph.
  * nsets = 0 detaches.  Sets that accumulate into the same output must not be attached together. */
 int mrs_plan_set_zstage(mrs_plan* plan, const mrs_zset* sets, size_t nsets, char* err, size_t errlen);

@@ -13,6 +13,8 @@
 
 namespace mrs {
 hipError_t launch_zstage(int mt, bool adjoint, const ZSet* sets, const ZBlock* blocks, int nblocks, hipStream_t st);
+hipError_t launch_partition_noise(float2* k, int nz, int C, int ny, int nx, const int32_t* rows, int nrows, uint64_t g,
+                                  uint64_t seed, int n_spikes, double amp, double sigma, hipStream_t st);
 int zstage_columns_per_block();
 }  // namespace mrs
 
@@ -243,5 +245,32 @@ extern "C" int mrs_zstage(const mrs_zset* in, size_t nsets, int32_t adjoint, voi
         e = hipStreamSynchronize(st);
     hipFree(mem);
     if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "z-stage: %s", hipGetErrorString(e));
+    return MRS_OK;
+}
+
+// partitions_to_slices' spikes and noise (src/kspace3d.rs:235-264) on one volume's acquired 3D k-space
+// [p][coil][ky][kx] (device), for volume index `volume` on `seed` (the echo's salted seed).
+extern "C" int mrs_partition_noise(mrs_c32* kspace, size_t nz, size_t n_coils, size_t nx, size_t ny,
+                                   const mrs_acquisition* acq, uint64_t volume, uint64_t seed, void* stream, char* err,
+                                   size_t errlen) {
+    if (!kspace || !acq) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    const double sigma = acq->noise_variance > 0.0 ? std::sqrt(acq->noise_variance / (double)(nx * ny)) : 0.0;
+    if (acq->n_spikes == 0 && sigma == 0.0) return MRS_OK;
+    std::vector<uint8_t> mask(ny);
+    mrs_sampling_mask(1, ny, acq, mask.data());
+    std::vector<int32_t> rows;
+    for (size_t ky = 0; ky < ny; ky++)
+        if (mask[ky]) rows.push_back((int32_t)ky);
+    int32_t* drows = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMalloc(&drows, std::max<size_t>(rows.size(), 1) * 4);
+    if (e == hipSuccess && !rows.empty())
+        e = hipMemcpyAsync(drows, rows.data(), rows.size() * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+        e = launch_partition_noise(reinterpret_cast<float2*>(kspace), (int)nz, (int)n_coils, (int)ny, (int)nx, drows,
+                                   (int)rows.size(), volume, seed, (int)acq->n_spikes, acq->spike_amplitude, sigma, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(drows);
+    if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "partition noise: %s", hipGetErrorString(e));
     return MRS_OK;
 }

@@ -1493,6 +1493,72 @@ hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGr
     return hipGetLastError();
 }
 
+// Spikes and noise of a 3D train's acquired k-space (partitions_to_slices, src/kspace3d.rs:235-264),
+// one block per (partition, coil), layout [p][coil][ky][kx]: the stream seed of partition p of
+// volume g is pseed = ((g * 0x100_0001 + p) * 0x9E37) ^ seed ^ SEED_SALT_3D; spikes overwrite
+// acquired samples picked by Rng(pseed ^ 0xA5A5_1234_5678_9ABC ^ q * 0xC2B2_AE3D_27D4_EB4F) with
+// the partition's first peak times the amplitude; then noise on every acquired sample in index
+// order from Rng((pseed ^ q * 0x9E37_79B9) | 1), sample j using draws 4j+1..4j+4.
+__global__ __launch_bounds__(256) void partition_noise(float2* __restrict__ kall, int C, int ny, int nx,
+                                                       const int32_t* __restrict__ rows, int nrows, uint64_t g,
+                                                       uint64_t seed, int n_spikes, double amp, double sigma) {
+    const int p = blockIdx.y, q = blockIdx.x;
+    float2* K = kall + ((size_t)p * C + q) * ny * nx;
+    const uint64_t pseed = ((g * 0x1000001ULL + (uint64_t)p) * 0x9E37ULL) ^ seed ^ 0x334452454144ULL;
+    const int nacq = nrows * nx;
+    if (n_spikes > 0) {
+        __shared__ float best_m[256];
+        __shared__ int best_i[256];
+        float bm = -1.0f;
+        int bi = 0x7fffffff;
+        for (int t = threadIdx.x; t < ny * nx; t += blockDim.x) {
+            const float2 v = K[t];
+            const float m = sqrtf(v.x * v.x + v.y * v.y);
+            if (m > bm) { bm = m; bi = t; }
+        }
+        best_m[threadIdx.x] = bm;
+        best_i[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = -1.0f;
+            int idx = 0;
+            for (int t = 0; t < (int)blockDim.x; t++)
+                if (best_m[t] > m || (best_m[t] == m && best_i[t] < idx)) { m = best_m[t]; idx = best_i[t]; }
+            const float2 peak = m > 0.0f ? K[idx] : make_float2(0.f, 0.f);
+            const float2 spike = make_float2((float)((double)peak.x * amp), (float)((double)peak.y * amp));
+            if (nacq > 0) {
+                const uint64_t s = pseed ^ 0xA5A5123456789ABCULL ^ ((uint64_t)q * 0xC2B2AE3D27D4EB4FULL);
+                for (int k = 0; k < n_spikes; k++) {
+                    const int j = (int)(splitmix_at(s, (uint64_t)k + 1) % (uint64_t)nacq);
+                    K[(size_t)rows[j / nx] * nx + (j % nx)] = spike;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (sigma > 0.0) {
+        const uint64_t s0 = (pseed ^ ((uint64_t)q * 0x9E3779B9ULL)) | 1ULL;
+        for (int j = threadIdx.x; j < nacq; j += blockDim.x) {
+            const uint64_t b = 4ull * (uint64_t)j;
+            const double u1 = unit_of(splitmix_at(s0, b + 1)), u2 = unit_of(splitmix_at(s0, b + 2));
+            const double u3 = unit_of(splitmix_at(s0, b + 3)), u4 = unit_of(splitmix_at(s0, b + 4));
+            const double gre = sqrt(-2.0 * log(u1)) * cos(kTau * u2);
+            const double gim = sqrt(-2.0 * log(u3)) * cos(kTau * u4);
+            float2& k = K[(size_t)rows[j / nx] * nx + (j % nx)];
+            k.x = (float)((double)k.x + gre * sigma);
+            k.y = (float)((double)k.y + gim * sigma);
+        }
+    }
+}
+
+hipError_t launch_partition_noise(float2* k, int nz, int C, int ny, int nx, const int32_t* rows, int nrows, uint64_t g,
+                                  uint64_t seed, int n_spikes, double amp, double sigma, hipStream_t st) {
+    if (nz <= 0 || C <= 0) return hipSuccess;
+    hipLaunchKernelGGL(partition_noise, dim3(C, nz), dim3(256), 0, st, k, C, ny, nx, rows, nrows, g, seed, n_spikes,
+                       amp, sigma);
+    return hipGetLastError();
+}
+
 hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u, const DevGroup* g,
                                const int32_t* rows, hipStream_t st) {
     if (nunits <= 0) return hipSuccess;

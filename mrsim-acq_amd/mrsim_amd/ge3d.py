@@ -11,8 +11,11 @@ Where every relaxation input is uniform the reference applies the decay as a lin
 z-DFT instead; the two agree exactly in real arithmetic (exp(-(TE+t)(1/T2 + 1/T2'))).
 
 Output layout per (volume, echo): [p][coil][ky][kx] complex64 (the reference's per-coil
-[p][ky][kx] with the coil axis moved outside ky).  Noise, the inverse z-DFT and reconstruction
-(partitions_to_slices, src/kspace3d.rs:235-279) are not part of this forward.
+[p][ky][kx] with the coil axis moved outside ky).  `acquire` continues as the reference does after
+the forward: per (volume, echo) the spikes and noise of partitions_to_slices on the echo's stream
+(src/kspace3d.rs:235-264, seed ^ echo_salt(e)), the inverse z-DFT over 1/nz (:265-278), GRAPPA per
+slice under acceleration (kernels fitted by `grappa_calibration`, src/kspace3d.rs:303-323), and the
+reconstruction of every slice (reconstruct_slices, :283-296).
 """
 from __future__ import annotations
 
@@ -20,7 +23,15 @@ from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 from . import abi
-from .gpu import Group, MrsError, Plan, Slice, Unit, ZSet, grase_block
+from .gpu import (Group, MrsError, Plan, Recon, Slice, Unit, ZSet, grappa_apply, grappa_fit, grase_block,
+                  partition_noise, sampling_mask, zstage)
+
+SEED_SALT_GRAPPA_CAL = 0x47525043414C
+
+
+def echo_salt(e: int) -> int:
+    """The receiver-noise salt of echo e, echo 0 unsalted (src/kspace.rs:2130-2132)."""
+    return (e * 0xD1B54A32D192ED03) & 0xFFFFFFFFFFFFFFFF
 
 
 @dataclass
@@ -105,6 +116,7 @@ class Ge3dForward:
         self.volumes = list(volumes)
         self.tes = [float(t) for t in echo_times_ms]
         self.ky_segments = ky_segments
+        self.acq, self.t_line_ms, self.precision = acq, t_line_ms, precision
         groups: List[Group] = []
         slices: List[Slice] = []
         units: List[Unit] = []
@@ -164,6 +176,67 @@ class Ge3dForward:
     def forward(self, stream=None):
         self.plan.forward(stream)
         return self.out
+
+    def _slices(self, vi, e, g, seed, acq):
+        """Spikes and noise on a copy of the (volume, echo) 3D k-space, then the inverse z-DFT / nz:
+        (nz, C, ny, nx) complex64 2D coil k-spaces per slice."""
+        import torch
+        v = self.volumes[vi]
+        k3 = self.out[vi][e].clone()
+        partition_noise(k3, acq, g, seed)
+        nz, C = v.nz, v.coil_maps.shape[1]
+        slices = torch.empty_like(k3)
+        w = torch.full((nz,), 1.0 / nz, dtype=torch.float32, device=k3.device)
+        zstage([ZSet(k3, slices, nz, C, v.ny, v.nx, weights=w)], adjoint=True)
+        return slices
+
+    def grappa_calibration(self, seed: int):
+        """GRAPPA kernels per (volume, echo, slice), fitted on a reference scan of this forward's
+        volumes (grappa_kernels_3d, src/kspace3d.rs:303-323): the acquisition without acceleration and
+        spikes, volume index 0, noise on seed ^ echo_salt(e) ^ SEED_SALT_GRAPPA_CAL.  Returns
+        weights[v][e][z] device tensors; None without acceleration."""
+        if self.acq.accel <= 1:
+            return None
+        acq_cal = self.acq.copy(accel=1, n_spikes=0)
+        cal = Ge3dForward(self.volumes, acq_cal, self.tes, self.ky_segments, self.t_line_ms, self.precision)
+        cal.forward()
+        out = []
+        for vi, v in enumerate(self.volumes):
+            lines = sampling_mask(1, v.ny, acq_cal).reshape(-1)
+            row = []
+            for e in range(len(self.tes)):
+                sl = cal._slices(vi, e, 0, seed ^ echo_salt(e) ^ SEED_SALT_GRAPPA_CAL, acq_cal)
+                row.append([grappa_fit(sl[z].contiguous(), self.acq.accel, self.acq.acs_lines, lines)
+                            for z in range(v.nz)])
+            out.append(row)
+        cal.close()
+        return out
+
+    def acquire(self, seed: int, volume_index: Optional[Sequence[int]] = None, grappa=None):
+        """The acquired, reconstructed complex volumes after `forward`: images[v][e] of (nz, ny, nx)
+        complex64 (vox = x + nx*(y + ny*z)).  `volume_index[v]` is volume v's index g in its series
+        (default v); `grappa` the kernels of `grappa_calibration` (required under acceleration)."""
+        import torch
+        if self.acq.accel > 1 and grappa is None:
+            _refuse("an accelerated slice needs its GRAPPA kernel (fitted on a reference scan)")
+        images = []
+        for vi, v in enumerate(self.volumes):
+            g = vi if volume_index is None else int(volume_index[vi])
+            C = v.coil_maps.shape[1]
+            rec = Recon(v.nx, v.ny, C, self.acq)
+            acquired = sampling_mask(1, v.ny, self.acq).reshape(-1)
+            row = []
+            for e in range(len(self.tes)):
+                sl = self._slices(vi, e, g, seed ^ echo_salt(e), self.acq)
+                if grappa is not None:
+                    for z in range(v.nz):
+                        grappa_apply([sl[z]], self.acq.accel, acquired, grappa[vi][e][z])
+                img = torch.empty((v.nz, v.ny, v.nx), dtype=torch.complex64, device=sl.device)
+                rec.run([sl[z] for z in range(v.nz)], [img[z] for z in range(v.nz)])
+                row.append(img)
+            rec.close()
+            images.append(row)
+        return images
 
     def forward_graph(self, stream=None):
         self.plan.forward_graph(stream)

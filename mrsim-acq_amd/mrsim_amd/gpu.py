@@ -90,6 +90,8 @@ def lib():
     L.mrs_grappa_apply.argtypes = [P(ct.c_void_p), ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
                                    P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
     L.mrs_simulate_slice.argtypes = [P(abi.SliceInput), P(abi.Acquisition), P(ct.c_float)] + E
+    L.mrs_partition_noise.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                                      P(abi.Acquisition), ct.c_uint64, ct.c_uint64, ct.c_void_p] + E
     L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
     _lib = L
     return L
@@ -270,6 +272,18 @@ def grappa_apply(kspaces, accel: int, acquired: np.ndarray, weights, stream=None
     err = _errbuf()
     _check(lib().mrs_grappa_apply(ptrs, n, C, nx, ny, accel, fl.ctypes.data_as(ct.POINTER(ct.c_uint8)),
                                   weights.data_ptr(), ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+
+def partition_noise(k3, acq: abi.Acquisition, volume: int, seed: int, stream=None):
+    """mrs_partition_noise: spikes and noise of one volume's acquired 3D k-space, (nz, C, ny, nx)
+    complex64 device tensor, in place."""
+    import torch
+    nz, C, ny, nx = k3.shape
+    _check_dev("k3", k3, torch.complex64, nz * C * ny * nx)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_partition_noise(k3.data_ptr(), nz, C, nx, ny, ct.byref(acq), volume & 0xFFFFFFFFFFFFFFFF,
+                                     seed & 0xFFFFFFFFFFFFFFFF, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
 
 
 def simulate_slice(b: abi.SliceInputBuilder, acq: abi.Acquisition) -> np.ndarray:

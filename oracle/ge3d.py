@@ -241,3 +241,60 @@ def varying_fmap(snx, sny, nz):
     fx, fy = x / snx - 0.5, y / sny - 0.5
     base = 25.0 * fx - 10.0 * fy + 15.0 * np.exp(-(fx * fx + fy * fy) / 0.02)
     return np.stack([(base + 2.0 * z).reshape(-1) for z in range(nz)]).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------- after the forward
+
+SEED_SALT_3D = 0x334452454144            # src/kspace3d.rs:41
+SEED_SALT_GRAPPA_CAL = 0x47525043414C    # src/kspace.rs (the GRAPPA calibration stream salt)
+_M64 = (1 << 64) - 1
+
+
+def echo_salt(e):
+    """src/kspace.rs:2130-2132."""
+    return (e * 0xD1B54A32D192ED03) & _M64
+
+
+def _rng(seed):
+    import ctypes as ct
+    return ct.c_uint64(seed & _M64)
+
+
+def partitions_to_slices(k3, mask_rows, g, seed, acq):
+    """src/kspace3d.rs:235-279 for every coil of one volume: k3 (nz, C, ny, nx) complex128 is changed
+    in place by the spikes and the noise; returns the slices (nz, C, ny, nx), the inverse z-DFT / nz."""
+    import ctypes as ct
+    L = O.lib()
+    nz, C, ny, nx = k3.shape
+    sigma = np.sqrt(acq.noise_variance / (nx * ny))
+    mask = np.repeat(np.asarray(mask_rows, dtype=bool), nx)
+    acquired = np.nonzero(mask)[0]
+    for p in range(nz):
+        pseed = (((g * 0x1000001 + p) & _M64) * 0x9E37 & _M64) ^ seed ^ SEED_SALT_3D
+        for q in range(C):
+            part = k3[p, q].reshape(-1)
+            if acq.n_spikes > 0:
+                mags = np.abs(part)
+                peak = part[int(np.argmax(mags))] if mags.max() > 0 else 0.0
+                st = _rng(pseed ^ 0xA5A5123456789ABC ^ ((q * 0xC2B2AE3D27D4EB4F) & _M64))
+                for _ in range(int(acq.n_spikes)):
+                    v = L.orc_rng_next(ct.byref(st))
+                    part[acquired[v % len(acquired)]] = peak * acq.spike_amplitude
+            if acq.noise_variance > 0.0:
+                st = _rng((pseed ^ ((q * 0x9E3779B9) & _M64)) | 1)
+                for i in acquired:
+                    re = L.orc_rng_gauss(ct.byref(st)) * sigma
+                    im = L.orc_rng_gauss(ct.byref(st)) * sigma
+                    part[i] += re + 1j * im
+    zk = zkernel(nz)
+    return np.einsum("pz,pcyx->zcyx", zk.conj(), k3) / nz
+
+
+def reconstruct_slices(slices, acq, kernels=None):
+    """src/kspace3d.rs:283-296: every slice through the 2D coil reconstruction -> (nz, ny*nx)."""
+    nz, C, ny, nx = slices.shape
+    out = np.zeros((nz, ny * nx), dtype=np.complex128)
+    for z in range(nz):
+        w = None if kernels is None else kernels[z]
+        out[z] = O.reconstruct(slices[z].reshape(C, -1).copy(), nx, ny, acq, w)
+    return out

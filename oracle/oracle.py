@@ -59,6 +59,8 @@ def lib():
         L.orc_rng_next.restype = ct.c_uint64
         L.orc_rng_unit.argtypes = [P(ct.c_uint64)]
         L.orc_rng_unit.restype = ct.c_double
+        L.orc_rng_gauss.argtypes = [P(ct.c_uint64)]
+        L.orc_rng_gauss.restype = ct.c_double
         L.orc_window_at.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_double]
         L.orc_window_at.restype = ct.c_double
         _lib = L

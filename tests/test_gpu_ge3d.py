@@ -17,6 +17,7 @@ import pytest
 
 from mrsim_amd import abi, gpu, workloads
 from oracle import ge3d as G
+from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -185,3 +186,89 @@ def test_ge3d_full_size_partition_round_trip(full):
             r = float(torch.linalg.vector_norm(back - k2) / torch.linalg.vector_norm(k2))
             print(f"ge3d volume {vi} echo {e}: partition round trip rel-L2 {r:.2e}", flush=True)
             assert r <= 1e-5
+
+
+def _oracle_pipeline(w, v, e, acq, seed, g, k3=None, kernels=None):
+    """The oracle's acquired k-space -> partitions_to_slices -> reconstruct_slices: (nz, ny*nx)."""
+    if k3 is None:
+        k3 = _oracle_volume(w, v, e)
+    mask = O.sampling_mask(1, v.ny, acq)
+    sl = G.partitions_to_slices(k3.copy(), mask, g, seed ^ G.echo_salt(e), acq)
+    return G.reconstruct_slices(sl, acq, kernels)
+
+
+def test_ge3d_acquire_matches_oracle_pipeline(torch_dev):
+    # spikes, receiver noise on the (volume, partition, echo) streams, inverse z-DFT, reconstruction:
+    # the GPU against the oracle's restatement on the same seeds.  Two split-BF16 stages (forward and
+    # reconstruction) compose, so the bound is 2e-5.
+    w = workloads.make_ge3d("ge3d_small")
+    acq = w.acq.copy(noise_variance=0.3, n_spikes=2, spike_amplitude=1.7)
+    w.acq = acq
+    fw, keep = workloads.to_ge3d(w)
+    fw.forward()
+    imgs = fw.acquire(seed=1234, volume_index=[3, 8])
+    for vi, v in enumerate(w.volumes):
+        for e in range(len(w.echo_times_ms)):
+            want = _oracle_pipeline(w, v, e, acq, 1234, [3, 8][vi])
+            got = imgs[vi][e].cpu().numpy().reshape(v.nz, -1)
+            r = _rel(got, want)
+            print(f"acquire volume {vi} echo {e}: rel-L2 vs oracle {r:.2e}", flush=True)
+            assert r <= 2e-5
+    fw.close()
+
+
+def test_ge3d_acquire_with_grappa(torch_dev):
+    # Accelerated acquisition: kernels per (echo, slice) fitted on a reference scan (no acceleration,
+    # no spikes, volume index 0, seed ^ echo_salt(e) ^ the calibration salt).
+    # 1. the oracle's kernels fed to the GPU pipeline vs the oracle's pipeline with the same kernels
+    #    (one linear map on both sides): rel-L2 <= 2e-5;
+    # 2. the GPU's own calibration and pipeline vs the oracle's, bounded by 3x the image error the
+    #    oracle's pipeline shows when its k-spaces carry random relative errors of the GPU's measured
+    #    k-space size (the regularised fit amplifies them).
+    torch = torch_dev
+    w = workloads.make_ge3d("ge3d_small")
+    w.volumes = w.volumes[:1]
+    acq = w.acq.copy(noise_variance=0.2, accel=2, acs_lines=6)
+    w.acq = acq
+    fw, keep = workloads.to_ge3d(w)
+    fw.forward()
+    v = w.volumes[0]
+    ne = len(w.echo_times_ms)
+    acq_cal = acq.copy(accel=1, n_spikes=0)
+    lines = O.sampling_mask(1, v.ny, acq_cal)
+    shape = (acq.accel - 1, w.n_coils, 6 * w.n_coils)
+    k_cal, ker = [], []
+    for e in range(ne):
+        k_cal.append(G.ge3d_kspace(v.snx, v.sny, v.nx, v.ny, v.nz, v.images, v.t2, v.t_inhom, v.fmap, None,
+                                   v.coil_maps, acq_cal, w.echo_times_ms[e], w.ky_segments, w.t_line_ms,
+                                   v.weights.astype(np.float32)))
+        sl = G.partitions_to_slices(k_cal[e].copy(), lines, 0, 99 ^ G.echo_salt(e) ^ G.SEED_SALT_GRAPPA_CAL,
+                                    acq_cal)
+        ker.append([O.grappa_fit(sl[z].reshape(w.n_coils, -1), v.nx, v.ny, acq.accel, acq.acs_lines, lines)
+                    for z in range(v.nz)])
+    given = [[[torch.from_numpy(k.reshape(shape)).cuda() for k in ker[e]] for e in range(ne)]]
+    imgs = fw.acquire(seed=99, grappa=given)
+    wants = []
+    for e in range(ne):
+        wants.append(_oracle_pipeline(w, v, e, acq, 99, 0, kernels=ker[e]))
+        r = _rel(imgs[0][e].cpu().numpy().reshape(v.nz, -1), wants[e])
+        print(f"grappa echo {e}: GPU pipeline with the oracle's kernels rel-L2 {r:.2e}", flush=True)
+        assert r <= 2e-5
+    # the whole accelerated pipeline on the GPU, its calibration included
+    cal = fw.grappa_calibration(seed=99)
+    imgs = fw.acquire(seed=99, grappa=cal)
+    rng = np.random.default_rng(1)
+    for e in range(ne):
+        k3 = _oracle_volume(w, v, e)
+        ek = _rel(fw.out[0][e].cpu().numpy(), k3)
+        pert = lambda k: k + ek * np.sqrt(np.mean(np.abs(k[k != 0]) ** 2)) * (k != 0) * (
+            rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)) / np.sqrt(2)
+        sl = G.partitions_to_slices(pert(k_cal[e]), lines, 0, 99 ^ G.echo_salt(e) ^ G.SEED_SALT_GRAPPA_CAL, acq_cal)
+        kp = [O.grappa_fit(sl[z].reshape(w.n_coils, -1), v.nx, v.ny, acq.accel, acq.acs_lines, lines)
+              for z in range(v.nz)]
+        e_pert = _rel(_oracle_pipeline(w, v, e, acq, 99, 0, k3=pert(k3), kernels=kp), wants[e])
+        r = _rel(imgs[0][e].cpu().numpy().reshape(v.nz, -1), wants[e])
+        print(f"grappa echo {e}: whole GPU pipeline rel-L2 {r:.2e}; oracle with k-space errors of "
+              f"{ek:.1e}: {e_pert:.2e}", flush=True)
+        assert r <= max(2e-5, 3.0 * e_pert)
+    fw.close()

@@ -252,3 +252,60 @@ def test_compartments_add_with_their_weights_and_unused_ones_change_nothing():
     w3 = np.concatenate([w, np.zeros((nz, kseg, 1))], axis=2)
     three = _ge([a, b, None], [80.0, 120.0, 60.0], None, w3, acq, 12.0, nz=nz, kseg=kseg)
     assert np.array_equal(three, both)
+
+
+# ------------------------------------------------------------------ after the forward
+
+
+def _k3(nz=NZ, nc=2, te=12.0):
+    img = G.blobs(SNX, SNY, nz)
+    acq = _acq(nc)
+    return acq, _ge([img], [80.0], [45.0], np.ones((nz, 1, 1)), acq, te, nz=nz)
+
+
+def test_partitions_to_slices_without_noise_is_the_inverse_z_dft():
+    acq, k3 = _k3()
+    mask = O.sampling_mask(1, NY, acq)
+    sl = G.partitions_to_slices(k3.copy(), mask, 0, 123, acq)
+    back = np.einsum("pz,zcyx->pcyx", G.zkernel(NZ), sl)
+    assert np.abs(back - k3).max() <= 1e-12 * np.abs(k3).max()
+
+
+def test_partition_noise_statistics_and_independence():
+    # noise only on acquired rows, zero mean, variance sigma^2 per component; the two echoes' streams
+    # (seed ^ echo_salt(e)) are uncorrelated, echo 0's is the series seed itself (src/kspace3d.rs:2662-2705)
+    nz, nc = 4, 2
+    acq0, k3 = _k3(nz, nc)
+    acq = acq0.copy(noise_variance=0.5, accel=2, acs_lines=6)
+    mask = O.sampling_mask(1, NY, acq)
+    sigma = np.sqrt(0.5 / (NX * NY))
+    r = []
+    for e in (0, 1):
+        k = k3.copy()
+        G.partitions_to_slices(k, mask, 3, 77 ^ G.echo_salt(e), acq)
+        r.append((k - k3))
+    assert G.echo_salt(0) == 0
+    for d in r:
+        assert np.all(d[:, :, ~mask, :] == 0)
+        v = d[:, :, mask, :].reshape(-1)
+        assert abs(np.mean(v.real)) < 4 * sigma / np.sqrt(v.size)
+        tol = 5 * np.sqrt(2.0 / v.size)   # five standard errors of a variance estimate
+        assert abs(np.var(v.real) / sigma ** 2 - 1) < tol and abs(np.var(v.imag) / sigma ** 2 - 1) < tol
+    a, b = r[0][:, :, mask, :].reshape(-1), r[1][:, :, mask, :].reshape(-1)
+    corr = np.real(np.vdot(a, b)) / np.sqrt(np.vdot(a, a).real * np.vdot(b, b).real)
+    assert abs(corr) < 3.0 / np.sqrt(2 * a.size)
+
+
+def test_partition_spikes_replace_samples_with_the_scaled_peak():
+    nz, nc = 4, 2
+    acq0, k3 = _k3(nz, nc)
+    acq = acq0.copy(n_spikes=3, spike_amplitude=2.5)
+    mask = O.sampling_mask(1, NY, acq)
+    k = k3.copy()
+    G.partitions_to_slices(k, mask, 1, 5, acq)
+    for p in range(nz):
+        for q in range(nc):
+            changed = np.nonzero(k[p, q] != k3[p, q])
+            peak = k3[p, q].reshape(-1)[np.argmax(np.abs(k3[p, q]))]
+            assert 1 <= len(changed[0]) <= 3
+            assert np.all(k[p, q][changed] == peak * 2.5)
