@@ -14,6 +14,9 @@
  * Parity pins (tests/test_oracle.py):
  *   - the 39 per-slice golden fixtures tests/golden/p4_baseline/default/slice-*.bin
  *     (tests/golden_baseline.rs:28-89 of the reference), compared bit for bit;
+ *   - the two series records series-{uniform,mapped}.bin (tests/golden_baseline.rs:91-124), through
+ *     acquire_volumes with per-slice GRAPPA kernels and the hbcd_like object phase, bit for bit
+ *     (tests/test_oracle_series.py);
  *   - EXPECTED_BITS_STD (src/kspace.rs:3484), the first eight raw coefficients, bit for bit;
  *   - the literal per-line sum (src/kspace.rs:2225-2406) within 1e-10 x peak;
  *   - the line-timing table against SingleShotEpi tick by tick (src/kspace.rs:3337-3353);
@@ -30,6 +33,15 @@
 #define TAU 6.283185307179586476925286766559
 
 typedef mrs_c64 C;
+
+/* PhaseModel (src/phase.rs:129-137): global + BackgroundPhase coefficients (1, x, y, z, x^2, y^2,
+ * z^2, xy, xz, yz) + an optional PrepPhase. */
+typedef struct {
+    double global;
+    double coeffs[10];
+    int32_t has_prep;
+    double c_q, sigma_dx, sigma_rot;
+} orc_phase_model;
 
 static C c_add(C a, C b) { C r = {a.re + b.re, a.im + b.im}; return r; }
 static C c_mul(C a, C b) { C r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; return r; }
@@ -810,45 +822,245 @@ void orc_reconstruct(C* coil_kspace, size_t nc, size_t nx, size_t ny, const mrs_
 
 #define SEED_SALT_GRAPPA_CAL 0x47525043414CULL
 
-/* simulate_slice (:1229-1235) via simulate_slice_with (:1271-1284), kernel = None.
- * out: nx*ny (re, im) f32 pairs. */
-int orc_simulate_slice(const mrs_slice_input* inp, const mrs_acquisition* acq, float* out) {
+/* grappa_reference_kernel (:1256-1262): the slice without eddy or prep drive, read without
+ * acceleration or spikes, fitted on its acquired lines.  The caller sets inp->slice_seed.
+ * Returns (accel-1) * nc * 6nc weights (malloc), or NULL on a refused input. */
+static C* reference_kernel(const mrs_slice_input* inp, const mrs_acquisition* acq) {
     size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
     size_t nc = acq->n_coils > 1 ? acq->n_coils : 1;
-    size_t nfeat = 6 * nc;
-    C* weights = NULL;
+    mrs_slice_input ref = *inp;
+    ref.has_eddy_drive = 0;
+    ref.has_prep_drive = 0;
+    mrs_acquisition acq_ref = *acq;
+    acq_ref.accel = 1;
+    acq_ref.n_spikes = 0;
+    C* coils = malloc(sizeof(C) * nc * nx * ny);
     int rc = 0;
-    if (acq->accel > 1) {
-        /* grappa_reference_kernel (:1256-1262) on the salted slice seed */
-        mrs_slice_input ref = *inp;
-        ref.slice_seed = inp->slice_seed ^ SEED_SALT_GRAPPA_CAL;
-        ref.has_eddy_drive = 0;
-        ref.has_prep_drive = 0;
-        mrs_acquisition acq_ref = *acq;
-        acq_ref.accel = 1;
-        acq_ref.n_spikes = 0;
-        C* coils = malloc(sizeof(C) * nc * nx * ny);
-        for (size_t c = 0; c < nc && rc == 0; c++) rc = orc_build_coil_kspace(&ref, &acq_ref, c, nc, coils + c * nx * ny);
+    for (size_t c = 0; c < nc && rc == 0; c++) rc = orc_build_coil_kspace(&ref, &acq_ref, c, nc, coils + c * nx * ny);
+    C* weights = NULL;
+    if (rc == 0) {
         uint8_t* lines = malloc(ny);
         orc_sampling_mask(1, ny, &acq_ref, lines);
         C** cp = malloc(sizeof(C*) * nc);
         for (size_t c = 0; c < nc; c++) cp[c] = coils + c * nx * ny;
-        weights = malloc(sizeof(C) * (acq->accel - 1) * nc * nfeat);
+        weights = malloc(sizeof(C) * (acq->accel - 1) * nc * 6 * nc);
         orc_grappa_fit(cp, nc, nx, ny, acq->accel, acq->acs_lines, lines, weights);
-        free(cp); free(lines); free(coils);
+        free(cp); free(lines);
     }
+    free(coils);
+    return weights;
+}
+
+/* simulate_slice_with (:1271-1284) with the kernel given (NULL without acceleration): every coil's
+ * k-space, then reconstruct_coils_full.  img: nx*ny complex f64 (SliceRecon::combined). */
+static int simulate_slice_with(const mrs_slice_input* inp, const mrs_acquisition* acq, const C* weights, C* img) {
+    size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    size_t nc = acq->n_coils > 1 ? acq->n_coils : 1;
     C* coils = malloc(sizeof(C) * nc * nx * ny);
+    int rc = 0;
     for (size_t c = 0; c < nc && rc == 0; c++) rc = orc_build_coil_kspace(inp, acq, c, nc, coils + c * nx * ny);
+    if (rc == 0) orc_reconstruct(coils, nc, nx, ny, acq, weights, img);
+    free(coils);
+    return rc;
+}
+
+/* simulate_slice (:1229-1235) via simulate_slice_with (:1271-1284), kernel = None: under
+ * acceleration the kernel is fitted on a reference scan of the slice itself, on the slice seed
+ * salted by SEED_SALT_GRAPPA_CAL.  out: nx*ny (re, im) f32 pairs. */
+int orc_simulate_slice(const mrs_slice_input* inp, const mrs_acquisition* acq, float* out) {
+    size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    C* weights = NULL;
+    if (acq->accel > 1) {
+        mrs_slice_input ref = *inp;
+        ref.slice_seed = inp->slice_seed ^ SEED_SALT_GRAPPA_CAL;
+        if (!(weights = reference_kernel(&ref, acq))) return -1;
+    }
     C* img = malloc(sizeof(C) * nx * ny);
-    if (rc == 0) {
-        orc_reconstruct(coils, nc, nx, ny, acq, weights, img);
+    int rc = simulate_slice_with(inp, acq, weights, img);
+    if (rc == 0)
         for (size_t i = 0; i < nx * ny; i++) {
             out[2 * i] = (float)img[i].re;
             out[2 * i + 1] = (float)img[i].im;
         }
-    }
-    free(coils); free(img); free(weights);
+    free(img); free(weights);
     return rc;
+}
+
+/* ---------------------------------------------------------------- object phase (src/phase.rs) */
+/* PrepPhase::shot (phase.rs:108-125): one shot's q_eff, translation and rotation, drawn from
+ * Rng(seed ^ volume*0x9E37_79B9_7F4A_7C15 ^ slice_group*0xBF58_476D_1CE4_E5B9).
+ * shot: q_eff[3], dx[3], rot[3]. */
+void orc_prep_shot(const orc_phase_model* m, double magnitude, const double direction[3], uint64_t volume,
+                   uint64_t slice_group, uint64_t seed, double shot[9]) {
+    for (int i = 0; i < 9; i++) shot[i] = 0.0;
+    double n = sqrt(direction[0] * direction[0] + direction[1] * direction[1] + direction[2] * direction[2]);
+    if (magnitude <= 0.0 || n < 1e-12) return;
+    double s = m->c_q * sqrt(magnitude) / n;
+    for (int i = 0; i < 3; i++) shot[i] = direction[i] * s;
+    uint64_t rng = seed ^ (volume * 0x9E3779B97F4A7C15ULL) ^ (slice_group * 0xBF58476D1CE4E5B9ULL);
+    for (int i = 0; i < 3; i++) shot[3 + i] = orc_rng_gauss(&rng) * m->sigma_dx;
+    for (int i = 0; i < 3; i++) shot[6 + i] = orc_rng_gauss(&rng) * m->sigma_rot;
+}
+
+/* phase_slice (kspace.rs:1156-1180) with PhaseModel::at (phase.rs:146-148), BackgroundPhase::at
+ * (:52-57) and ShotPhase::at (:76-84), in their operation order.  out: snx*sny rad. */
+void orc_phase_slice(const orc_phase_model* m, const double shot[9], size_t snx, size_t sny, size_t o, size_t z,
+                     size_t nz, double* out) {
+    double sxs = (double)snx / 2.0, sys = (double)sny / 2.0;
+    double off = ((double)o - 1.0) / 2.0;
+    double zc = (double)z - (double)nz / 2.0;
+    const double* c = m->coeffs;
+    const double* q = shot;
+    const double* dx = shot + 3;
+    const double* rot = shot + 6;
+    for (size_t y = 0; y < sny; y++)
+        for (size_t x = 0; x < snx; x++) {
+            double r0 = ((double)x - sxs - off) / (double)o;
+            double r1 = ((double)y - sys - off) / (double)o;
+            double r2 = zc;
+            double bg = c[0] + c[1] * r0 + c[2] * r1 + c[3] * r2 + c[4] * r0 * r0 + c[5] * r1 * r1 + c[6] * r2 * r2 +
+                        c[7] * r0 * r1 + c[8] * r0 * r2 + c[9] * r1 * r2;
+            double cr0 = rot[1] * r2 - rot[2] * r1;
+            double cr1 = rot[2] * r0 - rot[0] * r2;
+            double cr2 = rot[0] * r1 - rot[1] * r0;
+            double sh = -0.0;
+            sh += q[0] * (dx[0] + cr0);
+            sh += q[1] * (dx[1] + cr1);
+            sh += q[2] * (dx[2] + cr2);
+            out[x + snx * y] = m->global + bg + sh;
+        }
+}
+
+/* ---------------------------------------------------------------- series (acquire_volumes) */
+/* acquire_volumes_complex (:1851-2125) with default options (every slice local, no capture, one
+ * echo time), as simulate_acquisition_oversampled (:1687-1718) calls it.
+ *   images[q]: (x + snx*(y + sny*z))*nv + g; t2[q] / t_inhom[q] (t_inhom may be NULL): a map over
+ *   the whole simulation volume (snx*sny*nz) or a uniform value; fmap snx*sny*nz;
+ *   eddy[g]: has_eddy[g] ? eddy_drive[3g..3g+2] : None; prep[g]: has_prep[g] ? (prep_mag[g],
+ *   prep_dir[3g..]) : None; model: the object phase (required: the reference always passes
+ *   phase_slice's values as phase0; zero coefficients give a real object); noise_sigma: nx*ny*nz
+ *   or NULL.
+ * GRAPPA (:1940-1985): one kernel per slice fitted on volume 0's images with no shot phase and
+ * grappa_cal_slice_seed(z, receiver_seed) (:1655-1657).
+ * Output re/im: nx*ny*nz*nv f32, (x + nx*(y + ny*z))*nv + g.  Returns 0, or -1 on a refused
+ * slice. */
+int orc_acquire_volumes(const size_t sim_dims[3], const size_t acq_dims[3], size_t nv, const float* const* images,
+                        size_t ncomp, const mrs_t2_slice* t2, const mrs_t2_slice* t_inhom, const float* fmap,
+                        const mrs_acquisition* acq, const int32_t* has_eddy, const double* eddy_drive,
+                        const int32_t* has_prep, const double* prep_mag, const double* prep_dir,
+                        const orc_phase_model* model, uint64_t excitation_seed, uint64_t receiver_seed,
+                        const float* noise_sigma, float* re_out, float* im_out) {
+    size_t snx = sim_dims[0], sny = sim_dims[1], nz = sim_dims[2];
+    size_t nx = acq_dims[0], ny = acq_dims[1];
+    if (acq_dims[2] != nz || snx % nx || sny % ny || snx / nx != sny / ny || ncomp > MRS_MAX_COMP) return -1;
+    size_t o = snx / nx, P = snx * sny;
+    size_t nvox_acq = nx * ny * nz;
+    double zero_shot[9] = {0};
+    double* phi = malloc(sizeof(double) * P);
+    float* cs = malloc(sizeof(float) * ncomp * P);
+    const float* refs[MRS_MAX_COMP];
+    mrs_t2_slice t2s[MRS_MAX_COMP], tis[MRS_MAX_COMP];
+    C* img = malloc(sizeof(C) * nx * ny);
+    C** kernels = calloc(nz, sizeof(C*));
+    int rc = 0;
+    /* slice inputs of local slice zl with plane pointers into `planes` (compartment images) */
+#define SLICE_AT(zl, planes, fmap_plane)                                                              \
+    mrs_slice_input inp;                                                                                \
+    memset(&inp, 0, sizeof inp);                                                                        \
+    for (size_t q = 0; q < ncomp; q++) {                                                                \
+        refs[q] = (planes) + q * P;                                                                     \
+        t2s[q] = t2[q];                                                                                 \
+        if (t2[q].map) t2s[q].map = t2[q].map + (zl) * P;                                               \
+        if (t_inhom) { tis[q] = t_inhom[q]; if (t_inhom[q].map) tis[q].map = t_inhom[q].map + (zl) * P; } \
+    }                                                                                                   \
+    inp.compartments = refs; inp.n_compartments = ncomp; inp.t2 = t2s; inp.t_inhom = t_inhom ? tis : NULL; \
+    inp.fmap = (fmap_plane); inp.phase0 = phi; inp.sim[0] = snx; inp.sim[1] = sny;                      \
+    inp.acq_matrix[0] = nx; inp.acq_matrix[1] = ny; inp.z = (zl); inp.nz = nz;
+    if (acq->accel > 1 && nv > 0) {
+        for (size_t zl = 0; zl < nz && rc == 0; zl++) {
+            for (size_t q = 0; q < ncomp; q++)
+                for (size_t i = 0; i < P; i++) cs[q * P + i] = images[q][(zl * P + i) * nv + 0];
+            orc_phase_slice(model, zero_shot, snx, sny, o, zl, nz, phi);
+            SLICE_AT(zl, cs, fmap + zl * P)
+            inp.slice_seed = (zl * 0x9E37ULL) ^ receiver_seed ^ SEED_SALT_GRAPPA_CAL;
+            if (!(kernels[zl] = reference_kernel(&inp, acq))) rc = -1;
+        }
+    }
+    for (size_t g = 0; g < nv && rc == 0; g++) {
+        for (size_t zl = 0; zl < nz && rc == 0; zl++) {
+            for (size_t q = 0; q < ncomp; q++)
+                for (size_t i = 0; i < P; i++) cs[q * P + i] = images[q][(zl * P + i) * nv + g];
+            double shot[9] = {0};
+            if (model->has_prep && has_prep && has_prep[g])
+                orc_prep_shot(model, prep_mag[g], prep_dir + 3 * g, g, zl, excitation_seed, shot);
+            orc_phase_slice(model, shot, snx, sny, o, zl, nz, phi);
+            SLICE_AT(zl, cs, fmap + zl * P)
+            if (has_eddy && has_eddy[g]) {
+                inp.has_eddy_drive = 1;
+                for (int k = 0; k < 3; k++) inp.eddy_drive[k] = eddy_drive[3 * g + k];
+            }
+            if (has_prep && has_prep[g]) {
+                inp.has_prep_drive = 1;
+                inp.prep_magnitude = prep_mag[g];
+                for (int k = 0; k < 3; k++) inp.prep_direction[k] = prep_dir[3 * g + k];
+            }
+            inp.slice_seed = (((uint64_t)g * 0x1000001ULL + zl) * 0x9E37ULL) ^ receiver_seed;
+            rc = simulate_slice_with(&inp, acq, acq->accel > 1 ? kernels[zl] : NULL, img);
+            for (size_t y = 0; y < ny && rc == 0; y++)
+                for (size_t x = 0; x < nx; x++) {
+                    C c = img[x + nx * y];
+                    float re = (float)c.re, im = (float)c.im;
+                    size_t vox = x + nx * (y + ny * zl);
+                    if (noise_sigma) {
+                        double sd = (double)noise_sigma[vox];
+                        if (sd > 0.0) {
+                            uint64_t rng = (receiver_seed ^ ((uint64_t)g * 0x9E3779B97F4A7C15ULL) ^
+                                            ((uint64_t)vox * 0xBF58476D1CE4E5B9ULL)) | 1ULL;
+                            re += (float)(orc_rng_gauss(&rng) * sd);
+                            im += (float)(orc_rng_gauss(&rng) * sd);
+                        }
+                    }
+                    re_out[vox * nv + g] = re;
+                    im_out[vox * nv + g] = im;
+                }
+        }
+    }
+#undef SLICE_AT
+    for (size_t zl = 0; zl < nz; zl++) free(kernels[zl]);
+    free(kernels); free(phi); free(cs); free(img);
+    (void)nvox_acq;
+    return rc;
+}
+
+/* acquire_volumes (:1840-1846): magnitude sqrt(re*re + im*im) and phase atan2(im, re) in f32. */
+void orc_mag_phase(const float* re, const float* im, size_t n, float* mag, float* ph) {
+    for (size_t i = 0; i < n; i++) {
+        float r = re[i], m = im[i];
+        mag[i] = sqrtf(r * r + m * m);
+        ph[i] = atan2f(m, r);
+    }
+}
+
+/* The series inputs of tests/golden_baseline.rs:91-116 (16x16x3, o = 2, 4 volumes, two
+ * compartments from one Rng(0xBA5E) stream).  images: 2 * nsim*nv in the entry point's layout
+ * (vox*nv + g); fmap, t2map: nsim. */
+void orc_series_inputs(float* images, float* fmap, float* t2map) {
+    const size_t nx = 16, ny = 16, nz = 3, o = 2, nv = 4;
+    const size_t snx = nx * o, sny = ny * o, nsim = snx * sny * nz;
+    uint64_t rng = 0xBA5E;
+    for (size_t c = 0; c < 2; c++)
+        for (size_t i = 0; i < nsim * nv; i++) {
+            size_t v = i / nsim, vox = i % nsim;
+            double x = (double)(vox % snx) / (double)snx, y = (double)((vox / snx) % sny) / (double)sny;
+            double dx = x - 0.5, dy = y - 0.5;
+            double blob = exp(-((dx * dx + dy * dy) / 0.05));
+            double val = (1.0 + 0.1 * (double)c + 0.01 * (double)v) * blob + 0.02 * orc_rng_unit(&rng);
+            images[c * nsim * nv + vox * nv + v] = (float)val;
+        }
+    for (size_t i = 0; i < nsim; i++) {
+        fmap[i] = (float)(40.0 * ((double)(i % snx) / (double)snx - 0.5));
+        t2map[i] = (float)(50.0 + (double)(i % 7) * 10.0);
+    }
 }
 
 /* ---------------------------------------------------------------- golden inputs (tests/golden_baseline.rs:32-55) */

@@ -63,8 +63,146 @@ def lib():
         L.orc_rng_gauss.restype = ct.c_double
         L.orc_window_at.argtypes = [ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_double]
         L.orc_window_at.restype = ct.c_double
+        L.orc_prep_shot.argtypes = [P(PhaseModel), ct.c_double, P(ct.c_double), ct.c_uint64, ct.c_uint64,
+                                    ct.c_uint64, P(ct.c_double)]
+        L.orc_phase_slice.argtypes = [P(PhaseModel), P(ct.c_double), ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                                      ct.c_size_t, ct.c_size_t, P(ct.c_double)]
+        L.orc_acquire_volumes.argtypes = [P(ct.c_size_t), P(ct.c_size_t), ct.c_size_t, P(P(ct.c_float)),
+                                          ct.c_size_t, P(abi.T2Slice), P(abi.T2Slice), P(ct.c_float),
+                                          P(Acquisition), P(ct.c_int32), P(ct.c_double), P(ct.c_int32),
+                                          P(ct.c_double), P(ct.c_double), P(PhaseModel), ct.c_uint64,
+                                          ct.c_uint64, P(ct.c_float), P(ct.c_float), P(ct.c_float)]
+        L.orc_mag_phase.argtypes = [P(ct.c_float), P(ct.c_float), ct.c_size_t, P(ct.c_float), P(ct.c_float)]
+        L.orc_series_inputs.argtypes = [P(ct.c_float), P(ct.c_float), P(ct.c_float)]
         _lib = L
     return _lib
+
+
+class PhaseModel(ct.Structure):
+    """PhaseModel (src/phase.rs:129-137) as the oracle takes it."""
+    _fields_ = [("global_", ct.c_double), ("coeffs", ct.c_double * 10), ("has_prep", ct.c_int32),
+                ("c_q", ct.c_double), ("sigma_dx", ct.c_double), ("sigma_rot", ct.c_double)]
+
+    @classmethod
+    def hbcd_like(cls):
+        """PhaseModel::hbcd_like (src/phase.rs:188-198)."""
+        m = cls()
+        for i, v in enumerate((0.0, 0.13, 0.10, 0.02)):
+            m.coeffs[i] = v
+        m.has_prep, m.c_q, m.sigma_dx, m.sigma_rot = 1, 0.0425, 1.0, 2.0e-3
+        return m
+
+
+def prep_shot(model, magnitude, direction, volume, slice_group, seed):
+    d = np.ascontiguousarray(direction, dtype=np.float64)
+    out = np.zeros(9)
+    lib().orc_prep_shot(ct.byref(model), magnitude, abi.dptr(d), volume, slice_group, seed & _M64, abi.dptr(out))
+    return out
+
+
+def phase_slice(model, shot, snx, sny, o, z, nz):
+    sh = np.ascontiguousarray(np.zeros(9) if shot is None else shot, dtype=np.float64)
+    out = np.zeros(snx * sny)
+    lib().orc_phase_slice(ct.byref(model), abi.dptr(sh), snx, sny, o, z, nz, abi.dptr(out))
+    return out
+
+
+_M64 = (1 << 64) - 1
+
+
+def _t2_volumes(vals):
+    keep, arr = [], (abi.T2Slice * max(len(vals), 1))()
+    for q, v in enumerate(vals):
+        if isinstance(v, (int, float)):
+            arr[q] = abi.T2Slice(None, float(v))
+        else:
+            a = np.ascontiguousarray(v, dtype=np.float32)
+            keep.append(a)
+            arr[q] = abi.T2Slice(abi.fptr(a), 0.0)
+    return arr, keep
+
+
+def acquire_volumes(sim_dims, acq_dims, nv, images, t2, fmap, t_inhom, acq, eddy_drive=None, prep_drive=None,
+                    model=None, excitation_seed=0, receiver_seed=0, noise_sigma=None):
+    """acquire_volumes_complex (src/kspace.rs:1851-2125) with default options: (re, im) f32 arrays of
+    nx*ny*nz*nv in the layout (x + nx*(y + ny*z))*nv + g.  images: per compartment, (vox*nv + g);
+    t2/t_inhom: per compartment a float or a simulation-volume map; eddy_drive/prep_drive: per volume
+    None or a value ((magnitude, direction) for prep)."""
+    nx, ny, nz = acq_dims
+    n = nx * ny * nz * nv
+    ims = [np.ascontiguousarray(im, dtype=np.float32) for im in images]
+    ptrs = (ct.POINTER(ct.c_float) * len(ims))(*[abi.fptr(a) for a in ims])
+    t2a, k1 = _t2_volumes(t2)
+    tia, k2 = _t2_volumes(t_inhom) if t_inhom is not None else (None, [])
+    fm = np.ascontiguousarray(fmap, dtype=np.float32)
+    he = np.array([e is not None for e in (eddy_drive or [None] * nv)], dtype=np.int32)
+    ed = np.array([e if e is not None else (0.0, 0.0, 0.0) for e in (eddy_drive or [None] * nv)], dtype=np.float64)
+    hp = np.array([p is not None for p in (prep_drive or [None] * nv)], dtype=np.int32)
+    pm = np.array([p[0] if p is not None else 0.0 for p in (prep_drive or [None] * nv)], dtype=np.float64)
+    pd = np.array([p[1] if p is not None else (0.0, 0.0, 0.0) for p in (prep_drive or [None] * nv)],
+                  dtype=np.float64)
+    model = model if model is not None else PhaseModel()
+    ns = None if noise_sigma is None else np.ascontiguousarray(noise_sigma, dtype=np.float32)
+    re, im = np.zeros(n, dtype=np.float32), np.zeros(n, dtype=np.float32)
+    sd = (ct.c_size_t * 3)(*sim_dims)
+    ad = (ct.c_size_t * 3)(*acq_dims)
+    rc = lib().orc_acquire_volumes(sd, ad, nv, ptrs, len(ims), t2a, tia, abi.fptr(fm), ct.byref(acq),
+                                   he.ctypes.data_as(ct.POINTER(ct.c_int32)), abi.dptr(ed.reshape(-1)),
+                                   hp.ctypes.data_as(ct.POINTER(ct.c_int32)), abi.dptr(pm), abi.dptr(pd.reshape(-1)),
+                                   ct.byref(model), excitation_seed & _M64, receiver_seed & _M64,
+                                   abi.fptr(ns) if ns is not None else ct.POINTER(ct.c_float)(),
+                                   abi.fptr(re), abi.fptr(im))
+    if rc != 0:
+        raise ValueError("oracle acquire_volumes refused the input")
+    return re, im
+
+
+def mag_phase(re, im):
+    """acquire_volumes' f32 magnitude and atan2 phase (src/kspace.rs:1840-1846)."""
+    re, im = np.ascontiguousarray(re, dtype=np.float32), np.ascontiguousarray(im, dtype=np.float32)
+    mag, ph = np.zeros_like(re), np.zeros_like(re)
+    lib().orc_mag_phase(abi.fptr(re), abi.fptr(im), len(re), abi.fptr(mag), abi.fptr(ph))
+    return mag, ph
+
+
+SERIES_DIMS = dict(nx=16, ny=16, nz=3, o=2, nv=4)
+
+
+def series_inputs():
+    """The series inputs of tests/golden_baseline.rs:91-116: (images per compartment, fmap, t2map)."""
+    d = SERIES_DIMS
+    nsim = d["nx"] * d["o"] * d["ny"] * d["o"] * d["nz"]
+    images = np.zeros(2 * nsim * d["nv"], dtype=np.float32)
+    fmap, t2map = np.zeros(nsim, dtype=np.float32), np.zeros(nsim, dtype=np.float32)
+    lib().orc_series_inputs(abi.fptr(images), abi.fptr(fmap), abi.fptr(t2map))
+    return list(images.reshape(2, -1)), fmap, t2map
+
+
+def series_cases():
+    """The two series configurations (tests/golden_baseline.rs:113-124): (name, t2, acq)."""
+    _, _, t2map = series_inputs()
+    acq = Acquisition.default(n_coils=3, accel=2, acs_lines=6, noise_variance=2.0, signal_scale=100.0,
+                              t_echo=30.0, t_line=0.5)
+    return [("uniform", [80.0, 160.0], acq), ("mapped", [t2map, 160.0], acq)]
+
+
+def series_golden(name):
+    """The reference's record of series-<name>: (magnitude, phase) f32, each nx*ny*nz*nv."""
+    raw = np.fromfile(os.path.join(ROOT, "tests", "golden", "p4_baseline", "default", f"series-{name}.bin"),
+                      dtype=np.float32)
+    return raw[: raw.size // 2], raw[raw.size // 2:]
+
+
+def series(name, model=None):
+    """One golden series through the oracle: (magnitude, phase) f32."""
+    d = SERIES_DIMS
+    images, fmap, _ = series_inputs()
+    t2, acq = {c[0]: c for c in series_cases()}[name][1:]
+    sim = (d["nx"] * d["o"], d["ny"] * d["o"], d["nz"])
+    re, im = acquire_volumes(sim, (d["nx"], d["ny"], d["nz"]), d["nv"], images, t2, fmap, None, acq,
+                             model=model if model is not None else PhaseModel.hbcd_like(),
+                             excitation_seed=42, receiver_seed=42)
+    return mag_phase(re, im)
 
 
 def _c64(a):
