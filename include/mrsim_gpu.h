@@ -348,6 +348,17 @@ int mrs_reconstruct_coils(const mrs_c64* coil_kspace, size_t n_coils, size_t nx,
  * reconstruction.  out: nx*ny interleaved (re, im) f32, the reference's Vec<(f32, f32)>. */
 int mrs_simulate_slice(const mrs_slice_input* inp, const mrs_acquisition* acq, float* out, char* err, size_t errlen);
 
+/* The tail of acquire_volumes (src/kspace.rs:2036-2060, 1840-1846) on a series' reconstructed images:
+ * images (device) [g][vox] complex64, vox = x + nx*(y + ny*z); noise_sigma (device, nvox, or NULL):
+ * image-space complex Gaussian noise of that SD per component from
+ * Rng((receiver_seed ^ g*0x9E37_79B9_7F4A_7C15 ^ vox*0xBF58_476D_1CE4_E5B9) | 1); out0/out1 (device,
+ * nvox*n_volumes f32) in the entry point's layout vox*n_volumes + g: mode 0 the real and imaginary
+ * parts (simulate_acquisition_complex), mode 1 magnitude and phase (simulate_acquisition_oversampled).
+ * Waits for `stream`. */
+int mrs_series_epilogue(const mrs_c32* images, size_t n_volumes, size_t nvox, const float* noise_sigma,
+                        uint64_t receiver_seed, int32_t mode, float* out0, float* out1, void* stream, char* err,
+                        size_t errlen);
+
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);
 

@@ -1559,6 +1559,49 @@ hipError_t launch_partition_noise(float2* k, int nz, int C, int ny, int nx, cons
     return hipGetLastError();
 }
 
+// The tail of acquire_volumes (src/kspace.rs:2036-2060, 1840-1846): combined images [g][vox] of a
+// series (vox = x + nx*(y + ny*z)) -> f32, plus the image-space noise of SD sigma[vox] from
+// Rng((seed ^ g*0x9E37_79B9_7F4A_7C15 ^ vox*0xBF58_476D_1CE4_E5B9) | 1), two draws per component
+// (re then im), written to the entry point's layout vox*nv + g.  mode 0: real and imaginary parts;
+// mode 1: magnitude sqrt(re*re + im*im) and phase atan2(im, re) in f32, without contraction.
+__global__ __launch_bounds__(256) void series_epilogue(const float2* __restrict__ img, int nv, int nvox,
+                                                       const float* __restrict__ sigma, uint64_t seed, int mode,
+                                                       float* __restrict__ o0, float* __restrict__ o1) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nv * nvox) return;
+    const int g = i / nvox, vox = i - g * nvox;
+    const float2 c = img[i];
+    float re = c.x, im = c.y;
+    if (sigma) {
+        const double sd = (double)sigma[vox];
+        if (sd > 0.0) {
+            const uint64_t s0 = (seed ^ ((uint64_t)g * 0x9E3779B97F4A7C15ULL) ^ ((uint64_t)vox * 0xBF58476D1CE4E5B9ULL)) | 1ULL;
+            const double u1 = unit_of(splitmix_at(s0, 1)), u2 = unit_of(splitmix_at(s0, 2));
+            const double u3 = unit_of(splitmix_at(s0, 3)), u4 = unit_of(splitmix_at(s0, 4));
+            re += (float)(sqrt(-2.0 * log(u1)) * cos(kTau * u2) * sd);
+            im += (float)(sqrt(-2.0 * log(u3)) * cos(kTau * u4) * sd);
+        }
+    }
+    const size_t o = (size_t)vox * nv + g;
+    if (mode == 0) {
+        o0[o] = re;
+        o1[o] = im;
+    } else {
+        o0[o] = sqrtf(re * re + im * im);
+        o1[o] = atan2f(im, re);
+    }
+}
+
+hipError_t launch_series_epilogue(const float2* img, int nv, int nvox, const float* sigma, uint64_t seed, int mode,
+                                  float* o0, float* o1, hipStream_t st) {
+    const long n = (long)nv * nvox;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(series_epilogue, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, img, nv, nvox, sigma, seed,
+                       mode, o0, o1);
+    return hipGetLastError();
+}
+
 hipError_t launch_spikes_noise(int unit0, int nunits, int maxC, const DevUnit* u, const DevGroup* g,
                                const int32_t* rows, hipStream_t st) {
     if (nunits <= 0) return hipSuccess;

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -17,6 +18,8 @@ hipError_t launch_recon(const ReconGeom* g, const ReconItem* items, int nitems, 
                         size_t lds, hipStream_t st);
 size_t recon_lds_bytes(int nxP, int nyP);
 hipError_t set_recon_lds_limit(size_t bytes);
+hipError_t launch_series_epilogue(const float2* img, int nv, int nvox, const float* sigma, uint64_t seed, int mode,
+                                  float* o0, float* o1, hipStream_t st);
 }  // namespace mrs
 
 using namespace mrs;
@@ -341,5 +344,20 @@ extern "C" int mrs_simulate_slice(const mrs_slice_input* inp, const mrs_acquisit
         out[2 * i] = (float)img[i].re;
         out[2 * i + 1] = (float)img[i].im;
     }
+    return MRS_OK;
+}
+
+// The tail of acquire_volumes (src/kspace.rs:2036-2060, 1840-1846) over a series' combined images.
+extern "C" int mrs_series_epilogue(const mrs_c32* images, size_t n_volumes, size_t nvox, const float* noise_sigma,
+                                   uint64_t receiver_seed, int32_t mode, float* out0, float* out1, void* stream,
+                                   char* err, size_t errlen) {
+    if (!images || !out0 || !out1) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    if (mode != 0 && mode != 1) return fail(err, errlen, MRS_E_INVALID, "mode must be 0 (re, im) or 1 (magnitude, phase)");
+    if (n_volumes * nvox > (size_t)INT32_MAX) return fail(err, errlen, MRS_E_INVALID, "series too large for one launch");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = launch_series_epilogue(reinterpret_cast<const float2*>(images), (int)n_volumes, (int)nvox, noise_sigma,
+                                          receiver_seed, mode, out0, out1, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "series epilogue: %s", hipGetErrorString(e));
     return MRS_OK;
 }

@@ -93,6 +93,8 @@ def lib():
     L.mrs_partition_noise.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
                                       P(abi.Acquisition), ct.c_uint64, ct.c_uint64, ct.c_void_p] + E
     L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
+    L.mrs_series_epilogue.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_void_p, ct.c_uint64, ct.c_int32,
+                                      ct.c_void_p, ct.c_void_p, ct.c_void_p] + E
     _lib = L
     return L
 
@@ -284,6 +286,23 @@ def partition_noise(k3, acq: abi.Acquisition, volume: int, seed: int, stream=Non
     err = _errbuf()
     _check(lib().mrs_partition_noise(k3.data_ptr(), nz, C, nx, ny, ct.byref(acq), volume & 0xFFFFFFFFFFFFFFFF,
                                      seed & 0xFFFFFFFFFFFFFFFF, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+
+def series_epilogue(images, noise_sigma, receiver_seed: int, magnitude_phase: bool, stream=None):
+    """mrs_series_epilogue: images (nv, nvox) complex64 device tensor -> two (nvox*nv,) float32 device
+    tensors in the layout vox*nv + g: (re, im), or (magnitude, phase) when `magnitude_phase`."""
+    import torch
+    nv, nvox = images.shape
+    _check_dev("images", images, torch.complex64, nv * nvox)
+    _check_dev("noise_sigma", noise_sigma, torch.float32, nvox)
+    o0 = torch.empty(nv * nvox, dtype=torch.float32, device=images.device)
+    o1 = torch.empty_like(o0)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_series_epilogue(images.data_ptr(), nv, nvox, _ptr(noise_sigma), receiver_seed & 0xFFFFFFFFFFFFFFFF,
+                                     1 if magnitude_phase else 0, o0.data_ptr(), o1.data_ptr(),
+                                     ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+    return o0, o1
 
 
 def simulate_slice(b: abi.SliceInputBuilder, acq: abi.Acquisition) -> np.ndarray:
