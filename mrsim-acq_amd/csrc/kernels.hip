@@ -33,6 +33,13 @@ template <class T>
 __device__ __forceinline__ const GAS T* gld(const T* p) { return (const GAS T*)p; }
 template <class T>
 __device__ __forceinline__ GAS T* gst(T* p) { return (GAS T*)p; }
+// Element `i` of a device array addressed as uniform base + 32-bit byte offset (the offset is formed
+// in 32-bit arithmetic, so the load takes the saddr form: no per-lane 64-bit address arithmetic).
+// Only for arrays below 4 GiB, which every per-slice plane and unit image is.
+template <class V, class T>
+__device__ __forceinline__ const GAS V* at32(const T* base, uint32_t i) {
+    return (const GAS V*)((const GAS char*)base + i * (uint32_t)sizeof(T));
+}
 
 static constexpr double kTau = 6.283185307179586476925286766559;
 static constexpr float kLog2e = 1.4426950408889634f;
@@ -261,51 +268,6 @@ struct Vox4 {
     float ci[V][CPLX ? Q : 1][4];
 };
 
-template <int Q, bool CPLX, bool EDDY, int V>
-__device__ __forceinline__ void load_vox4(Vox4<Q, CPLX, EDDY, V>& v, const DevSlice& sl, const void* const (&images)[V],
-                                          int snx, int sny, int x, int y0, int snyP, int PP) {
-    const int o = x * snyP + y0;   // 32-bit offsets from scalar bases: global loads with saddr
-    const f64x2 b01 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o);
-    const f64x2 b23 = *reinterpret_cast<const GAS f64x2*>(gld(sl.base) + o + 2);
-    v.base[0] = b01.x; v.base[1] = b01.y; v.base[2] = b23.x; v.base[3] = b23.y;
-    if constexpr (EDDY) {
-        const f64x2 p01 = *reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o);
-        const f64x2 p23 = *reinterpret_cast<const GAS f64x2*>(gld(sl.poly) + o + 2);
-        v.poly[0] = p01.x; v.poly[1] = p01.y; v.poly[2] = p23.x; v.poly[3] = p23.y;
-    }
-    const f32x4 r4 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rate) + o);
-    v.rate[0] = r4.x; v.rate[1] = r4.y; v.rate[2] = r4.z; v.rate[3] = r4.w;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const f32x4 a = *reinterpret_cast<const GAS f32x4*>(gld(sl.nr2) + q * PP + o);
-        const f32x4 b = *reinterpret_cast<const GAS f32x4*>(gld(sl.nri) + q * PP + o);
-        v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
-        v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
-    }
-    const int P = snx * sny;
-#pragma unroll
-    for (int vv = 0; vv < V; vv++) {
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int y = y0 + j;
-            const bool ok = y < sny;                       // x < snx always holds here
-            const int i = x + snx * (ok ? y : sny - 1);    // clamped: every load is in bounds
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                if constexpr (CPLX) {
-                    const f32x2 m =
-                        *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(images[vv])) + 2 * (q * P + i));
-                    v.cr[vv][q][j] = ok ? m.x : 0.0f;
-                    v.ci[vv][q][j] = ok ? m.y : 0.0f;
-                } else {
-                    const float m = gld(reinterpret_cast<const float*>(images[vv]))[q * P + i];
-                    v.cr[vv][q][j] = ok ? m : 0.0f;
-                }
-            }
-        }
-    }
-}
-
 // Phase (sin, cos of 2 pi cyc) and per-compartment decay of voxel (x, y0+j) at one line:
 // yy = y0 + j - (sys + yoff).  Shared by every unit of the slice.
 template <int Q, bool CPLX, bool EDDY, int V>
@@ -363,6 +325,58 @@ __device__ __forceinline__ void unit_e(const Vox4<Q, CPLX, EDDY, V>& v, int vv, 
     }
 }
 
+// ----------------------------------------------------------------------------- y-stage operand staging
+// A wave's operands of one 16-row chunk (column x, rows ch..ch+15) are copied from HBM/L2 straight
+// into a per-wave LDS buffer by direct-to-LDS loads (global_load_lds), one instruction per array:
+// SGPR base + a lane-constant 32-bit offset, lane k writing the k-th 16-byte (or 4-byte) piece.  The
+// next chunk's copy is in flight while the current chunk is evaluated from the other buffer, so the
+// evaluation reads its operands at LDS latency.  Layout of one buffer, in 16-byte pieces:
+//   base[16] f64 | poly[16] f64 (eddy) | rate[16] | nr2[Q][16] | nri[Q][16] | s_hi[8RB][16] |
+//   s_lo[8RB][16] | rot[16] f32x2 (chains) | rq[2][Q][16] (chains) | images[V][Q][16] (x2 complex)
+// The coil planes are staged up to 32 coils (SA); above that their double buffer would cost more
+// LDS than two blocks per CU allow, and they are loaded directly.
+template <int Q, int RB, bool CPLX, bool EDDY, int V, bool ROT>
+struct YStage {
+    static constexpr bool SA = RB <= 4;
+    static constexpr int O_BASE = 0;
+    static constexpr int O_POLY = 8;
+    static constexpr int O_RATE = O_POLY + (EDDY ? 8 : 0);
+    static constexpr int O_NR2 = O_RATE + 4;
+    static constexpr int O_NRI = O_NR2 + 4 * Q;
+    static constexpr int O_SHI = O_NRI + 4 * Q;
+    static constexpr int O_SLO = O_SHI + (SA ? 32 * RB : 0);
+    static constexpr int O_ROT = O_SLO + (SA ? 32 * RB : 0);
+    static constexpr int O_RQ = O_ROT + (ROT ? 8 : 0);
+    static constexpr int O_IMG = O_RQ + (ROT ? 8 * Q : 0);
+    static constexpr int IMG_DW = Q * 16 * (CPLX ? 2 : 1);          // dwords per unit
+    static constexpr int PIECES = O_IMG + V * IMG_DW / 4;
+    static constexpr int WORDS = PIECES * 4;                        // u32 per buffer
+    static constexpr int NS = (32 * RB + 63) / 64;                  // copies per coil plane
+    static constexpr int NIMG = (IMG_DW + 63) / 64;                 // copies per unit image
+    static constexpr int NDMA = 4 + (EDDY ? 1 : 0) + (SA ? 2 * NS : 0) + V * NIMG;   // without the chain planes
+    static_assert(4 * Q <= 64 && 8 * Q <= 64, "one copy per compartment table");
+};
+
+// Direct-to-LDS copy of `bytes` (4 or 16) per lane from uniform base + per-lane byte offset into
+// lds + lane*bytes.
+template <int BYTES>
+__device__ __forceinline__ void lds_copy(const void* base, uint32_t off, uint32_t* lds) {
+    static_assert(BYTES == 4 || BYTES == 16, "dword or dwordx4 copies");
+    const GAS void* src = (const GAS void*)((const GAS char*)base + off);
+    __attribute__((address_space(3))) void* dst = (__attribute__((address_space(3))) void*)lds;
+    if constexpr (BYTES == 16)
+        __builtin_amdgcn_global_load_lds(src, dst, 16, 0, 0);
+    else
+        __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
+}
+
+// s_waitcnt vmcnt(N) with the other counters left open (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x70 | 0xF00);
+}
+
 // ----------------------------------------------------------------------------- y-stage, MFMA
 // Block = 4 waves = one (unit set, x tile of `xt` columns).  Per column x a wave accumulates
 //   G[(coil, part), line] = sum_{y, part'} A[(coil,part), (y,part')] * B[(y,part'), line]
@@ -370,11 +384,12 @@ __device__ __forceinline__ void unit_e(const Vox4<Q, CPLX, EDDY, V>& v, int vv, 
 // GP tiles per pass and RB 16-row blocks (8 coils each).  A set holds V units of one slice (its
 // volumes): the phase, the decay and A are evaluated once per voxel and line for all of them.
 // Occupancy bound: 4 waves/SIMD (128 VGPRs) where the instantiation fits without spilling, 2 for the
-// 16-accumulator tiles of many coils over 64 lines and for unit sets, else 3.
+// 16-accumulator tiles of many coils over 64 lines, for 32 coils and more (their staging buffers) and
+// for unit sets, else 3.
 template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
 constexpr int ystage_waves() {
     if constexpr (V > 1) return (V == 2 && (CPLX ? 2 * Q : Q) <= 2) ? 3 : 2;
-    return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : (RB * GP >= 8 ? 2 : 3);
+    return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : ((RB * GP >= 8 || RB >= 4) ? 2 : 3);
 }
 
 template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
@@ -412,31 +427,145 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
     const int C = gr.C, LyP = gr.LyP, G = LyP >> 4;
     const int part = ln >> 3;
     const double sys_yoff = gr.sys_yoff;
-    for (int xi = wave; xi < xt; xi += 4) {
-        const int x = wi.tile * xt + xi;
-        if (x >= snx) break;
-        for (int g0 = 0; g0 < G; g0 += GP) {
-            f32x4 acc[V][RB][GP];
+    // The wave's stages: columns x = tile*xt + wave + 4k, passes of GP line tiles, 16-row chunks.
+    using L = YStage<Q, RB, CPLX, EDDY, V, GP == 4 && !EDDY>;
+    __shared__ __attribute__((aligned(16))) uint32_t ybuf[4][2][L::WORDS];
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int x0w = wi.tile * xt + wv;
+    const int nxw = x0w < snx ? min((xt - wv + 3) / 4, (snx - x0w + 3) / 4) : 0;
+    const int npass = (G + GP - 1) / GP, nch = snyP >> 4;
+    const int nstage = nxw * npass * nch;
+    const bool rot = (GP == 4 && !EDDY) && gr.rot_shared;
+    const uint32_t P = (uint32_t)(snx * sny);
+    // lane-constant byte offsets of the copies
+    const uint32_t o16 = (uint32_t)lane * 16u;                                           // contiguous pieces
+    const uint32_t oq = ((uint32_t)(lane >> 2) * (uint32_t)PP + 4u * (uint32_t)(lane & 3)) * 4u;   // [q][16]
+    const uint32_t oc = ((uint32_t)(lane >> 2) * (uint32_t)snyP + 4u * (uint32_t)(lane & 3)) * 4u; // [coil][16]
+    const uint32_t cm = CPLX ? 2u : 1u;
+    // image dword d of a unit: (q, row, part) = (d / (16 cm), (d / cm) % 16, d % cm)
+    auto img_off = [&](int d, int x, int ch) -> uint32_t {
+        const int q = d / (16 * (int)cm), yy = (d / (int)cm) & 15, pt = d % (int)cm;
+        const int y = min(ch + yy, sny - 1);
+        return (((uint32_t)q * P + (uint32_t)(x + snx * y)) * cm + (uint32_t)pt) * 4u;
+    };
+    auto stage_copy = [&](int x, int ch, uint32_t* buf) {
+        const uint32_t xy = (uint32_t)(x * snyP + ch);
+        if (lane < 8) lds_copy<16>(sl.base, xy * 8u + o16, buf + L::O_BASE * 4);
+        if constexpr (EDDY) {
+            if (lane < 8) lds_copy<16>(sl.poly, xy * 8u + o16, buf + L::O_POLY * 4);
+        }
+        if (lane < 4) lds_copy<16>(sl.rate, xy * 4u + o16, buf + L::O_RATE * 4);
+        if (lane < 4 * Q) lds_copy<16>(sl.nr2, xy * 4u + oq, buf + L::O_NR2 * 4);
+        if (lane < 4 * Q) lds_copy<16>(sl.nri, xy * 4u + oq, buf + L::O_NRI * 4);
+        if constexpr (L::SA) {
+            const uint32_t xs = (uint32_t)(x * (RB * 8) * snyP + ch) * 4u;
+#pragma unroll
+            for (int k = 0; k < L::NS; k++) {
+                const int nk = min(64, 32 * RB - 64 * k);
+                const uint32_t ok = xs + (uint32_t)(k * 16 * snyP) * 4u + oc;
+                if (lane < nk) lds_copy<16>(sl.s_hi, ok, buf + (L::O_SHI + 64 * k) * 4);
+                if (lane < nk) lds_copy<16>(sl.s_lo, ok, buf + (L::O_SLO + 64 * k) * 4);
+            }
+        }
+        if constexpr (GP == 4 && !EDDY) {
+            if (rot) {
+                if (lane < 8) lds_copy<16>(sl.rot, xy * 8u + o16, buf + L::O_ROT * 4);
+                if (lane < 8 * Q) lds_copy<16>(sl.rq, xy * 4u + oq, buf + L::O_RQ * 4);
+            }
+        }
+#pragma unroll
+        for (int vv = 0; vv < V; vv++) {
+#pragma unroll
+            for (int k = 0; k < L::NIMG; k++) {
+                const int d = 64 * k + lane;
+                if (d < L::IMG_DW) lds_copy<4>(img[vv], img_off(d, x, ch), buf + L::O_IMG * 4 + vv * L::IMG_DW + 64 * k);
+            }
+        }
+    };
+    f32x4 acc[V][RB][GP];
+    int sxw = 0, spi = 0, sch = 0;            // the stage being evaluated
+    if (nstage > 0) stage_copy(x0w, 0, &ybuf[wv][0][0]);
+    for (int st = 0; st < nstage; st++) {
+        const int x = x0w + 4 * sxw, g0 = spi * GP, ch = sch * 16;
+        // next stage's copy into the other buffer, then wait for this stage's
+        int nxw_ = sxw, npi = spi, nchi = sch + 1;
+        if (nchi == nch) { nchi = 0; if (++npi == npass) { npi = 0; ++nxw_; } }
+        if (st + 1 < nstage) {
+            stage_copy(x0w + 4 * nxw_, nchi * 16, &ybuf[wv][(st + 1) & 1][0]);
+            if constexpr (GP == 4 && !EDDY) {
+                if (rot) wait_vm<L::NDMA + 2>(); else wait_vm<L::NDMA>();
+            } else {
+                wait_vm<L::NDMA>();
+            }
+        } else {
+            wait_vm<0>();
+        }
+        const uint32_t* B = &ybuf[wv][st & 1][0];
+        if (sch == 0) {
 #pragma unroll
             for (int vv = 0; vv < V; vv++)
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                     for (int g = 0; g < GP; g++) acc[vv][rb][g] = f32x4{0.f, 0.f, 0.f, 0.f};
-            for (int ch = 0; ch < snyP; ch += 16) {
+        }
+        {
                 const int y0 = ch + lq * 4;
                 Vox4<Q, CPLX, EDDY, V> v;
-                load_vox4<Q, CPLX, EDDY, V>(v, sl, img, snx, sny, x, y0, snyP, (int)PP);
+                {
+                    const f64x2 b01 = *reinterpret_cast<const f64x2*>(B + L::O_BASE * 4 + lq * 8);
+                    const f64x2 b23 = *reinterpret_cast<const f64x2*>(B + L::O_BASE * 4 + lq * 8 + 4);
+                    v.base[0] = b01.x; v.base[1] = b01.y; v.base[2] = b23.x; v.base[3] = b23.y;
+                    if constexpr (EDDY) {
+                        const f64x2 p01 = *reinterpret_cast<const f64x2*>(B + L::O_POLY * 4 + lq * 8);
+                        const f64x2 p23 = *reinterpret_cast<const f64x2*>(B + L::O_POLY * 4 + lq * 8 + 4);
+                        v.poly[0] = p01.x; v.poly[1] = p01.y; v.poly[2] = p23.x; v.poly[3] = p23.y;
+                    }
+                    const f32x4 r4 = *reinterpret_cast<const f32x4*>(B + L::O_RATE * 4 + lq * 4);
+                    v.rate[0] = r4.x; v.rate[1] = r4.y; v.rate[2] = r4.z; v.rate[3] = r4.w;
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const f32x4 a = *reinterpret_cast<const f32x4*>(B + L::O_NR2 * 4 + q * 16 + lq * 4);
+                        const f32x4 b = *reinterpret_cast<const f32x4*>(B + L::O_NRI * 4 + q * 16 + lq * 4);
+                        v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
+                        v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
+                    }
+                    // padded rows (y >= sny) read the clamped last row: finite E against zero coil rows
+#pragma unroll
+                    for (int vv = 0; vv < V; vv++) {
+                        const uint32_t* ib = B + L::O_IMG * 4 + vv * L::IMG_DW;
+#pragma unroll
+                        for (int q = 0; q < Q; q++) {
+                            if constexpr (CPLX) {
+                                const f32x4 m01 = *reinterpret_cast<const f32x4*>(ib + (q * 16 + lq * 4) * 2);
+                                const f32x4 m23 = *reinterpret_cast<const f32x4*>(ib + (q * 16 + lq * 4) * 2 + 4);
+                                v.cr[vv][q][0] = m01.x; v.ci[vv][q][0] = m01.y; v.cr[vv][q][1] = m01.z; v.ci[vv][q][1] = m01.w;
+                                v.cr[vv][q][2] = m23.x; v.ci[vv][q][2] = m23.y; v.cr[vv][q][3] = m23.z; v.ci[vv][q][3] = m23.w;
+                            } else {
+                                const f32x4 m = *reinterpret_cast<const f32x4*>(ib + q * 16 + lq * 4);
+                                v.cr[vv][q][0] = m.x; v.cr[vv][q][1] = m.y; v.cr[vv][q][2] = m.z; v.cr[vv][q][3] = m.w;
+                            }
+                        }
+                    }
+                }
                 u32x4 ahi[RB], alo[RB];
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++) {
-                    // split maps are padded to RB*8 coils with zeros: no range check
-                    const int so = (x * (RB * 8) + rb * 8 + (ln & 7)) * snyP + y0;
-                    const u32x4 h = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_hi) + so);
-                    const u32x4 l = *reinterpret_cast<const GAS u32x4*>(gld(sl.s_lo) + so);
+                    u32x4 h, l;
+                    if constexpr (L::SA) {
+                        const int so = ((rb * 8 + (ln & 7)) * 16 + lq * 4);
+                        h = *reinterpret_cast<const u32x4*>(B + L::O_SHI * 4 + so);
+                        l = *reinterpret_cast<const u32x4*>(B + L::O_SLO * 4 + so);
+                    } else {
+                        // split maps are padded to RB*8 coils with zeros: no range check
+                        const uint32_t so = (uint32_t)((x * (RB * 8) + rb * 8 + (ln & 7)) * snyP + y0);
+                        h = *at32<u32x4>(sl.s_hi, so);
+                        l = *at32<u32x4>(sl.s_lo, so);
+                    }
                     ahi[rb] = rows_of(h, part);
                     alo[rb] = rows_of(l, part);
                 }
+                bool chained = false;
                 const double yy0 = (double)y0 - sys_yoff;
                 if constexpr (GP == 4 && !EDDY) {
                     if ((g0 >> 2) < gr.rotor_passes) {
@@ -445,13 +574,12 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         // the voxel's rotor cis(2 pi (rate dt + dkyn yy)) and each decay by the voxel's
                         // exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.
                         const LineRec& L0 = lines_s[g0 * 16 + ln];
-                        const int ro = x * snyP + y0;
-                        const f32x4 srot0 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro);
-                        const f32x4 srot1 = *reinterpret_cast<const GAS f32x4*>(gld(sl.rot) + ro + 2);
+                        const f32x4 srot0 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8);
+                        const f32x4 srot1 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8 + 4);
                         f32x4 srq[Q];
 #pragma unroll
                         for (int q = 0; q < Q; q++)
-                            srq[q] = *reinterpret_cast<const GAS f32x4*>(gld(sl.rq) + (L0.cls * Q + q) * (int)PP + ro);
+                            srq[q] = *reinterpret_cast<const f32x4*>(B + L::O_RQ * 4 + (L0.cls * Q + q) * 16 + lq * 4);
                         if constexpr (V == 1) {
                             u32x4 bh[4], bl[4];
 #pragma unroll
@@ -531,70 +659,74 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                                         acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
                             }
                         }
-                        continue;
+                        chained = true;
                     }
                 }
-#pragma unroll
-                for (int g = 0; g < GP; g++) {
-                    if (g0 + g < G) {
-                        const LineRec& L = lines_s[(g0 + g) * 16 + ln];
-                        const bool ok = L.ky >= 0;
-                        const double t_s = L.t_s, kyn = L.kyn, d = L.d;
-                        const float trf = L.trf, tau = L.tau;
-                        u32x4 bhi[V], blo[V];
-#pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            if constexpr (V == 1) {
-                                float er, ei;
-                                gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
-                                uint32_t h, l;
-                                split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
-                                bhi[0][j] = h;
-                                blo[0][j] = l;
-                            } else {
-                                float sn, cs, dq[Q];
-                                phase_decay(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, sn, cs, dq);
-#pragma unroll
-                                for (int vv = 0; vv < V; vv++) {
+                if (!chained) {
+    #pragma unroll
+                    for (int g = 0; g < GP; g++) {
+                        if (g0 + g < G) {
+                            const LineRec& L = lines_s[(g0 + g) * 16 + ln];
+                            const bool ok = L.ky >= 0;
+                            const double t_s = L.t_s, kyn = L.kyn, d = L.d;
+                            const float trf = L.trf, tau = L.tau;
+                            u32x4 bhi[V], blo[V];
+    #pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                if constexpr (V == 1) {
                                     float er, ei;
-                                    unit_e(v, vv, j, sn, cs, dq, er, ei);
+                                    gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
                                     uint32_t h, l;
                                     split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
-                                    bhi[vv][j] = h;
-                                    blo[vv][j] = l;
+                                    bhi[0][j] = h;
+                                    blo[0][j] = l;
+                                } else {
+                                    float sn, cs, dq[Q];
+                                    phase_decay(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, sn, cs, dq);
+    #pragma unroll
+                                    for (int vv = 0; vv < V; vv++) {
+                                        float er, ei;
+                                        unit_e(v, vv, j, sn, cs, dq, er, ei);
+                                        uint32_t h, l;
+                                        split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
+                                        bhi[vv][j] = h;
+                                        blo[vv][j] = l;
+                                    }
                                 }
                             }
+    #pragma unroll
+                            for (int vv = 0; vv < V; vv++)
+    #pragma unroll
+                                for (int rb = 0; rb < RB; rb++)
+                                    acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bhi[vv], blo[vv], acc[vv][rb][g]);
                         }
-#pragma unroll
-                        for (int vv = 0; vv < V; vv++)
-#pragma unroll
-                            for (int rb = 0; rb < RB; rb++)
-                                acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bhi[vv], blo[vv], acc[vv][rb][g]);
                     }
                 }
-            }
-            // acc[vv][rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
+        }
+        if (sch == nch - 1) {
+        // acc[vv][rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
 #pragma unroll
-            for (int g = 0; g < GP; g++) {
-                if (g0 + g < G) {
-                    const int lp = (g0 + g) * 16 + ln;
+        for (int g = 0; g < GP; g++) {
+            if (g0 + g < G) {
+                const int lp = (g0 + g) * 16 + ln;
 #pragma unroll
-                    for (int vv = 0; vv < V; vv++)
+                for (int vv = 0; vv < V; vv++)
 #pragma unroll
-                        for (int rb = 0; rb < RB; rb++) {
+                    for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-                            for (int r = 0; r < 4; r++) {
-                                const int c = rb * 8 + 4 * (lq & 1) + r;
-                                if (c < C) {
-                                    GAS float* dst =
-                                        reinterpret_cast<GAS float*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C + c);
-                                    dst[lq >> 1] = acc[vv][rb][g][r];
-                                }
+                        for (int r = 0; r < 4; r++) {
+                            const int c = rb * 8 + 4 * (lq & 1) + r;
+                            if (c < C) {
+                                GAS float* dst =
+                                    reinterpret_cast<GAS float*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C + c);
+                                dst[lq >> 1] = acc[vv][rb][g][r];
                             }
                         }
-                }
+                    }
             }
         }
+        }
+        sxw = nxw_; spi = npi; sch = nchi;
     }
 }
 
@@ -622,68 +754,127 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
     const int kxb = wi.tile / nparts, part = wi.tile % nparts;
     const int kx0 = kxb * rows;
     const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
+    // Stage the twiddle rows with SU loads of each kind in flight per thread (one latency per SU
+    // rows of work rather than per row).
+    constexpr int SU = 8;
+    const int per_tab = rows * (snxP / 4);
     for (int tb = 0; tb < ntab; tb++) {
-        for (int i = threadIdx.x; i < rows * (snxP / 4); i += 256) {
-            const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
-            const int kx = kx0 + r;
-            u32x4 h = u32x4{0u, 0u, 0u, 0u}, l = u32x4{0u, 0u, 0u, 0u};
-            if (kx < nxP) {
-                h = *reinterpret_cast<const GAS u32x4*>(gld(gr.t_hi[tb]) + (size_t)kx * snxP + x4);
-                l = *reinterpret_cast<const GAS u32x4*>(gld(gr.t_lo[tb]) + (size_t)kx * snxP + x4);
+        for (int i0 = threadIdx.x; i0 < per_tab; i0 += 256 * SU) {
+            u32x4 h[SU], l[SU];
+#pragma unroll
+            for (int k = 0; k < SU; k++) {
+                const int i = i0 + k * 256;
+                const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
+                const int kx = kx0 + r;
+                h[k] = l[k] = u32x4{0u, 0u, 0u, 0u};
+                if (i < per_tab && kx < nxP) {
+                    h[k] = *at32<u32x4>(gr.t_hi[tb], (uint32_t)(kx * snxP + x4));
+                    l[k] = *at32<u32x4>(gr.t_lo[tb], (uint32_t)(kx * snxP + x4));
+                }
             }
-            *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h;
-            *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l;
-        }
-    }
-    __syncthreads();
-    const int ntiles = (LyP * C) >> 4;
-    for (int nt = part * 4 + wave; nt < ntiles; nt += nparts * 4) {
-        const int n0 = nt * 16;
-        const int n = n0 + ln;
-        const int lp = n / C, c = n % C;
-        const int tb = ntab == 2 ? gr.lines[n0 / C].table : 0;
-        const uint32_t* __restrict__ th = tl + (tb * 2 + 0) * rows * stride;
-        const uint32_t* __restrict__ tlo = tl + (tb * 2 + 1) * rows * stride;
-        f32x4 accr[MT], acci[MT];
 #pragma unroll
-        for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const GAS f32x2* __restrict__ gcol = reinterpret_cast<const GAS f32x2*>(gld(u.gbuf) + (size_t)lp * C + c);
-        const size_t xstep = (size_t)LyP * C;
-        for (int ch = 0; ch < snxP; ch += 16) {
-            const int x0 = ch + lq * 4;
-            u32x4 bhi, blo;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const f32x2 gv = gcol[(size_t)(x0 + j) * xstep];
-                uint32_t h, l;
-                split2(gv.x, gv.y, h, l);
-                bhi[j] = h;
-                blo[j] = l;
-            }
-            // B_re = [G_re, -G_im] (negate the high halves), B_im = [G_im, G_re] (swap halves)
-            const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
-            const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
-#pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const int r = m * 16 + ln;
-                const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
-                const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
-                accr[m] = mfma3(h, l, brh, brl, accr[m]);
-                acci[m] = mfma3(h, l, bih, bil, acci[m]);
-            }
-        }
-        const LineRec Lrow = gr.lines[lp];
-        if (Lrow.ky >= 0) {
-            GAS f32x2* orow = reinterpret_cast<GAS f32x2*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
-#pragma unroll
-            for (int m = 0; m < MT; m++) {
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int kx = kx0 + m * 16 + 4 * lq + r;
-                    if (kx < nx) orow[kx] = f32x2{accr[m][r], acci[m][r]};
+            for (int k = 0; k < SU; k++) {
+                const int i = i0 + k * 256;
+                if (i < per_tab) {
+                    const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
+                    *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h[k];
+                    *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l[k];
                 }
             }
         }
+    }
+    __syncthreads();
+    // The wave's column tiles nt = part*4 + wave + k*nparts*4, each a sequence of 16-x chunks of G
+    // loaded in batches of XB chunks (4 complex values per lane per chunk).  Batches run as one flat
+    // sequence over the wave's tiles, double-buffered: the next batch (of this tile or the next) is in
+    // flight while the current one is split and multiplied, so each wave keeps 4*XB loads in flight.
+    constexpr int XB = 8;
+    const int ntiles = (LyP * C) >> 4;
+    const int first = part * 4 + wave, step = nparts * 4;
+    const int ntw = first < ntiles ? (ntiles - first + step - 1) / step : 0;
+    const int nch = snxP >> 4;
+    const int nbt = (nch + XB - 1) / XB;
+    const int nb = ntw * nbt;
+    const uint32_t xstep = (uint32_t)(LyP * C);
+    const f32x2* gb = reinterpret_cast<const f32x2*>(u.gbuf);
+    auto load_batch = [&](int b, f32x2 (&buf)[XB][4]) {
+        const int nt = first + (b / nbt) * step, k = b % nbt;
+        const uint32_t n = (uint32_t)(nt * 16 + ln);
+#pragma unroll
+        for (int c = 0; c < XB; c++) {
+            const int ch = k * XB + c;
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                if (ch < nch) {
+                    buf[c][jj] = *at32<f32x2>(gb, (uint32_t)(ch * 16 + lq * 4 + jj) * xstep + n);
+                } else {
+                    buf[c][jj] = f32x2{0.f, 0.f};
+                }
+            }
+        }
+    };
+    f32x4 accr[MT], acci[MT];
+    auto compute_batch = [&](int b, const f32x2 (&buf)[XB][4]) {
+        const int nt = first + (b / nbt) * step, k = b % nbt;
+        const int n0 = nt * 16;
+        const int tb = ntab == 2 ? gr.lines[n0 / C].table : 0;
+        const uint32_t* __restrict__ th = tl + (tb * 2 + 0) * rows * stride;
+        const uint32_t* __restrict__ tlo = tl + (tb * 2 + 1) * rows * stride;
+        if (k == 0) {
+#pragma unroll
+            for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int c = 0; c < XB; c++) {
+            const int ch = k * XB + c;
+            if (ch < nch) {
+                const int x0 = ch * 16 + lq * 4;
+                u32x4 bhi, blo;
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    uint32_t h, l;
+                    split2(buf[c][jj].x, buf[c][jj].y, h, l);
+                    bhi[jj] = h;
+                    blo[jj] = l;
+                }
+                // B_re = [G_re, -G_im] (negate the high halves), B_im = [G_im, G_re] (swap halves)
+                const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+                const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    const int r = m * 16 + ln;
+                    const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
+                    const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
+                    accr[m] = mfma3(h, l, brh, brl, accr[m]);
+                    acci[m] = mfma3(h, l, bih, bil, acci[m]);
+                }
+            }
+        }
+        if (k == nbt - 1) {
+            const int n = n0 + ln;
+            const int lp = n / C, c = n % C;
+            const LineRec Lrow = gr.lines[lp];
+            if (Lrow.ky >= 0) {
+                GAS f32x2* orow = reinterpret_cast<GAS f32x2*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int kx = kx0 + m * 16 + 4 * lq + r;
+                        if (kx < nx) orow[kx] = f32x2{accr[m][r], acci[m][r]};
+                    }
+                }
+            }
+        }
+    };
+    f32x2 bufa[XB][4], bufb[XB][4];
+    if (nb > 0) load_batch(0, bufa);
+    for (int b = 0; b < nb; b += 2) {
+        if (b + 1 < nb) load_batch(b + 1, bufb);
+        compute_batch(b, bufa);
+        if (b + 1 >= nb) break;
+        if (b + 2 < nb) load_batch(b + 2, bufa);
+        compute_batch(b + 1, bufb);
     }
 }
 
