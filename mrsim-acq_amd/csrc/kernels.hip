@@ -370,6 +370,26 @@ __device__ __forceinline__ void lds_copy(const void* base, uint32_t off, uint32_
         __builtin_amdgcn_global_load_lds(src, dst, 4, 0, 0);
 }
 
+// The images of the V units at the lane's four voxels (rows 4*lq..4*lq+3) from a stage buffer.
+template <int Q, bool CPLX, bool EDDY, int V>
+__device__ __forceinline__ void read_images(Vox4<Q, CPLX, EDDY, V>& v, const uint32_t* ib, int img_dw, int lq) {
+#pragma unroll
+    for (int vv = 0; vv < V; vv++) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if constexpr (CPLX) {
+                const f32x4 m01 = *reinterpret_cast<const f32x4*>(ib + vv * img_dw + (q * 16 + lq * 4) * 2);
+                const f32x4 m23 = *reinterpret_cast<const f32x4*>(ib + vv * img_dw + (q * 16 + lq * 4) * 2 + 4);
+                v.cr[vv][q][0] = m01.x; v.ci[vv][q][0] = m01.y; v.cr[vv][q][1] = m01.z; v.ci[vv][q][1] = m01.w;
+                v.cr[vv][q][2] = m23.x; v.ci[vv][q][2] = m23.y; v.cr[vv][q][3] = m23.z; v.ci[vv][q][3] = m23.w;
+            } else {
+                const f32x4 m = *reinterpret_cast<const f32x4*>(ib + vv * img_dw + q * 16 + lq * 4);
+                v.cr[vv][q][0] = m.x; v.cr[vv][q][1] = m.y; v.cr[vv][q][2] = m.z; v.cr[vv][q][3] = m.w;
+            }
+        }
+    }
+}
+
 // s_waitcnt vmcnt(N) with the other counters left open (gfx9 encoding).
 template <int N>
 __device__ __forceinline__ void wait_vm() {
@@ -429,6 +449,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
     const double sys_yoff = gr.sys_yoff;
     // The wave's stages: columns x = tile*xt + wave + 4k, passes of GP line tiles, 16-row chunks.
     using L = YStage<Q, RB, CPLX, EDDY, V, GP == 4 && !EDDY>;
+    constexpr bool img_late = V >= 4;
     __shared__ __attribute__((aligned(16))) uint32_t ybuf[4][2][L::WORDS];
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int x0w = wi.tile * xt + wv;
@@ -530,23 +551,10 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                         v.nr2[q][0] = a.x; v.nr2[q][1] = a.y; v.nr2[q][2] = a.z; v.nr2[q][3] = a.w;
                         v.nri[q][0] = b.x; v.nri[q][1] = b.y; v.nri[q][2] = b.z; v.nri[q][3] = b.w;
                     }
-                    // padded rows (y >= sny) read the clamped last row: finite E against zero coil rows
-#pragma unroll
-                    for (int vv = 0; vv < V; vv++) {
-                        const uint32_t* ib = B + L::O_IMG * 4 + vv * L::IMG_DW;
-#pragma unroll
-                        for (int q = 0; q < Q; q++) {
-                            if constexpr (CPLX) {
-                                const f32x4 m01 = *reinterpret_cast<const f32x4*>(ib + (q * 16 + lq * 4) * 2);
-                                const f32x4 m23 = *reinterpret_cast<const f32x4*>(ib + (q * 16 + lq * 4) * 2 + 4);
-                                v.cr[vv][q][0] = m01.x; v.ci[vv][q][0] = m01.y; v.cr[vv][q][1] = m01.z; v.ci[vv][q][1] = m01.w;
-                                v.cr[vv][q][2] = m23.x; v.ci[vv][q][2] = m23.y; v.cr[vv][q][3] = m23.z; v.ci[vv][q][3] = m23.w;
-                            } else {
-                                const f32x4 m = *reinterpret_cast<const f32x4*>(ib + q * 16 + lq * 4);
-                                v.cr[vv][q][0] = m.x; v.cr[vv][q][1] = m.y; v.cr[vv][q][2] = m.z; v.cr[vv][q][3] = m.w;
-                            }
-                        }
-                    }
+                    // padded rows (y >= sny) read the clamped last row: finite E against zero coil rows.
+                    // Sets of four re-read the images per line (img_late) so only one line's worth of
+                    // them is live next to the four units' accumulators.
+                    if constexpr (!img_late) read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
                 }
                 u32x4 ahi[RB], alo[RB];
 #pragma unroll
@@ -630,6 +638,10 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                                             dq[j]);
 #pragma unroll
                             for (int g = 0; g < 4; g++) {
+                                if constexpr (img_late) {
+                                    asm volatile("" ::: "memory");   // keep the reads in this line's iteration
+                                    read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
+                                }
                                 u32x4 bh[V], bl[V];
 #pragma unroll
                                 for (int j = 0; j < 4; j++) {
@@ -670,6 +682,10 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                             const bool ok = L.ky >= 0;
                             const double t_s = L.t_s, kyn = L.kyn, d = L.d;
                             const float trf = L.trf, tau = L.tau;
+                            if constexpr (img_late) {
+                                asm volatile("" ::: "memory");
+                                read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
+                            }
                             u32x4 bhi[V], blo[V];
     #pragma unroll
                             for (int j = 0; j < 4; j++) {
@@ -1590,8 +1606,8 @@ static hipError_t ylaunch_rb(int rb, int gp, int nv, dim3 grid, hipStream_t st, 
         if constexpr (!EDDY) {
             if (rb == 1 && gp >= 4) {
                 if (nv == 2) return ylaunch<Q, 1, 4, CPLX, EDDY, 2>(grid, st, u, s, g, w, usets, xt);
-                // four units' images and accumulators fit without spilling for up to two image planes
-                if constexpr ((CPLX ? 2 * Q : Q) <= 2) {
+                // four units' images and accumulators fit without spilling for up to three image planes
+                if constexpr ((CPLX ? 2 * Q : Q) <= 3) {
                     if (nv == 4) return ylaunch<Q, 1, 4, CPLX, EDDY, 4>(grid, st, u, s, g, w, usets, xt);
                 }
             }

@@ -843,8 +843,8 @@ static std::vector<size_t> unit_order(const mrs_plan* p) {
 // decay); a class gets them only where its kernel has the variant (one row block, four line tiles,
 // no eddy term).  Leftover units run one per item.
 static bool set_eligible(const mrs_plan* p, const mrs_plan::YClass& c, int gp) {
-    const int planes = c.cplx ? 2 * c.Q : c.Q;   // sets of four: up to two image planes per unit
-    return p->nv > 1 && c.RB == 1 && gp >= 4 && !c.eddy && (p->nv == 2 || planes <= 2);
+    const int planes = c.cplx ? 2 * c.Q : c.Q;   // sets of four: up to three image planes per unit
+    return p->nv > 1 && c.RB == 1 && gp >= 4 && !c.eddy && (p->nv == 2 || planes <= 3);
 }
 
 static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs_plan::YClass& c, int xt, int gp,
@@ -1208,7 +1208,7 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
         bool any2 = false, any4 = false;
         for (auto& c : p->yclasses) {
             any2 |= c.RB == 1 && !c.eddy;
-            any4 |= c.RB == 1 && !c.eddy && (c.cplx ? 2 * c.Q : c.Q) <= 2;
+            any4 |= c.RB == 1 && !c.eddy && (c.cplx ? 2 * c.Q : c.Q) <= 3;
         }
         if (any2 && most >= 2) nvs.push_back(2);
         if (any4 && most >= 4) nvs.push_back(4);

@@ -214,6 +214,7 @@ def _set_plan_outputs(w, tiles):
 
 
 @pytest.mark.parametrize("name,kw,nv", [("batch_128", dict(n_slices=3, n_volumes=6), 2),
+                                        ("batch_128", dict(n_slices=2, n_volumes=7), 4),
                                         ("unitsets_q1", dict(), 4), ("unitsets_q1", dict(), 2),
                                         ("unitsets_cplx", dict(), 4)])
 def test_unit_sets_match_single_units_and_oracle(name, kw, nv):
