@@ -224,33 +224,48 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
     if (!sl.s_owner) return;
     const bool w_split = mode & PREP_SPLIT, w_s32 = mode & PREP_S32;
     const int Cp = w_split ? gr.RB * 8 : (w_s32 ? C : 0);   // split maps padded with zero coils to 8-coil row blocks
-    for (int c = 0; c < Cp; c++) {
-        __syncthreads();
-        for (int k = 0; k < 4; k++) {
-            const int yl = ty + 8 * k, x = x0 + tx, y = y0 + yl;
-            float2 v = make_float2(0.f, 0.f);
-            if (c < C && x < snx && y < sny) {
-                const f32x2 m = *reinterpret_cast<const GAS f32x2*>(gld(reinterpret_cast<const float*>(sl.coil_maps)) +
-                                                                    2 * ((size_t)c * P + (size_t)x + (size_t)snx * y));
-                const float a = samp[tx][yl];
-                v = make_float2(m.x * a, m.y * a);
+    // Coils in groups of CG: every coil map value of the group is loaded before the first is
+    // written, so a block waits for one memory latency per group rather than one per coil.
+    constexpr int CG = 4;
+    for (int c0 = 0; c0 < Cp; c0 += CG) {
+        f32x2 m[CG][4];
+#pragma unroll
+        for (int cc = 0; cc < CG; cc++) {
+            const int c = c0 + cc;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int yl = ty + 8 * k, x = x0 + tx, y = y0 + yl;
+                m[cc][k] = f32x2{0.f, 0.f};
+                if (c < C && x < snx && y < sny)
+                    m[cc][k] = *at32<f32x2>(sl.coil_maps, (uint32_t)c * (uint32_t)P + (uint32_t)(x + snx * y));
             }
-            sc[tx][yl] = v;
         }
-        __syncthreads();
-        for (int k = 0; k < 4; k++) {
-            const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
-            if (x >= snxP || y >= snyP) continue;
-            const float2 v = sc[xl][wy];
-            if (w_split) {
-                uint32_t hi, lo;
-                split2(v.x, v.y, hi, lo);
-                const size_t so = ((size_t)x * Cp + c) * snyP + y;
-                gst(sl.s_hi)[so] = hi;
-                gst(sl.s_lo)[so] = lo;
+#pragma unroll
+        for (int cc = 0; cc < CG; cc++) {
+            const int c = c0 + cc;
+            if (c >= Cp) break;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int yl = ty + 8 * k;
+                const float a = samp[tx][yl];
+                sc[tx][yl] = make_float2(m[cc][k].x * a, m[cc][k].y * a);
             }
-            if (w_s32 && c < C)
-                reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
+            __syncthreads();
+            for (int k = 0; k < 4; k++) {
+                const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
+                if (x >= snxP || y >= snyP) continue;
+                const float2 v = sc[xl][wy];
+                if (w_split) {
+                    uint32_t hi, lo;
+                    split2(v.x, v.y, hi, lo);
+                    const size_t so = ((size_t)x * Cp + c) * snyP + y;
+                    gst(sl.s_hi)[so] = hi;
+                    gst(sl.s_lo)[so] = lo;
+                }
+                if (w_s32 && c < C)
+                    reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
+            }
         }
     }
 }
