@@ -261,13 +261,93 @@ static void spikes_noise(C* k, size_t nx, size_t ny, const mrs_acquisition* a, c
     }
 }
 
+/* ---------------------------------------------------------------- type-1 NUFFT (src/nufft.rs:25-113)
+ * F[k] = sum_j w_j exp(+i 2 pi k v_j / n), k = k_lo .. k_lo + n_out - 1, by gridding with the
+ * exponential-of-semicircle kernel at oversampling 2: width w = clamp(ceil(-log10 tol) + 1, 2, 16),
+ * beta = 2.30 w, fine grid nf = 2n, spread, inverse DFT of the fine grid (rustfft's unnormalised
+ * inverse in the reference; evaluated here directly at the n_out needed bins), divided by the
+ * kernel's own transform corr[j] = 1 / (psi(0) + sum_{1 <= m < w/2} 2 psi(m / (w/2)) cos(2 pi k m / nf)).
+ * pos: nsrc cells; wre/wim: nw vectors of nsrc; out_re/out_im: nw vectors of n_out. */
+static double es_psi(double z, double beta) { return fabs(z) < 1.0 ? exp(beta * (sqrt(1.0 - z * z) - 1.0)) : 0.0; }
+
+int orc_nufft1(size_t n, long k_lo, size_t n_out, double tol, const double* pos, size_t nsrc, const double* wre,
+               const double* wim, size_t nw, double* out_re, double* out_im) {
+    if (n_out == 0 || k_lo < -(long)(n / 2) || k_lo + (long)n_out - 1 > (long)(n / 2)) return -1;
+    long wl = (long)ceil(-log10(tol)) + 1;
+    size_t w = (size_t)(wl < 2 ? 2 : (wl > 16 ? 16 : wl));
+    const double beta = 2.30 * (double)w;
+    const size_t nf = 2 * n;
+    const double half = (double)w / 2.0;
+    double* corr = malloc(sizeof(double) * n_out);
+    for (size_t j = 0; j < n_out; j++) {
+        double k = (double)(k_lo + (long)j);
+        double sum = es_psi(0.0, beta);
+        for (double m = 1.0; m < half; m += 1.0) sum += 2.0 * es_psi(m / half, beta) * cos(TAU * k * m / (double)nf);
+        corr[j] = 1.0 / sum;
+    }
+    double* g = calloc(2 * nf * nw, sizeof(double));
+    double* kern = malloc(sizeof(double) * w);
+    for (size_t j = 0; j < nsrc; j++) {
+        double p = fmod(2.0 * pos[j], (double)nf);
+        if (p < 0.0) p += (double)nf;
+        long m0 = (long)floor(p - half) + 1;
+        for (size_t i = 0; i < w; i++) kern[i] = es_psi(((double)m0 + (double)i - p) / half, beta);
+        for (size_t q = 0; q < nw; q++) {
+            double wr = wre[q * nsrc + j], wi = wim[q * nsrc + j];
+            double* gq = g + 2 * nf * q;
+            for (size_t i = 0; i < w; i++) {
+                long m = (m0 + (long)i) % (long)nf;
+                if (m < 0) m += (long)nf;
+                gq[2 * m] += wr * kern[i];
+                gq[2 * m + 1] += wi * kern[i];
+            }
+        }
+    }
+    for (size_t q = 0; q < nw; q++) {
+        const double* gq = g + 2 * nf * q;
+        for (size_t j = 0; j < n_out; j++) {
+            long bin = (k_lo + (long)j) % (long)nf;
+            if (bin < 0) bin += (long)nf;
+            double ar = 0.0, ai = 0.0;
+            for (size_t m = 0; m < nf; m++) {
+                double ph = TAU * (double)(((size_t)bin * m) % nf) / (double)nf;
+                double c = cos(ph), sn = sin(ph);
+                ar += gq[2 * m] * c - gq[2 * m + 1] * sn;
+                ai += gq[2 * m] * sn + gq[2 * m + 1] * c;
+            }
+            out_re[q * n_out + j] = ar * corr[j];
+            out_im[q * n_out + j] = ai * corr[j];
+        }
+    }
+    free(corr); free(g); free(kern);
+    return 0;
+}
+
+/* The literal sum the NUFFT approximates (the `direct` of src/nufft.rs:301-315). */
+void orc_nufft_direct(size_t n, long k_lo, size_t n_out, const double* pos, size_t nsrc, const double* re,
+                      const double* im, double* out_re, double* out_im) {
+    for (size_t j = 0; j < n_out; j++) {
+        double k = (double)(k_lo + (long)j);
+        double ar = 0.0, ai = 0.0;
+        for (size_t s = 0; s < nsrc; s++) {
+            double ph = TAU * k * pos[s] / (double)n;
+            double c = cos(ph), sn = sin(ph);
+            ar += re[s] * c - im[s] * sn;
+            ai += re[s] * sn + im[s] * c;
+        }
+        out_re[j] = ar;
+        out_im[j] = ai;
+    }
+}
+
 /* ---------------------------------------------------------------- build_coil_kspace_timed (:607-1042),
  * default (std-only) build: rotor y-phase with REANCHOR = 32, per-voxel or per-line relaxation,
  * separable eddy factors, twiddle-table x-stage.  Returns 0, or -1 with the reference's assert. */
 typedef struct { size_t dky, par; double dt, dkyn; double* re; double* im; } Rotor;
 
-int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil,
-                                size_t ncoils, const mrs_line_timing* timing, const double* log_amp, C* kspace) {
+static int build_timed(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil, size_t ncoils,
+                       const mrs_line_timing* timing, const double* log_amp, int use_nufft, int* used_nufft,
+                       C* kspace) {
     const size_t snx = inp->sim[0], sny = inp->sim[1];
     const size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
     if (snx % nx != 0 || sny % ny != 0) return -1;
@@ -337,6 +417,60 @@ int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisitio
     XStage xst;
     xstage_new(&xst, snx, nx, xs, sxs, xoff, acq->ghost_offset);
 
+    /* NUFFT y-stage (kspace.rs:778-831, feature `kspace`): with t affine in the line index per parity
+     * (and the readout polarity the parity's), the per-line y-sum is one type-1 NUFFT per column of
+     * the sources displaced to v = (y - c) + sny tau rate, per (compartment, parity) weight vector. */
+    double* rows = NULL;   /* [c][re/im][x + snx*kyi] */
+    if (use_nufft && !do_eddy && ny >= 3 && uniform) {
+        const double tau = (t_ms[2] - t_ms[0]) / 2.0, t0 = t_ms[0];
+        const double delta = t_ms[1] - (tau + t0);
+        double tmax = 0.0;
+        for (size_t k = 0; k < ny; k++) tmax = fmax(tmax, fabs(t_ms[k]));
+        tmax = fmax(tmax, 1e-300);
+        int affine = 1;
+        for (size_t k = 0; k < ny && affine; k++) {
+            double pred = tau * (double)k + t0 + ((k % 2 == 1) ? delta : 0.0);
+            if (!(fabs(t_ms[k] - pred) <= 1e-9 * tmax) || timing->polarity[k] != ((k % 2 == 1) ? -1 : 1)) affine = 0;
+        }
+        if (affine) {
+            const double tau_s = tau / 1000.0, delta_s = delta / 1000.0;
+            const double t_centre = (tau * (double)ys + t0) / 1000.0;
+            rows = calloc(2 * Q * snx * ny, sizeof(double));
+            double* pos = malloc(sizeof(double) * sny);
+            double* wre = malloc(sizeof(double) * 2 * Q * sny);
+            double* wim = malloc(sizeof(double) * 2 * Q * sny);
+            double* ore = malloc(sizeof(double) * 2 * Q * ny);
+            double* oim = malloc(sizeof(double) * 2 * Q * ny);
+            for (size_t x = 0; x < snx; x++) {
+                for (size_t y = 0; y < sny; y++) {
+                    size_t i = x + snx * y;
+                    pos[y] = ((double)y - (double)sys - yoff) + (double)sny * tau_s * rate[i];
+                    double ph = phi0[i] + TAU * rate[i] * t_centre;
+                    double c0 = cos(ph), s0 = sin(ph);
+                    double phd = TAU * rate[i] * delta_s;
+                    double cd = cos(phd), sd = sin(phd);
+                    double c1 = c0 * cd - s0 * sd, s1 = c0 * sd + s0 * cd;
+                    for (size_t c = 0; c < Q; c++) {
+                        double a = amp[i] * (double)inp->compartments[c][i];
+                        wre[(2 * c) * sny + y] = a * c0;
+                        wim[(2 * c) * sny + y] = a * s0;
+                        wre[(2 * c + 1) * sny + y] = a * c1;
+                        wim[(2 * c + 1) * sny + y] = a * s1;
+                    }
+                }
+                orc_nufft1(sny, -(long)ys, ny, 1e-13, pos, sny, wre, wim, 2 * Q, ore, oim);
+                for (size_t c = 0; c < Q; c++)
+                    for (size_t kyi = 0; kyi < ny; kyi++) {
+                        size_t q = 2 * c + (kyi % 2);
+                        rows[(2 * c) * snx * ny + x + snx * kyi] = ore[q * ny + kyi];
+                        rows[(2 * c + 1) * snx * ny + x + snx * kyi] = oim[q * ny + kyi];
+                    }
+            }
+            free(pos); free(wre); free(wim); free(ore); free(oim);
+        }
+    }
+    if (used_nufft) *used_nufft = rows != NULL;
+
     for (size_t i = 0; i < nx * ny; i++) { kspace[i].re = 0.0; kspace[i].im = 0.0; }
     int have_prev = 0;
     size_t prev = 0, since_anchor = 0;
@@ -344,8 +478,8 @@ int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisitio
         if (!mask[nx * kyi]) continue;
         const double t = t_ms[kyi] / 1000.0;
         const double trf = trf_ms[kyi];
-        int anchor = 1;
-        if (have_prev && since_anchor < REANCHOR) {
+        int anchor = rows == NULL;
+        if (rows == NULL && have_prev && since_anchor < REANCHOR) {
             size_t p = prev;
             size_t dky = kyi - p, par = p % 2;
             double dt = t - t_ms[p] / 1000.0, dkyn = KYN_OF(kyi) - KYN_OF(p);
@@ -428,7 +562,14 @@ int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisitio
             ez_im = sin(ph);
         }
         for (size_t x = 0; x < snx; x++) { g_re[x] = 0.0; g_im[x] = 0.0; }
-        for (size_t y = 0; y < sny; y++) {
+        if (rows) {
+            for (size_t c = 0; c < Q; c++)
+                for (size_t x = 0; x < snx; x++) {
+                    g_re[x] += rel[c] * rows[(2 * c) * snx * ny + x + snx * kyi];
+                    g_im[x] += rel[c] * rows[(2 * c + 1) * snx * ny + x + snx * kyi];
+                }
+        }
+        for (size_t y = 0; y < sny && !rows; y++) {
             size_t row = snx * y;
             double eyr = ey_re[y], eyi = ey_im[y];
             for (size_t x = 0; x < snx; x++) {
@@ -479,11 +620,39 @@ int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisitio
     xstage_free(&xst);
     for (size_t r = 0; r < nrot; r++) { free(rotors[r].re); free(rotors[r].im); }
     free(mask); free(amp); free(phi0); free(rate); free(st_re); free(st_im);
-    free(ex_re); free(ex_im); free(ey_re); free(ey_im); free(g_re); free(g_im);
+    free(ex_re); free(ex_im); free(ey_re); free(ey_im); free(g_re); free(g_im); free(rows);
     return 0;
 #undef XC_OF
 #undef YC_OF
 #undef KYN_OF
+}
+
+/* Which build the per-slice entry points (build_coil_kspace and everything above it: simulate_slice,
+ * the GRAPPA reference scan, the series) restate: 0 the default (std-only) build, 1 the `kspace`
+ * feature build (NUFFT y-stage where its conditions hold).  Test switch; not thread-safe. */
+static int g_kspace_feature = 0;
+void orc_set_kspace_feature(int on) { g_kspace_feature = on ? 1 : 0; }
+
+/* build_coil_kspace_timed of the default (std-only) build: the rotor y-stage. */
+int orc_build_coil_kspace_timed(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil,
+                                size_t ncoils, const mrs_line_timing* timing, const double* log_amp, C* kspace) {
+    return build_timed(inp, acq, coil, ncoils, timing, log_amp, 0, NULL, kspace);
+}
+
+/* The same as the `kspace` feature builds it: the NUFFT y-stage where its conditions hold (uniform
+ * relaxation, no eddy polynomial, ny >= 3, affine per-parity timing), else the rotor y-stage;
+ * *used_nufft says which ran.  The x-stage stays the twiddle DFT (the feature's FFT x-stage is the
+ * same transform). */
+int orc_build_coil_kspace_nufft(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil, size_t ncoils,
+                                int* used_nufft, C* kspace) {
+    size_t nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    double* t = malloc(sizeof(double) * ny * 3);
+    int8_t* pol = malloc(ny);
+    orc_line_timing(acq, nx, ny, t, t + ny, t + 2 * ny, pol);
+    mrs_line_timing lt = {t, t + ny, t + 2 * ny, pol, ny};
+    int rc = build_timed(inp, acq, coil, ncoils, &lt, NULL, 1, used_nufft, kspace);
+    free(t); free(pol);
+    return rc;
 }
 
 /* build_coil_kspace (:597-600): the table from for_acquisition */
@@ -494,7 +663,7 @@ int orc_build_coil_kspace(const mrs_slice_input* inp, const mrs_acquisition* acq
     int8_t* pol = malloc(ny);
     orc_line_timing(acq, nx, ny, t, t + ny, t + 2 * ny, pol);
     mrs_line_timing lt = {t, t + ny, t + 2 * ny, pol, ny};
-    int rc = orc_build_coil_kspace_timed(inp, acq, coil, ncoils, &lt, NULL, out);
+    int rc = build_timed(inp, acq, coil, ncoils, &lt, NULL, g_kspace_feature, NULL, out);
     free(t);
     free(pol);
     return rc;

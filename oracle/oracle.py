@@ -74,6 +74,13 @@ def lib():
                                           ct.c_uint64, P(ct.c_float), P(ct.c_float), P(ct.c_float)]
         L.orc_mag_phase.argtypes = [P(ct.c_float), P(ct.c_float), ct.c_size_t, P(ct.c_float), P(ct.c_float)]
         L.orc_series_inputs.argtypes = [P(ct.c_float), P(ct.c_float), P(ct.c_float)]
+        L.orc_nufft1.argtypes = [ct.c_size_t, ct.c_long, ct.c_size_t, ct.c_double, P(ct.c_double), ct.c_size_t,
+                                 P(ct.c_double), P(ct.c_double), ct.c_size_t, P(ct.c_double), P(ct.c_double)]
+        L.orc_nufft_direct.argtypes = [ct.c_size_t, ct.c_long, ct.c_size_t, P(ct.c_double), ct.c_size_t,
+                                       P(ct.c_double), P(ct.c_double), P(ct.c_double), P(ct.c_double)]
+        L.orc_build_coil_kspace_nufft.argtypes = [P(SliceInput), P(Acquisition), ct.c_size_t, ct.c_size_t,
+                                                  P(ct.c_int), P(C64)]
+        L.orc_set_kspace_feature.argtypes = [ct.c_int]
         _lib = L
     return _lib
 
@@ -163,6 +170,54 @@ def mag_phase(re, im):
     mag, ph = np.zeros_like(re), np.zeros_like(re)
     lib().orc_mag_phase(abi.fptr(re), abi.fptr(im), len(re), abi.fptr(mag), abi.fptr(ph))
     return mag, ph
+
+
+def nufft1(n, k_lo, n_out, tol, pos, weights):
+    """Nufft1::new(n, k_lo, n_out, tol).run(pos, weights) (src/nufft.rs:25-113): weights is a list of
+    complex vectors over the sources; returns a list of complex n_out vectors."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    w = np.ascontiguousarray(np.stack([np.asarray(x) for x in weights]), dtype=np.complex128)
+    wre, wim = np.ascontiguousarray(w.real), np.ascontiguousarray(w.imag)
+    ore, oim = np.zeros((len(weights), n_out)), np.zeros((len(weights), n_out))
+    rc = lib().orc_nufft1(n, k_lo, n_out, tol, abi.dptr(pos), len(pos), abi.dptr(wre), abi.dptr(wim), len(weights),
+                          abi.dptr(ore), abi.dptr(oim))
+    if rc:
+        raise ValueError("output band must sit inside the coarse Nyquist band")
+    return list(ore + 1j * oim)
+
+
+def nufft_direct(n, k_lo, n_out, pos, weight):
+    """The literal sum a type-1 NUFFT approximates (src/nufft.rs:301-315)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    w = np.asarray(weight, dtype=np.complex128)
+    re, im = np.ascontiguousarray(w.real), np.ascontiguousarray(w.imag)
+    ore, oim = np.zeros(n_out), np.zeros(n_out)
+    lib().orc_nufft_direct(n, k_lo, n_out, abi.dptr(pos), len(pos), abi.dptr(re), abi.dptr(im), abi.dptr(ore),
+                           abi.dptr(oim))
+    return ore + 1j * oim
+
+
+def coil_kspace_nufft(b, acq, coil=0, ncoils=1):
+    """build_coil_kspace as the `kspace` feature builds it: (k-space, whether the NUFFT y-stage ran)."""
+    nx, ny = b.acq_matrix
+    out = np.zeros(nx * ny, dtype=np.complex128)
+    used = ct.c_int(0)
+    rc = lib().orc_build_coil_kspace_nufft(ct.byref(b.inp), ct.byref(acq), coil, ncoils, ct.byref(used), _c64(out))
+    if rc != 0:
+        raise ValueError("oracle refused the input")
+    return out, bool(used.value)
+
+
+class kspace_feature:
+    """Context manager: the per-slice entry points restate the `kspace` feature build inside it."""
+
+    def __enter__(self):
+        lib().orc_set_kspace_feature(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_kspace_feature(0)
+        return False
 
 
 SERIES_DIMS = dict(nx=16, ny=16, nz=3, o=2, nv=4)
