@@ -359,6 +359,24 @@ int mrs_series_epilogue(const mrs_c32* images, size_t n_volumes, size_t nvox, co
                         uint64_t receiver_seed, int32_t mode, float* out0, float* out1, void* stream, char* err,
                         size_t errlen);
 
+/* Nufft1 (src/nufft.rs:25-113) on the GPU in FP64: the type-1 NUFFT
+ * F[k] = sum_j w_j exp(+i 2 pi k v_j / n), k = k_lo .. k_lo + n_out - 1, by gridding with the
+ * exponential-of-semicircle kernel at oversampling 2 (width from `tol` as the reference chooses it).
+ * Device buffers: pos [pos_cols][nsrc] cells (column c uses pos[c % pos_cols]); weights
+ * [ncols][nw][nsrc]; out [ncols][nw][n_out].  The output band must sit inside [-n/2, n/2]; the fine
+ * grids of a column (nw * 2n complex f64) must fit in 160 KiB of LDS.  Waits for `stream`. */
+int mrs_nufft1(const double* pos, size_t pos_cols, const mrs_c64* weights, size_t nsrc, size_t nw, size_t ncols,
+               size_t n, long k_lo, size_t n_out, double tol, mrs_c64* out, void* stream, char* err, size_t errlen);
+
+/* build_coil_kspaces (src/kspace.rs:1248-1251) as the reference's `kspace` feature build computes it
+ * where its NUFFT y-stage applies (src/kspace.rs:778-831): uniform relaxation, no eddy polynomial,
+ * ny >= 3, line times affine per parity with the parity's readout polarity; other inputs are refused
+ * with the reason.  FP64 on the GPU: the NUFFT y-stage (tolerance 1e-13), the per-line relaxation
+ * combine, the twiddle x-stage, spikes and noise.  Host buffers as mrs_build_coil_kspaces; the
+ * comparator of the exact forward. */
+int mrs_build_coil_kspaces_nufft(const mrs_slice_input* inp, const mrs_acquisition* acq, mrs_c64* out, char* err,
+                                 size_t errlen);
+
 /* Size of a public struct as compiled into the library (ABI check for bindings). */
 size_t mrs_sizeof(const char* name);
 

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(ROOT, "build")
 LIB = os.path.join(OUT_DIR, "libmrsim_gpu.so")
-SOURCES = ["kernels.hip", "plan.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip"]
+SOURCES = ["kernels.hip", "plan.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip", "nufft.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -95,6 +95,9 @@ def lib():
     L.mrs_plan_set_zstage.argtypes = [ct.c_void_p, P(abi.ZSetDesc), ct.c_size_t] + E
     L.mrs_series_epilogue.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_void_p, ct.c_uint64, ct.c_int32,
                                       ct.c_void_p, ct.c_void_p, ct.c_void_p] + E
+    L.mrs_nufft1.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                             ct.c_long, ct.c_size_t, ct.c_double, ct.c_void_p, ct.c_void_p] + E
+    L.mrs_build_coil_kspaces_nufft.argtypes = [P(abi.SliceInput), P(abi.Acquisition), P(abi.C64)] + E
     _lib = L
     return L
 
@@ -303,6 +306,36 @@ def series_epilogue(images, noise_sigma, receiver_seed: int, magnitude_phase: bo
                                      1 if magnitude_phase else 0, o0.data_ptr(), o1.data_ptr(),
                                      ct.c_void_p(stream.cuda_stream), err, len(err)), err)
     return o0, o1
+
+
+def nufft1(pos, weights, n: int, k_lo: int, n_out: int, tol: float, stream=None):
+    """mrs_nufft1 (Nufft1, src/nufft.rs:25-113) in FP64: pos (pos_cols, nsrc) float64 and weights
+    (ncols, nw, nsrc) complex128 device tensors -> (ncols, nw, n_out) complex128."""
+    import torch
+    pos_cols, nsrc = pos.shape
+    ncols, nw, nsrc_w = weights.shape
+    if nsrc_w != nsrc:
+        raise ValueError("weights and positions disagree on the number of sources")
+    _check_dev("pos", pos, torch.float64, pos_cols * nsrc)
+    _check_dev("weights", weights, torch.complex128, ncols * nw * nsrc)
+    out = torch.empty((ncols, nw, n_out), dtype=torch.complex128, device=weights.device)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_nufft1(pos.data_ptr(), pos_cols, weights.data_ptr(), nsrc, nw, ncols, n, k_lo, n_out, tol,
+                            out.data_ptr(), ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+    return out
+
+
+def build_coil_kspaces_nufft(b: abi.SliceInputBuilder, acq: abi.Acquisition) -> np.ndarray:
+    """mrs_build_coil_kspaces_nufft: every coil's k-space as the `kspace` feature build computes it
+    (NUFFT y-stage, FP64), (n_coils, nx*ny) complex128."""
+    nx, ny = b.acq_matrix
+    nc = max(int(acq.n_coils), 1)
+    out = np.zeros(nc * nx * ny, dtype=np.complex128)
+    err = _errbuf()
+    _check(lib().mrs_build_coil_kspaces_nufft(ct.byref(b.inp), ct.byref(acq), out.ctypes.data_as(ct.POINTER(abi.C64)),
+                                              err, len(err)), err)
+    return out.reshape(nc, nx * ny)
 
 
 def simulate_slice(b: abi.SliceInputBuilder, acq: abi.Acquisition) -> np.ndarray:

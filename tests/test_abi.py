@@ -168,3 +168,29 @@ def test_recon_refusals_before_any_device_work(L):
     assert L.mrs_recon_create(16, 16, 3, ct.byref(acq), None, ct.byref(h), err, 256) == -1
     assert b"one k-space per coil" in err.value
     assert L.mrs_recon_create(200, 16, 4, ct.byref(acq), None, ct.byref(h), err, 256) == -4
+
+
+def test_nufft_refusals_before_any_device_work(L):
+    # the NUFFT comparator's domain (src/kspace.rs:775-792) and the NUFFT's band (src/nufft.rs:41-42)
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(1)
+    nx, ny, o = O.GOLDEN_SHAPES[1]
+    err = ct.create_string_buffer(256)
+    out = np.zeros(4 * nx * ny, dtype=np.complex128)
+    P = ct.POINTER(abi.C64)
+
+    def run(b, acq):
+        return L.mrs_build_coil_kspaces_nufft(ct.byref(b.inp), ct.byref(acq), out.ctypes.data_as(P), err, 256)
+
+    acq = abi.Acquisition.default(n_coils=4)
+    mapped = abi.SliceInputBuilder(list(comps), [t2m, 100.0, t2m], fmap, (nx * o, ny * o), (nx, ny))
+    assert run(mapped, acq) == -1 and b"uniform relaxation" in err.value
+    eddy = abi.SliceInputBuilder(list(comps), [70.0, 100.0, 2000.0], fmap, (nx * o, ny * o), (nx, ny),
+                                 eddy_drive=(0.3, -0.8, 0.5))
+    assert run(eddy, acq.copy(eddy_strength=3.0)) == -1 and b"eddy polynomial" in err.value
+    thin = abi.SliceInputBuilder([np.ones(16 * 2, dtype=np.float32)], [70.0], np.zeros(32, dtype=np.float32), (16, 2),
+                                 (16, 2))
+    assert run(thin, abi.Acquisition.default()) == -1 and b"at least 3 phase-encode lines" in err.value
+    x = np.zeros(64)
+    assert L.mrs_nufft1(x.ctypes.data, 1, x.ctypes.data, 4, 1, 1, 16, -9, 16, 1e-12, x.ctypes.data, None, err,
+                        256) == -1
+    assert b"Nyquist" in err.value
