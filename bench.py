@@ -71,6 +71,41 @@ def _cpu_unit(i):
     return i
 
 
+def nufft_pass(iters: int):
+    """The NUFFT comparator (uniform relaxation: the reference's `kspace` feature y-stage) on one slice,
+    128^2 -> 64^2, 8 coils, 3 compartments of uniform T2: the FP64 NUFFT y-stage kernel alone (8 coils
+    x 128 columns of 128 displaced sources, 6 weight vectors each, 64 outputs) and the two host-buffer
+    drop-ins, the NUFFT forward (FP64) and the exact forward (split-BF16 plan), on the same slice."""
+    import torch
+
+    from mrsim_amd import abi
+    from mrsim_amd import gpu as G
+    from mrsim_amd import workloads
+    w = workloads.make("batch_128", n_slices=1, n_volumes=1)
+    sd = w.slices[0]
+    acq = w.acq.copy()
+    b = abi.SliceInputBuilder([im for im in w.units[0][1]], [80.0, 45.0, 120.0], sd.fmap, (w.snx, w.sny),
+                              (w.nx, w.ny), phase0=sd.phase0)
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n * 1e3
+    ncols, nw, nsrc = w.n_coils * w.snx, 2 * w.n_comp, w.sny
+    pos = torch.randn((w.snx, nsrc), dtype=torch.float64, device="cuda") + torch.arange(nsrc, device="cuda")
+    wts = torch.randn((ncols, nw, nsrc), dtype=torch.complex128, device="cuda")
+    ms_kernel = timed(lambda: G.nufft1(pos, wts, nsrc, -(w.ny // 2), w.ny, 1e-13), iters)
+    ms_nufft = timed(lambda: G.build_coil_kspaces_nufft(b, acq), max(3, iters // 4))
+    ms_exact = timed(lambda: G.build_coil_kspaces(b, acq), max(3, iters // 4))
+    return {"workload": f"1 slice {w.snx}^2->{w.nx}^2, {w.n_coils} coils, {w.n_comp} compartments, uniform T2",
+            "nufft_ystage_kernel_ms": round(ms_kernel, 4), "nufft_forward_dropin_ms": round(ms_nufft, 3),
+            "exact_forward_dropin_ms": round(ms_exact, 3),
+            "note": "FP64 NUFFT (tol 1e-13, width 14) vs split-BF16 exact; drop-ins include host transfers"}
+
+
 def grad_pass(iters: int):
     """One slice, 128^2 -> 64^2, 8 coils, 3 compartments, complex64 images: forward time, forward +
     adjoint (gradients for images, fieldmap and T2') time, and a timed 50-iteration Adam fit of the
@@ -248,6 +283,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-nufft", action="store_true")
     ap.add_argument("--no-ge3d", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--traffic-tiles", type=str, default="8,4,2", help="tiles of the traffic passes")
@@ -368,6 +404,9 @@ def main():
     log(f"reconstruction: {recon}")
 
     grad = None if args.no_grad else grad_pass(max(5, args.steps))
+    nufft = None if args.no_nufft else nufft_pass(max(5, args.steps))
+    if nufft:
+        log(f"NUFFT comparator: {nufft}")
     if grad:
         log(f"gradient pass: {grad}")
     ge3d = None if args.no_ge3d else ge3d_pass(max(3, args.steps // 4), check=args.check_units > 0,
@@ -441,6 +480,7 @@ def main():
         "stage_ms": stage_ms,
         "tiles": plan.tiles(),
         "grad_pass": grad,
+        "nufft_comparator": nufft,
         "reconstruction": recon,
         "ge3d": ge3d,
         "roofline": roofline,
