@@ -296,38 +296,13 @@ __device__ __forceinline__ void phase_decay(const Vox4<Q, CPLX, EDDY, V>& v, int
     for (int q = 0; q < Q; q++) dq[q] = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
 }
 
-// E of a single unit at voxel j, the decay folded into the compartment sum as it is evaluated.
-template <int Q, bool CPLX, bool EDDY>
-__device__ __forceinline__ void gen_e(const Vox4<Q, CPLX, EDDY, 1>& v, int j, double yy, double t_s, double kyn,
-                                      double d, float trf, float tau, float& er, float& ei) {
-    double cyc = __builtin_fma(kyn, yy, v.base[j]);
-    cyc = __builtin_fma((double)v.rate[j], t_s, cyc);
-    if constexpr (EDDY) cyc = __builtin_fma(v.poly[j], d, cyc);
-    float sn, cs;
-    sincos_turns(cyc, sn, cs);
-    float wr = 0.0f, wi = 0.0f;
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, v.nr2[q][j], tau * v.nri[q][j]));
-        wr = __builtin_fmaf(v.cr[0][q][j], dq, wr);
-        if constexpr (CPLX) wi = __builtin_fmaf(v.ci[0][q][j], dq, wi);
-    }
-    if constexpr (CPLX) {
-        er = wr * cs - wi * sn;
-        ei = wr * sn + wi * cs;
-    } else {
-        er = wr * cs;
-        ei = wr * sn;
-    }
-}
-
 // E of unit vv at voxel j: (sum_q m_q dq_q) (cs + i sn).
 template <int Q, bool CPLX, bool EDDY, int V>
 __device__ __forceinline__ void unit_e(const Vox4<Q, CPLX, EDDY, V>& v, int vv, int j, float sn, float cs,
                                        const float (&dq)[Q], float& er, float& ei) {
-    float wr = 0.0f, wi = 0.0f;
+    float wr = v.cr[vv][0][j] * dq[0], wi = CPLX ? v.ci[vv][0][j] * dq[0] : 0.0f;
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
+    for (int q = 1; q < Q; q++) {
         wr = __builtin_fmaf(v.cr[vv][q][j], dq[q], wr);
         if constexpr (CPLX) wi = __builtin_fmaf(v.ci[vv][q][j], dq[q], wi);
     }
@@ -423,7 +398,7 @@ __device__ __forceinline__ void wait_vm() {
 // for unit sets, else 3.
 template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
 constexpr int ystage_waves() {
-    if constexpr (V > 1) return (V == 2 && (CPLX ? 2 * Q : Q) <= 2) ? 3 : 2;
+    if constexpr (V > 1) return (V == 2 && (CPLX ? 2 * Q : Q) <= 3) ? 3 : 2;
     return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : ((RB * GP >= 8 || RB >= 4) ? 2 : 3);
 }
 
@@ -463,7 +438,10 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
     const int part = ln >> 3;
     const double sys_yoff = gr.sys_yoff;
     // The wave's stages: columns x = tile*xt + wave + 4k, passes of GP line tiles, 16-row chunks.
-    using L = YStage<Q, RB, CPLX, EDDY, V, GP == 4 && !EDDY>;
+    constexpr bool CHAINS = GP == 4 && !EDDY;
+    constexpr bool PAIRED = RB % 2 == 0;
+    constexpr bool ECHAIN = CHAINS && V == 1 && Q == 1 && !CPLX;
+    using L = YStage<Q, RB, CPLX, EDDY, V, CHAINS>;
     constexpr bool img_late = V >= 4;
     __shared__ __attribute__((aligned(16))) uint32_t ybuf[4][2][L::WORDS];
     const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -571,96 +549,78 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                     // them is live next to the four units' accumulators.
                     if constexpr (!img_late) read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
                 }
+                // A operands.  16 coils and more (RB even): tile 2k holds the real rows [S_re, -S_im]
+                // and tile 2k+1 the imaginary rows [S_im, S_re] of coils 16k..16k+15 (lane row =
+                // coil), both formed from one read at one op per dword, and the two tiles'
+                // accumulators are (re, im) of the same coils.  Up to 8 coils: one tile of rows
+                // (coil, part), part = bit 3 of the lane.
                 u32x4 ahi[RB], alo[RB];
 #pragma unroll
-                for (int rb = 0; rb < RB; rb++) {
+                for (int rp = 0; rp < (PAIRED ? RB / 2 : RB); rp++) {
+                    const int coil = PAIRED ? rp * 16 + ln : rp * 8 + (ln & 7);
                     u32x4 h, l;
                     if constexpr (L::SA) {
-                        const int so = ((rb * 8 + (ln & 7)) * 16 + lq * 4);
+                        const int so = coil * 16 + lq * 4;
                         h = *reinterpret_cast<const u32x4*>(B + L::O_SHI * 4 + so);
                         l = *reinterpret_cast<const u32x4*>(B + L::O_SLO * 4 + so);
                     } else {
                         // split maps are padded to RB*8 coils with zeros: no range check
-                        const uint32_t so = (uint32_t)((x * (RB * 8) + rb * 8 + (ln & 7)) * snyP + y0);
+                        const uint32_t so = (uint32_t)((x * (RB * 8) + coil) * snyP + y0);
                         h = *at32<u32x4>(sl.s_hi, so);
                         l = *at32<u32x4>(sl.s_lo, so);
                     }
-                    ahi[rb] = rows_of(h, part);
-                    alo[rb] = rows_of(l, part);
+                    if constexpr (PAIRED) {
+                        ahi[2 * rp] = rows_of(h, 0);
+                        alo[2 * rp] = rows_of(l, 0);
+                        ahi[2 * rp + 1] = rows_of(h, 1);
+                        alo[2 * rp + 1] = rows_of(l, 1);
+                    } else {
+                        ahi[rp] = rows_of(h, part);
+                        alo[rp] = rows_of(l, part);
+                    }
                 }
-                bool chained = false;
                 const double yy0 = (double)y0 - sys_yoff;
-                if constexpr (GP == 4 && !EDDY) {
-                    if ((g0 >> 2) < gr.rotor_passes) {
-                        // The lane's four lines form an arithmetic progression in (t, kyn, trf, tau) with
-                        // the group's shared steps: evaluate line 0 directly, then advance the phase by
-                        // the voxel's rotor cis(2 pi (rate dt + dkyn yy)) and each decay by the voxel's
-                        // exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.
-                        const LineRec& L0 = lines_s[g0 * 16 + ln];
-                        const f32x4 srot0 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8);
-                        const f32x4 srot1 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8 + 4);
-                        f32x4 srq[Q];
+                // Chain passes: the lane's four lines form an arithmetic progression in (t, kyn, trf,
+                // tau) with the group's shared steps.  Line 0 is evaluated directly; lines 1..3
+                // advance the phase by the voxel's rotor cis(2 pi (rate dt + dkyn yy)) and each decay
+                // by the voxel's exp2(dtrf nr2 + dtau nri), both precomputed by the slice prep.  With
+                // one compartment of real images (ECHAIN) the chain carries E itself, stepped by the
+                // rotor times the decay step.
+                const bool chain = CHAINS && (g0 >> 2) < gr.rotor_passes;
+                f32x4 srot0, srot1, srq[Q];
+                auto chain_steps = [&](int cls) {
+                    srot0 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8);
+                    srot1 = *reinterpret_cast<const f32x4*>(B + L::O_ROT * 4 + lq * 8 + 4);
 #pragma unroll
-                        for (int q = 0; q < Q; q++)
-                            srq[q] = *reinterpret_cast<const f32x4*>(B + L::O_RQ * 4 + (L0.cls * Q + q) * 16 + lq * 4);
-                        if constexpr (V == 1) {
-                            u32x4 bh[4], bl[4];
+                    for (int q = 0; q < Q; q++)
+                        srq[q] = *reinterpret_cast<const f32x4*>(B + L::O_RQ * 4 + (cls * Q + q) * 16 + lq * 4);
+                };
+                // One loop for both kinds of pass where the accumulators are many (paired tiles) or the
+                // chain state is small (ECHAIN); otherwise separate loops, which keep fewer operands live.
+                if constexpr (V == 1 && (PAIRED || ECHAIN)) {
+                    if (CHAINS && chain) chain_steps(lines_s[g0 * 16 + ln].cls);
+                    float ps[4], pc[4], dq[4][Q];      // phase and decays of the current line
+                    float zr[4], zi[4], sr[4], si[4];  // ECHAIN: E of the current line and its step
 #pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                const double yy = yy0 + (double)j;
-                                double c0 = __builtin_fma(L0.kyn, yy, v.base[j]);
-                                c0 = __builtin_fma((double)v.rate[j], L0.t_s, c0);
-                                float ps, pc;
-                                sincos_turns(c0, ps, pc);
-                                const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
-                                const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
-                                float dq[Q], rq[Q];
+                    for (int g = 0; g < GP; g++) {
+                        // Every tile runs the same MFMA block (zero B past the last tile), so the
+                        // accumulators stay in place whichever way B was generated.
+                        u32x4 bh[V], bl[V];
 #pragma unroll
-                                for (int q = 0; q < Q; q++) {
-                                    dq[q] = __builtin_amdgcn_exp2f(
-                                        __builtin_fmaf(L0.trf, v.nr2[q][j], L0.tau * v.nri[q][j]));
-                                    rq[q] = srq[q][j];
-                                }
-#pragma unroll
-                                for (int g = 0; g < 4; g++) {
-                                    if (g) {
-                                        const float npc = pc * rc - ps * rs;
-                                        ps = __builtin_fmaf(pc, rs, ps * rc);
-                                        pc = npc;
-#pragma unroll
-                                        for (int q = 0; q < Q; q++) dq[q] *= rq[q];
-                                    }
-                                    float er, ei;
-                                    unit_e(v, 0, j, ps, pc, dq, er, ei);
-                                    uint32_t h, l;
-                                    split2(er, ei, h, l);
-                                    bh[g][j] = h;
-                                    bl[g][j] = l;
-                                }
+                        for (int vv = 0; vv < V; vv++) bh[vv] = bl[vv] = u32x4{0u, 0u, 0u, 0u};
+                        if (g0 + g < G) {
+                            if constexpr (img_late) {
+                                asm volatile("" ::: "memory");   // keep the reads in this line's iteration
+                                read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
                             }
-#pragma unroll
-                            for (int g = 0; g < 4; g++)
-#pragma unroll
-                                for (int rb = 0; rb < RB; rb++)
-                                    acc[0][rb][g] = mfma3(ahi[rb], alo[rb], bh[g], bl[g], acc[0][rb][g]);
-                        } else {
-                            // line by line across the lane's four voxels, so only one line's B
-                            // operands per unit are live at a time
-                            float ps[4], pc[4], dq[4][Q];
-#pragma unroll
-                            for (int j = 0; j < 4; j++)
-                                phase_decay(v, j, yy0 + (double)j, L0.t_s, L0.kyn, 0.0, L0.trf, L0.tau, ps[j], pc[j],
-                                            dq[j]);
-#pragma unroll
-                            for (int g = 0; g < 4; g++) {
-                                if constexpr (img_late) {
-                                    asm volatile("" ::: "memory");   // keep the reads in this line's iteration
-                                    read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
-                                }
-                                u32x4 bh[V], bl[V];
+                            if (CHAINS && g > 0 && chain) {
 #pragma unroll
                                 for (int j = 0; j < 4; j++) {
-                                    if (g) {
+                                    if constexpr (ECHAIN) {
+                                        const float nr = zr[j] * sr[j] - zi[j] * si[j];
+                                        zi[j] = __builtin_fmaf(zr[j], si[j], zi[j] * sr[j]);
+                                        zr[j] = nr;
+                                    } else {
                                         const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
                                         const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
                                         const float npc = pc[j] * rc - ps[j] * rs;
@@ -672,9 +632,127 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 #pragma unroll
                                     for (int vv = 0; vv < V; vv++) {
                                         float er, ei;
-                                        unit_e(v, vv, j, ps[j], pc[j], dq[j], er, ei);
+                                        if constexpr (ECHAIN) {
+                                            er = zr[j];
+                                            ei = zi[j];
+                                        } else {
+                                            unit_e(v, vv, j, ps[j], pc[j], dq[j], er, ei);
+                                        }
                                         uint32_t h, l;
                                         split2(er, ei, h, l);
+                                        bh[vv][j] = h;
+                                        bl[vv][j] = l;
+                                    }
+                                }
+                            } else {
+                                const LineRec& Lr = lines_s[(g0 + g) * 16 + ln];
+                                const bool ok = Lr.ky >= 0;
+                                const double t_s = Lr.t_s, kyn = Lr.kyn, d = Lr.d;
+                                const float trf = Lr.trf, tau = Lr.tau;
+#pragma unroll
+                                for (int j = 0; j < 4; j++) {
+                                    phase_decay(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, ps[j], pc[j], dq[j]);
+                                    if constexpr (ECHAIN) {
+                                        const float w = v.cr[0][0][j] * dq[j][0];
+                                        zr[j] = w * pc[j];
+                                        zi[j] = w * ps[j];
+                                        if constexpr (CHAINS) {
+                                            if (g == 0 && chain) {
+                                                const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
+                                                const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
+                                                sr[j] = rc * srq[0][j];
+                                                si[j] = rs * srq[0][j];
+                                            }
+                                        }
+                                    }
+#pragma unroll
+                                    for (int vv = 0; vv < V; vv++) {
+                                        float er, ei;
+                                        if constexpr (ECHAIN) {
+                                            er = zr[j];
+                                            ei = zi[j];
+                                        } else {
+                                            unit_e(v, vv, j, ps[j], pc[j], dq[j], er, ei);
+                                        }
+                                        uint32_t h, l;
+                                        split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
+                                        bh[vv][j] = h;
+                                        bl[vv][j] = l;
+                                    }
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int vv = 0; vv < V; vv++)
+#pragma unroll
+                            for (int rb = 0; rb < RB; rb++)
+                                acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
+                    }
+                } else {
+                    if (chain) {
+                        const LineRec& L0 = lines_s[g0 * 16 + ln];
+                        chain_steps(L0.cls);   // loaded here: live only on this path
+                        float ps[4], pc[4], dq[4][Q];
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            phase_decay(v, j, yy0 + (double)j, L0.t_s, L0.kyn, 0.0, L0.trf, L0.tau, ps[j], pc[j],
+                                        dq[j]);
+#pragma unroll
+                        for (int g = 0; g < GP; g++) {
+                            if constexpr (img_late) {
+                                asm volatile("" ::: "memory");   // keep the reads in this line's iteration
+                                read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
+                            }
+                            u32x4 bh[V], bl[V];
+#pragma unroll
+                            for (int j = 0; j < 4; j++) {
+                                if (g) {
+                                    const float rc = (j < 2 ? srot0 : srot1)[2 * (j & 1)];
+                                    const float rs = (j < 2 ? srot0 : srot1)[2 * (j & 1) + 1];
+                                    const float npc = pc[j] * rc - ps[j] * rs;
+                                    ps[j] = __builtin_fmaf(pc[j], rs, ps[j] * rc);
+                                    pc[j] = npc;
+#pragma unroll
+                                    for (int q = 0; q < Q; q++) dq[j][q] *= srq[q][j];
+                                }
+#pragma unroll
+                                for (int vv = 0; vv < V; vv++) {
+                                    float er, ei;
+                                    unit_e(v, vv, j, ps[j], pc[j], dq[j], er, ei);
+                                    uint32_t h, l;
+                                    split2(er, ei, h, l);
+                                    bh[vv][j] = h;
+                                    bl[vv][j] = l;
+                                }
+                            }
+#pragma unroll
+                            for (int vv = 0; vv < V; vv++)
+#pragma unroll
+                                for (int rb = 0; rb < RB; rb++)
+                                    acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
+                        }
+                    } else {
+#pragma unroll
+                        for (int g = 0; g < GP; g++) {
+                            if (g0 + g < G) {
+                                const LineRec& Lr = lines_s[(g0 + g) * 16 + ln];
+                                const bool ok = Lr.ky >= 0;
+                                if constexpr (img_late) {
+                                    asm volatile("" ::: "memory");
+                                    read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
+                                }
+                                u32x4 bh[V], bl[V];
+#pragma unroll
+                                for (int j = 0; j < 4; j++) {
+                                    float sn, cs, dqj[Q];
+                                    phase_decay(v, j, yy0 + (double)j, Lr.t_s, Lr.kyn, Lr.d, Lr.trf, Lr.tau, sn, cs,
+                                                dqj);
+#pragma unroll
+                                    for (int vv = 0; vv < V; vv++) {
+                                        float er, ei;
+                                        unit_e(v, vv, j, sn, cs, dqj, er, ei);
+                                        uint32_t h, l;
+                                        split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
                                         bh[vv][j] = h;
                                         bl[vv][j] = l;
                                     }
@@ -686,74 +764,47 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                                         acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
                             }
                         }
-                        chained = true;
-                    }
-                }
-                if (!chained) {
-    #pragma unroll
-                    for (int g = 0; g < GP; g++) {
-                        if (g0 + g < G) {
-                            const LineRec& L = lines_s[(g0 + g) * 16 + ln];
-                            const bool ok = L.ky >= 0;
-                            const double t_s = L.t_s, kyn = L.kyn, d = L.d;
-                            const float trf = L.trf, tau = L.tau;
-                            if constexpr (img_late) {
-                                asm volatile("" ::: "memory");
-                                read_images(v, B + L::O_IMG * 4, L::IMG_DW, lq);
-                            }
-                            u32x4 bhi[V], blo[V];
-    #pragma unroll
-                            for (int j = 0; j < 4; j++) {
-                                if constexpr (V == 1) {
-                                    float er, ei;
-                                    gen_e<Q, CPLX, EDDY>(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, er, ei);
-                                    uint32_t h, l;
-                                    split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
-                                    bhi[0][j] = h;
-                                    blo[0][j] = l;
-                                } else {
-                                    float sn, cs, dq[Q];
-                                    phase_decay(v, j, yy0 + (double)j, t_s, kyn, d, trf, tau, sn, cs, dq);
-    #pragma unroll
-                                    for (int vv = 0; vv < V; vv++) {
-                                        float er, ei;
-                                        unit_e(v, vv, j, sn, cs, dq, er, ei);
-                                        uint32_t h, l;
-                                        split2(ok ? er : 0.0f, ok ? ei : 0.0f, h, l);
-                                        bhi[vv][j] = h;
-                                        blo[vv][j] = l;
-                                    }
-                                }
-                            }
-    #pragma unroll
-                            for (int vv = 0; vv < V; vv++)
-    #pragma unroll
-                                for (int rb = 0; rb < RB; rb++)
-                                    acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bhi[vv], blo[vv], acc[vv][rb][g]);
-                        }
                     }
                 }
         }
         if (sch == nch - 1) {
-        // acc[vv][rb][g]: lane holds rows 4*lq..4*lq+3 (part lq/2, coils rb*8 + 4*(lq&1) + r), line 16*g + ln
+        // acc[vv][rb][g], line 16*g + ln: lane holds rows 4*lq..4*lq+3 of the tile: paired tiles,
+        // coils 16*(rb/2) + 4*lq + r (rb even: real parts, odd: imaginary parts); else part lq/2 of
+        // coils rb*8 + 4*(lq&1) + r.
 #pragma unroll
         for (int g = 0; g < GP; g++) {
             if (g0 + g < G) {
                 const int lp = (g0 + g) * 16 + ln;
 #pragma unroll
-                for (int vv = 0; vv < V; vv++)
+                for (int vv = 0; vv < V; vv++) {
+                    GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C);
+                    if constexpr (PAIRED) {
 #pragma unroll
-                    for (int rb = 0; rb < RB; rb++) {
+                        for (int cp = 0; cp < RB / 2; cp++) {
+                            const int c0 = cp * 16 + 4 * lq;
+                            const f32x4 re = acc[vv][2 * cp][g], im = acc[vv][2 * cp + 1][g];
+                            if (c0 + 4 <= C && (C & 1) == 0) {
+                                // 16-byte aligned: the buffer is, and (x*LyP + lp)*C + c0 is even
+                                GAS f32x4* d4 = reinterpret_cast<GAS f32x4*>(row + c0);
+                                d4[0] = f32x4{re[0], im[0], re[1], im[1]};
+                                d4[1] = f32x4{re[2], im[2], re[3], im[3]};
+                            } else {
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int c = rb * 8 + 4 * (lq & 1) + r;
-                            if (c < C) {
-                                GAS float* dst =
-                                    reinterpret_cast<GAS float*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C + c);
-                                dst[lq >> 1] = acc[vv][rb][g][r];
+                                for (int r = 0; r < 4; r++)
+                                    if (c0 + r < C) row[c0 + r] = f32x2{re[r], im[r]};
+                            }
+                        }
+                    } else {
+#pragma unroll
+                        for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int c = rb * 8 + 4 * (lq & 1) + r;
+                                if (c < C) reinterpret_cast<GAS float*>(row + c)[lq >> 1] = acc[vv][rb][g][r];
                             }
                         }
                     }
+                }
             }
         }
         }
