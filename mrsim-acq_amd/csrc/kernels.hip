@@ -407,7 +407,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ystage_wave
 void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
                  const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work,
                  const int32_t* __restrict__ usets, int xt) {
-    __shared__ LineRec lines_s[256];
+    constexpr bool CHAINS = GP == 4 && !EDDY;
+    constexpr bool PAIRED = RB % 2 == 0;
+    constexpr bool ECHAIN = CHAINS && V == 1 && Q == 1 && !CPLX;
+    using L = YStage<Q, RB, CPLX, EDDY, V, CHAINS>;
+    // All of the block's LDS in one object (per-wave double buffers of the stage operands, then the
+    // line table): with a single LDS variable the compiler keeps the alias scopes that let the
+    // stage loop read one buffer while the copy into the other is in flight.
+    struct Smem {
+        uint32_t ybuf[4][2][L::WORDS];
+        LineRec lines[256];
+    };
+    __shared__ __attribute__((aligned(16))) Smem sm;
+    auto& ybuf = sm.ybuf;
+    LineRec* const lines_s = sm.lines;
     const WorkItem wi = work[blockIdx.x];
     int uid[V];
     if constexpr (V == 1) {
@@ -438,12 +451,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
     const int part = ln >> 3;
     const double sys_yoff = gr.sys_yoff;
     // The wave's stages: columns x = tile*xt + wave + 4k, passes of GP line tiles, 16-row chunks.
-    constexpr bool CHAINS = GP == 4 && !EDDY;
-    constexpr bool PAIRED = RB % 2 == 0;
-    constexpr bool ECHAIN = CHAINS && V == 1 && Q == 1 && !CPLX;
-    using L = YStage<Q, RB, CPLX, EDDY, V, CHAINS>;
     constexpr bool img_late = V >= 4;
-    __shared__ __attribute__((aligned(16))) uint32_t ybuf[4][2][L::WORDS];
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int x0w = wi.tile * xt + wv;
     const int nxw = x0w < snx ? min((xt - wv + 3) / 4, (snx - x0w + 3) / 4) : 0;
@@ -499,13 +507,17 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
     f32x4 acc[V][RB][GP];
     int sxw = 0, spi = 0, sch = 0;            // the stage being evaluated
     if (nstage > 0) stage_copy(x0w, 0, &ybuf[wv][0][0]);
-    for (int st = 0; st < nstage; st++) {
+    // One stage: evaluate from buffer B while the next stage's copy lands in Bn.  The buffers and
+    // the line table are restrict parameters, so the compiler knows that the reads of B and of the
+    // line table do not depend on the copy in flight and waits only as far as wait_vm asks.
+    auto run_stage = [&](int st, const uint32_t* __restrict__ B, uint32_t* __restrict__ Bn,
+                         const LineRec* __restrict__ lines) {
         const int x = x0w + 4 * sxw, g0 = spi * GP, ch = sch * 16;
         // next stage's copy into the other buffer, then wait for this stage's
         int nxw_ = sxw, npi = spi, nchi = sch + 1;
         if (nchi == nch) { nchi = 0; if (++npi == npass) { npi = 0; ++nxw_; } }
         if (st + 1 < nstage) {
-            stage_copy(x0w + 4 * nxw_, nchi * 16, &ybuf[wv][(st + 1) & 1][0]);
+            stage_copy(x0w + 4 * nxw_, nchi * 16, Bn);
             if constexpr (GP == 4 && !EDDY) {
                 if (rot) wait_vm<L::NDMA + 2>(); else wait_vm<L::NDMA>();
             } else {
@@ -514,7 +526,6 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
         } else {
             wait_vm<0>();
         }
-        const uint32_t* B = &ybuf[wv][st & 1][0];
         if (sch == 0) {
 #pragma unroll
             for (int vv = 0; vv < V; vv++)
@@ -598,7 +609,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                 // One loop for both kinds of pass where the accumulators are many (paired tiles) or the
                 // chain state is small (ECHAIN); otherwise separate loops, which keep fewer operands live.
                 if constexpr (V == 1 && (PAIRED || ECHAIN)) {
-                    if (CHAINS && chain) chain_steps(lines_s[g0 * 16 + ln].cls);
+                    if (CHAINS && chain) chain_steps(lines[g0 * 16 + ln].cls);
                     float ps[4], pc[4], dq[4][Q];      // phase and decays of the current line
                     float zr[4], zi[4], sr[4], si[4];  // ECHAIN: E of the current line and its step
 #pragma unroll
@@ -645,7 +656,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                                     }
                                 }
                             } else {
-                                const LineRec& Lr = lines_s[(g0 + g) * 16 + ln];
+                                const LineRec& Lr = lines[(g0 + g) * 16 + ln];
                                 const bool ok = Lr.ky >= 0;
                                 const double t_s = Lr.t_s, kyn = Lr.kyn, d = Lr.d;
                                 const float trf = Lr.trf, tau = Lr.tau;
@@ -690,7 +701,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                     }
                 } else {
                     if (chain) {
-                        const LineRec& L0 = lines_s[g0 * 16 + ln];
+                        const LineRec& L0 = lines[g0 * 16 + ln];
                         chain_steps(L0.cls);   // loaded here: live only on this path
                         float ps[4], pc[4], dq[4][Q];
 #pragma unroll
@@ -735,7 +746,7 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 #pragma unroll
                         for (int g = 0; g < GP; g++) {
                             if (g0 + g < G) {
-                                const LineRec& Lr = lines_s[(g0 + g) * 16 + ln];
+                                const LineRec& Lr = lines[(g0 + g) * 16 + ln];
                                 const bool ok = Lr.ky >= 0;
                                 if constexpr (img_late) {
                                     asm volatile("" ::: "memory");
@@ -809,7 +820,13 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
         }
         }
         sxw = nxw_; spi = npi; sch = nchi;
-    }
+    };
+    // The line table goes in through an offset the compiler cannot fold (a scalar zero from asm): a
+    // restrict parameter bound to the array itself would get no alias scope.
+    uint32_t zero;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(zero));
+    const LineRec* lines_p = lines_s + zero;
+    for (int st = 0; st < nstage; st++) run_stage(st, &ybuf[wv][st & 1][0], &ybuf[wv][(st + 1) & 1][0], lines_p);
 }
 
 // ----------------------------------------------------------------------------- x-stage, MFMA
