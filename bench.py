@@ -181,7 +181,7 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
     if tiles:
         fw.plan.set_tiles(*(int(v) for v in tiles.split(",")))
     else:
-        log(fw.tune(2).strip().splitlines()[-1])
+        log(fw.tune(5).strip())
     log(fw.describe().strip())
     if stages_only:
         st = fw.stage_times(iters=steps)
@@ -332,7 +332,7 @@ def main():
     if args.tiles:
         plan.set_tiles(*(int(v) for v in args.tiles.split(",")))
     else:
-        log(plan.tune(trials=2).strip())
+        log(plan.tune(trials=5).strip())
     nunits = len(w.units)
     if args.stages_only:
         st_ms = plan.stage_times(iters=args.steps)
