@@ -1747,6 +1747,7 @@ hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const D
     switch (mt) {
     case 1: hipLaunchKernelGGL((xstage_mfma<1>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
     case 2: hipLaunchKernelGGL((xstage_mfma<2>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    case 3: hipLaunchKernelGGL((xstage_mfma<3>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
     case 4: hipLaunchKernelGGL((xstage_mfma<4>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
     case 8: hipLaunchKernelGGL((xstage_mfma<8>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
     default: return hipErrorInvalidValue;
@@ -1758,6 +1759,7 @@ hipError_t set_xstage_lds_limit(size_t bytes) {
     hipError_t e;
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
+    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
     return hipFuncSetAttribute((const void*)xstage_mfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }

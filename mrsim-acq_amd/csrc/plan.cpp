@@ -765,11 +765,15 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const DevGroup& dg = p->groups[gi].dev;
             const int snxP = (dg.snx + 15) & ~15, nxP = (dg.nx + 7) & ~7;
             const int ntab = a_ghost(p->groups[gi]) ? 2 : 1;
-            // kx tiles of 16 rows; MT tiles per block
-            int mt = 1;
-            while (mt * 16 < nxP && mt < 4) mt *= 2;
+            // kx tiles of 16 rows, MT (1..4) tiles per block: the fewest kx blocks (each block of a
+            // unit reads all of its G), then the fewest padded rows, within the LDS budget
             auto lds_of = [&](int m) { return (size_t)ntab * 2 * 16 * m * (snxP + 4) * 4; };
-            while (mt > 1 && lds_of(mt) > 80 * 1024) mt /= 2;
+            int mt = 1;
+            for (int m = 2; m <= 4; m++) {
+                if (lds_of(m) > 80 * 1024) break;
+                const int kb = (nxP + 16 * m - 1) / (16 * m), kb0 = (nxP + 16 * mt - 1) / (16 * mt);
+                if (kb < kb0 || (kb == kb0 && kb * m < kb0 * mt)) mt = m;
+            }
             const int nkxb = (nxP + 16 * mt - 1) / (16 * mt);
             const int ntiles = dg.LyP * dg.C / 16;
             const size_t want = 1024 / std::max<size_t>(1, g_units[gi] * nkxb) + 1;

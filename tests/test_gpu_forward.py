@@ -166,6 +166,15 @@ def test_pf_256_matches_fp64_oracle():
         assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("name", ["tile_96", "tile_192"])
+def test_three_row_tile_x_stage_matches_fp64_oracle(name):
+    # 48 kx in one block of three 16-row tiles, and 96 kx in two such blocks
+    w = workloads.make(name)
+    errs = _plan_vs_oracle(w, abi.PREC_SPLIT_BF16, [0])
+    print(f"{name} rel-L2 vs FP64: {errs}", flush=True)
+    assert max(errs) <= TOL
+
+
 def test_complex_images_match_fp64_oracle():
     w = workloads.make("grad_128")
     errs = _plan_vs_oracle(w, abi.PREC_SPLIT_BF16, [0])
