@@ -1059,32 +1059,66 @@ __global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ 
 
 // ----------------------------------------------------------------------------- adjoint (VJP)
 // Cotangent g = dL/dK (PyTorch's conjugate-Wirtinger convention), per unit.
-// x-adjoint: Gb_c(x, l) = sum_kx conj(T_tab(l)[kx, x]) g_c[kx, ky_l]  (FP32), thread per (x, line, coil).
+// x-adjoint: Gb_c(x, l) = sum_kx conj(T_tab(l)[kx, x]) g_c[kx, ky_l]  (FP32).  Block = 4 waves; a wave
+// owns one line and a group of XA_CG coils, its 64 lanes consecutive x.  The twiddle column reads
+// are coalesced along x and reused for the group's coils; the cotangent values are uniform over the
+// wave (scalar loads).  grid = (ceil(snx / 64), LyP/4 * ceil(C / XA_CG), units).
+constexpr int XA_CG = 8;
 __global__ __launch_bounds__(256) void xadjoint(const DevUnit* __restrict__ units, const DevGroup* __restrict__ groups,
                                                 const VjpUnit* __restrict__ vj, int unit0) {
-    const int unit = unit0 + blockIdx.y;
+    const int unit = unit0 + blockIdx.z;
     const DevUnit& u = units[unit];
     const VjpUnit& v = vj[unit];
     const DevGroup& gr = groups[u.group];
-    const int C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15, nx = gr.nx;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = t % C, lp = (t / C) % LyP, x = t / (C * LyP);
-    if (x >= snx) return;
+    const int C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15, nx = gr.nx, ny = gr.ny;
+    const int ncg = (C + XA_CG - 1) / XA_CG;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int cg = blockIdx.y % ncg, lp = (blockIdx.y / ncg) * 4 + wave;   // LyP is a multiple of 16
+    const int x = blockIdx.x * 64 + lane, c0 = cg * XA_CG;
     const LineRec L = gr.lines[lp];
-    float2 acc = make_float2(0.f, 0.f);
+    float ar[XA_CG], ai[XA_CG];
+#pragma unroll
+    for (int k = 0; k < XA_CG; k++) ar[k] = ai[k] = 0.0f;
     if (L.ky >= 0 && v.gout) {
-        const float2* __restrict__ tw = gr.t32[L.table];
-        const float2* __restrict__ g = v.gout + ((size_t)c * gr.ny + L.ky) * nx;
-        for (int kx = 0; kx < nx; kx++) {
-            const float2 w = tw[(size_t)kx * snxP + x], gv = g[kx];
-            acc.x = __builtin_fmaf(w.x, gv.x, __builtin_fmaf(w.y, gv.y, acc.x));
-            acc.y = __builtin_fmaf(w.x, gv.y, __builtin_fmaf(-w.y, gv.x, acc.y));
-        }
+        // Lanes past the row read column snx - 1 and coils past C read coil C - 1 (both in bounds);
+        // neither is stored.  Four kx per step, every load of the step issued before its products.
+        const GAS f32x2* tw = reinterpret_cast<const GAS f32x2*>(gld(gr.t32[L.table])) + min(x, snx - 1);
+        const GAS f32x2* g = reinterpret_cast<const GAS f32x2*>(gld(v.gout)) + (size_t)L.ky * nx;
+        const size_t cstride = (size_t)ny * nx;
+        size_t coff[XA_CG];
+#pragma unroll
+        for (int k = 0; k < XA_CG; k++) coff[k] = (size_t)min(c0 + k, C - 1) * cstride;
+        auto step = [&](int kx, int nk) {
+            f32x2 w[4], gv[XA_CG][4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[j] = j < nk ? tw[(size_t)(kx + j) * snxP] : f32x2{0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < XA_CG; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) gv[k][j] = j < nk ? g[coff[k] + kx + j] : f32x2{0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < XA_CG; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    ar[k] = __builtin_fmaf(w[j].x, gv[k][j].x, __builtin_fmaf(w[j].y, gv[k][j].y, ar[k]));
+                    ai[k] = __builtin_fmaf(w[j].x, gv[k][j].y, __builtin_fmaf(-w[j].y, gv[k][j].x, ai[k]));
+                }
+        };
+        int kx = 0;
+        for (; kx + 4 <= nx; kx += 4) step(kx, 4);
+        if (kx < nx) step(kx, nx - kx);
     }
-    v.gbar[((size_t)x * LyP + lp) * C + c] = acc;
+    if (x < snx) {
+        float2* __restrict__ o = v.gbar + ((size_t)x * LyP + lp) * C + c0;
+#pragma unroll
+        for (int k = 0; k < XA_CG; k++)
+            if (c0 + k < C) o[k] = make_float2(ar[k], ai[k]);
+    }
 }
 
-// y-adjoint: one lane per voxel (x, y), looping over the lines.  With Eb = sum_c conj(S_c) Gb_c,
+// y-adjoint: one lane per voxel (x, y), looping over the lines; a block owns 256 y of one column x,
+// and each 64-line pass of the column's Gb (every lane reads the same values) is staged in LDS.
+// grid = (snx * ceil(sny / 256), units).  With Eb = sum_c conj(S_c) Gb_c,
 // t_l = conj(P_l) Eb_l and D_q,l = exp(-trf/T2_q - tau/T2'_q):
 //   dL/dm_q   = sum_l D_q,l t_l                      (complex images; real images take the real part)
 //   dL/dfmap  = 2 pi Im( sum_q conj(m_q) sum_l tb0_l D_q,l t_l )
@@ -1095,19 +1129,21 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
                                                 const DevGroup* __restrict__ groups, const VjpUnit* __restrict__ vj,
                                                 int unit0) {
     __shared__ LineRec lines_s[256];
+    __shared__ float2 gs[64 * CM];
     const int unit = unit0 + blockIdx.y;
     const DevUnit& u = units[unit];
     const VjpUnit& vv = vj[unit];
     const DevGroup& gr = groups[u.group];
     const DevSlice& sl = slices[u.slice];
     for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
-    __syncthreads();
     const int snx = gr.snx, sny = gr.sny, snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
     const int C = gr.C, LyP = gr.LyP;
     const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
-    const int vox = blockIdx.x * blockDim.x + threadIdx.x;
-    const int y = vox % sny, x = vox / sny;
-    if (x >= snx) return;
+    const int ycb = (sny + 255) / 256;
+    const int x = blockIdx.x / ycb, y0 = (blockIdx.x % ycb) * 256 + threadIdx.x;
+    // lanes past the column evaluate its last voxel (in bounds), take part in the staging, store nothing
+    const bool act = y0 < sny;
+    const int y = act ? y0 : sny - 1;
     const size_t o = (size_t)x * snyP + y, i = (size_t)x + (size_t)snx * y;
     const double base = sl.base[o], rate = (double)sl.rate[o], poly = sl.poly[o];
     const double yy = (double)y - gr.sys_yoff;
@@ -1134,6 +1170,11 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
     const float2* __restrict__ gb = vv.gbar + (size_t)x * LyP * C;
     const int G = LyP >> 4;
     for (int g0 = 0; g0 < G; g0 += 4) {
+        // stage this pass's lines (up to 64) x C of the column: gs[line - 16 g0][coil]
+        const int nl = min(64, LyP - g0 * 16);
+        __syncthreads();   // the previous pass's reads (and, first, the line table) are done
+        for (int k = threadIdx.x; k < nl * C; k += blockDim.x) gs[(k / C) * CM + k % C] = gb[(size_t)g0 * 16 * C + k];
+        __syncthreads();
         const bool chain = (g0 >> 2) < gr.rotor_passes && g0 + 4 <= G;
         for (int ln = 0; ln < 16; ln++) {
             float pc = 0.f, ps = 0.f, rc = 1.f, rs = 0.f;
@@ -1167,7 +1208,7 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
                     for (int q = 0; q < Q; q++) dq[q] *= rq[q];
                 }
                 // Eb = sum_c conj(S_c) Gb_c
-                const float2* __restrict__ gl = gb + (size_t)lp * C;
+                const float2* gl = gs + (size_t)(lp - g0 * 16) * CM;
                 float er = 0.f, ei = 0.f;
 #pragma unroll
                 for (int c = 0; c < CM; c++) {
@@ -1194,6 +1235,7 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
             }
         }
     }
+    if (!act) return;
     if (vv.gimg) {
 #pragma unroll
         for (int q = 0; q < Q; q++) {
@@ -1927,13 +1969,19 @@ static hipError_t yadj_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit*
 // Adjoint of units [unit0, unit0 + n) sharing one group shape.
 hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, const DevUnit* u,
                             const DevSlice* s, const DevGroup* g, const VjpUnit* v, hipStream_t st) {
-    if (n <= 0) return hipSuccess;
-    const int tx = snx * LyP * C;
-    hipLaunchKernelGGL(xadjoint, dim3((tx + 255) / 256, n), dim3(256), 0, st, u, g, v, unit0);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    dim3 grid((snx * sny + 255) / 256, n);
-    return cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0) : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0);
+    // units ride on a grid dimension of at most 65535 blocks
+    for (int off = 0; off < n; off += 65535) {
+        const int m = std::min(65535, n - off);
+        hipLaunchKernelGGL(xadjoint, dim3((snx + 63) / 64, (LyP / 4) * ((C + XA_CG - 1) / XA_CG), m), dim3(256), 0, st,
+                           u, g, v, unit0 + off);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        dim3 grid(snx * ((sny + 255) / 256), m);
+        e = cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)
+                 : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0 + off);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 

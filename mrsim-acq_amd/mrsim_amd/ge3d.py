@@ -124,6 +124,9 @@ class Ge3dForward:
         self.out: List[List[object]] = []
         zsets: List[ZSet] = []
         self._keep = []
+        self._umeta = []      # per unit: (volume, echo, index among the used compartments, partition)
+        self._used = []       # per volume: the compartments with an image
+        self._zadj = {}       # (volume, echo) -> (n_used, weights) of its partition transform
         for vi, v in enumerate(self.volumes):
             check_volume(v, acq, ky_segments)
             dev = v.fmap.device
@@ -131,6 +134,7 @@ class Ge3dForward:
             C = v.coil_maps.shape[1]
             ncomp = len(v.t2)
             used = [c for c in range(ncomp) if v.images[c] is not None]
+            self._used.append(used)
             t, pol = grase_block(v.ny, ky_segments, t_line_ms, bool(acq.reverse_phase))
             w_all = v.weights.reshape(v.nz * ky_segments, ncomp)
             w_used = w_all[:, used].contiguous().reshape(-1) if used else None
@@ -160,7 +164,9 @@ class Ge3dForward:
                                             else (float(tic),) * abi.MRS_MAX_COMP,
                                             z=z, nz=v.nz))
                         units.append(Unit(si, img[z], k2[ci, z]))
+                        self._umeta.append((vi, e, ci, z))
                 if used:
+                    self._zadj[(vi, e)] = (len(used), w_used)
                     zsets.append(ZSet(inp=k2, out=out, nz=v.nz, n_coils=C, ny=v.ny, nx=v.nx, n_comp=len(used),
                                       ky_segments=ky_segments, weights=w_used))
                 k2_v.append(k2)
@@ -176,6 +182,61 @@ class Ge3dForward:
     def forward(self, stream=None):
         self.plan.forward(stream)
         return self.out
+
+    def vjp(self, grad_out, stream=None):
+        """Vector-Jacobian product of the train for a real loss L: grad_out[v][e] = dL/d out[v][e]
+        ((nz, C, ny, nx) complex64, torch's convention dL/dRe + i dL/dIm).
+
+        The partition transform's adjoint (conjugate transpose with the per-excitation weights,
+        src/kspace3d.rs:90-100, 716-722) takes each (volume, echo) cotangent to its per-partition 2D
+        cotangents; mrs_plan_vjp runs the adjoint of every partition unit (gradient-echo decay at the
+        echo's TE); the units' gradients are summed over the echoes, and for the fieldmap over the
+        compartments, that share an input.  The forward's inputs must be bound as at construction.
+        Returns per volume a dict of device tensors:
+          images:  per compartment (nz*P,) float32, None for a compartment without an image
+          fmap:    (nz*P,) float32, per Hz
+          t_inhom: per compartment (nz*P,) float32, per ms of T2' (at the uniform value where T2' is
+                   uniform: sum it over the voxels for the scalar's gradient)
+        """
+        import torch
+        if not hasattr(self, "_vjp_buf"):
+            gk2 = {k: torch.empty_like(self.k2[k[0]][k[1]]) for k in self._zadj}
+            per_vol = []
+            for vi, v in enumerate(self.volumes):
+                shape = (len(self.tes), max(len(self._used[vi]), 1), v.nz, v.snx * v.sny)
+                dev = v.fmap.device
+                per_vol.append([torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3)])
+            self._vjp_buf = (gk2, per_vol)
+        gk2, per_vol = self._vjp_buf
+        sets = []
+        for (vi, e), (n_used, w) in self._zadj.items():
+            v = self.volumes[vi]
+            g = grad_out[vi][e]
+            if g.dtype != torch.complex64 or not g.is_contiguous() or g.shape != self.out[vi][e].shape:
+                raise ValueError(f"grad_out[{vi}][{e}] must be a contiguous complex64 tensor of shape "
+                                 f"{tuple(self.out[vi][e].shape)}")
+            sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
+                             n_comp=n_used, ky_segments=self.ky_segments, weights=w))
+        zstage(sets, adjoint=True, stream=stream)
+        gout, gimg, gfm, gti = [], [], [], []
+        for vi, e, ci, z in self._umeta:
+            gout.append(gk2[(vi, e)][ci, z])
+            b = per_vol[vi]
+            gimg.append(b[0][e, ci, z])
+            gfm.append(b[1][e, ci, z])
+            gti.append(b[2][e, ci, z])
+        self.plan.vjp(gout, gimg, gfm, gti, stream)
+        out = []
+        for vi, v in enumerate(self.volumes):
+            b = per_vol[vi]
+            used = self._used[vi]
+            imgs, tis = [None] * len(v.images), [None] * len(v.images)
+            for ci, c in enumerate(used):
+                imgs[c] = b[0][:, ci].sum(dim=0).reshape(-1)
+                tis[c] = b[2][:, ci].sum(dim=0).reshape(-1)
+            fm = b[1].sum(dim=(0, 1)).reshape(-1) if used else torch.zeros_like(v.fmap)
+            out.append({"images": imgs, "fmap": fm, "t_inhom": tis})
+        return out
 
     def _slices(self, vi, e, g, seed, acq):
         """Spikes and noise on a copy of the (volume, echo) 3D k-space, then the inverse z-DFT / nz:

@@ -344,6 +344,6 @@ def to_ge3d(w: Ge3dWorkload, precision=abi.PREC_SPLIT_BF16, device="cuda"):
                                images=[dev(im.reshape(-1)) for im in v.images],
                                t2=[dev(t.reshape(-1)) for t in v.t2], fmap=dev(v.fmap.reshape(-1)),
                                coil_maps=dev(v.coil_maps), weights=dev(v.weights.reshape(-1), torch.float32),
-                               t_inhom=list(v.t_inhom)))
+                               t_inhom=[dev(t.reshape(-1)) if isinstance(t, np.ndarray) else t for t in v.t_inhom]))
     fw = Ge3dForward(vols, w.acq, w.echo_times_ms, w.ky_segments, w.t_line_ms, precision)
     return fw, vols

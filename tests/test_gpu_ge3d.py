@@ -7,6 +7,9 @@ FP64 oracle (oracle/ge3d.py).
 - the batched forward at reduced sizes (two geometries of uneven size in one plan, two echoes, two
   compartments, z-varying coil maps, 2 ky segments): every (volume, echo) against the oracle,
   rel-L2 <= 1e-5, split-BF16 and plain FP32, eager and graph replay;
+- the train's vector-Jacobian product (Ge3dForward.vjp) on the small train with T2' maps: the adjoint
+  identity for the images, and the fieldmap, T2' and image gradients at sampled voxels against FP64
+  central differences of the oracle's train;
 - the full-size train (two volumes, 128^2 -> 64^2 and 192^2 -> 96^2, 24 partitions, 32 coils,
   4 echoes): one coil of each volume at the first and last echo against the oracle (rel-L2 <= 1e-5),
   and for every coil and echo the size-independent round trip: the output divided by its weights and
@@ -143,6 +146,97 @@ def test_ge3d_refusals(torch_dev):
         kw = dict(ok, **over)
         with pytest.raises(ValueError, match=msg):
             Ge3dForward([Ge3dVolume(24, 24, 12, 12, nz, **kw)], a, [12.0], 1, 0.4)
+
+
+@pytest.fixture(scope="module")
+def grad_setup(torch_dev):
+    # the small train with T2' maps (the gradient is per voxel) and a seeded complex cotangent
+    torch = torch_dev
+    w = workloads.make_ge3d("ge3d_small")
+    rng = np.random.default_rng(31)
+    for v in w.volumes:
+        v.t_inhom = [rng.uniform(35.0, 55.0, size=(v.nz, v.snx * v.sny)).astype(np.float32) for _ in v.images]
+    fw, vols = workloads.to_ge3d(w)
+    fw.forward()
+    g = [[(rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)).astype(np.complex64) for o in ov]
+         for ov in fw.out]
+    gd = [[torch.from_numpy(x).cuda() for x in gv] for gv in g]
+    grads = fw.vjp(gd)
+    torch.cuda.synchronize()
+    yield w, fw, vols, g, grads
+    fw.close()
+
+
+def test_ge3d_vjp_adjoint_identity(grad_setup):
+    # the train is linear in the images: Re <g, A m> = <A^H g, m> summed over volumes and echoes
+    w, fw, vols, g, grads = grad_setup
+    lhs, n_out, n_g = 0.0, 0.0, 0.0
+    for vi in range(len(vols)):
+        for e in range(len(w.echo_times_ms)):
+            k = fw.out[vi][e].cpu().numpy().astype(np.complex128)
+            lhs += float(np.real(np.vdot(g[vi][e], k)))
+            n_out += float(np.sum(np.abs(k) ** 2))
+            n_g += float(np.sum(np.abs(g[vi][e]) ** 2))
+    rhs = 0.0
+    for vi, v in enumerate(vols):
+        for c, im in enumerate(v.images):
+            rhs += float(np.dot(grads[vi]["images"][c].cpu().numpy().astype(np.float64),
+                                im.cpu().numpy().astype(np.float64)))
+    scale = np.sqrt(n_out * n_g)
+    print(f"3D vjp adjoint identity: {lhs:.6e} vs {rhs:.6e} (|diff| / (|g||Am|) = {abs(lhs - rhs) / scale:.2e})",
+          flush=True)
+    assert abs(lhs - rhs) <= 1e-5 * scale
+
+
+def _ge3d_oracle_loss(w, v, gv, images=None, fmap=None, t_inhom=None):
+    loss = 0.0
+    for e in range(len(w.echo_times_ms)):
+        k = G.ge3d_kspace(v.snx, v.sny, v.nx, v.ny, v.nz, v.images if images is None else images, v.t2,
+                          v.t_inhom if t_inhom is None else t_inhom, v.fmap if fmap is None else fmap, None,
+                          v.coil_maps, w.acq, w.echo_times_ms[e], w.ky_segments, w.t_line_ms,
+                          v.weights.astype(np.float32))
+        loss += float(np.real(np.vdot(gv[e], k)))
+    return loss
+
+
+def test_ge3d_vjp_matches_fp64_central_differences(grad_setup):
+    # volume 1 (32^2 -> 16^2, 6 partitions, 2 compartments, 2 echoes): the fieldmap, T2' and image
+    # gradients at bright voxels of two partitions against FP64 differences of the oracle's train
+    w, fw, vols, g, grads = grad_setup
+    vi = 1
+    v = w.volumes[vi]
+    P = v.snx * v.sny
+    gf = grads[vi]["fmap"].cpu().numpy()
+    gt = [t.cpu().numpy() for t in grads[vi]["t_inhom"]]
+    gi = [t.cpu().numpy() for t in grads[vi]["images"]]
+    mag = np.abs(v.images[0]) + np.abs(v.images[1])
+    for z in (1, 4):
+        p = int(np.argmax(mag[z]))
+        i = z * P + p
+        h = 1e-3 * max(abs(float(v.fmap[z, p])), 1.0)
+        fp, fm = v.fmap.copy(), v.fmap.copy()
+        fp[z, p] += h
+        fm[z, p] -= h
+        d = (_ge3d_oracle_loss(w, v, g[vi], fmap=fp) - _ge3d_oracle_loss(w, v, g[vi], fmap=fm)) / (fp[z, p] - fm[z, p])
+        print(f"partition {z} voxel {p}: dL/dfmap gpu {gf[i]:.6e} fp64 {d:.6e}", flush=True)
+        assert abs(gf[i] - d) <= 2e-3 * max(abs(d), 1e-3 * np.abs(gf).max())
+        base = _ge3d_oracle_loss(w, v, g[vi])
+        for c in range(len(v.images)):
+            h = 1e-3 * float(v.t_inhom[c][z, p])
+            tp = [t.copy() for t in v.t_inhom]
+            tm = [t.copy() for t in v.t_inhom]
+            tp[c][z, p] += h
+            tm[c][z, p] -= h
+            d = (_ge3d_oracle_loss(w, v, g[vi], t_inhom=tp) - _ge3d_oracle_loss(w, v, g[vi], t_inhom=tm)) / \
+                (tp[c][z, p] - tm[c][z, p])
+            print(f"  compartment {c}: dL/dT2' gpu {gt[c][i]:.6e} fp64 {d:.6e}", flush=True)
+            assert abs(gt[c][i] - d) <= 2e-3 * max(abs(d), 1e-3 * np.abs(gt[c]).max())
+            # image: linear, so a unit step gives the derivative up to FP64 rounding
+            im = [x.copy() for x in v.images]
+            im[c][z, p] += 1.0
+            d = _ge3d_oracle_loss(w, v, g[vi], images=im) - base
+            print(f"  compartment {c}: dL/dimage gpu {gi[c][i]:.6e} fp64 {d:.6e}", flush=True)
+            assert abs(gi[c][i] - d) <= 2e-4 * max(abs(d), 1e-3 * np.abs(gi[c]).max())
 
 
 @pytest.fixture(scope="module")
