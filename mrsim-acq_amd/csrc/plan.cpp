@@ -237,8 +237,9 @@ struct mrs_plan {
         WorkItem* d_items = nullptr;
         std::vector<size_t> off;
     };
-    // units split into chunks whose y -> x stage pairs run on forked streams, so one chunk's
-    // memory-bound x-stage overlaps the next chunk's compute-bound y-stage
+    // units split into chunks whose y -> x stage pairs run on forked streams: the chunks' y-stages
+    // run side by side (each one's last wave of blocks is filled by the other's), and one chunk's
+    // x-stage overlaps the rest of the other's y-stage
     int nchunks = 1;
     std::vector<int> unit_chunk;
     std::vector<hipStream_t> streams;
@@ -750,7 +751,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         // work lists: units grouped by shared coil planes, then slice, so an XCD's run covers the
         // volumes and echoes of few slices
         const std::vector<size_t> order = unit_order(p);
-        p->nchunks = nunits >= 64 ? 4 : 1;
+        // two chunks: measured ahead of one (the x-stage then trails every y-stage), four and eight
+        p->nchunks = nunits >= 64 ? 2 : 1;
         p->unit_chunk.assign(nunits, 0);
         for (size_t k = 0; k < nunits; k++) p->unit_chunk[order[k]] = (int)(k * p->nchunks / nunits);
         std::vector<std::vector<std::vector<WorkItem>>> xw;
