@@ -1260,6 +1260,181 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
     }
 }
 
+// y-adjoint on MFMA (split-BF16 plans).  The coil contraction of a column x,
+//   H(y, l) = sum_c conj(S_c(x, y)) Gb_c(x, l),
+// runs as split-BF16 products: A = the coil rows of 16 y (tile pair: real rows [S_re, S_im] and
+// imaginary rows [-S_im, S_re] per coil), B = Gb of 16 lines (k = coil, part).  Each lane then holds
+// H for its 4 voxels (rows 4 lq..4 lq + 3) at its line (column ln), evaluates their phase and decays
+// there and adds the same sums as the VALU y-adjoint; the sums over the 16 lines of a tile sit in 16
+// lanes and are added across them at the end.  Block = 4 waves on one (column, unit), a wave owning
+// the y tiles w, w + 4, ...; NCS 16-coil steps cover C <= 16 NCS.  grid = (snx, units).
+template <int Q, bool CPLX, int NCS>
+__global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__ units,
+                                                     const DevSlice* __restrict__ slices,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const VjpUnit* __restrict__ vj, int unit0) {
+    __shared__ LineRec lines_s[256];
+    const int unit = unit0 + blockIdx.y;
+    const DevUnit& u = units[unit];
+    const VjpUnit& vv = vj[unit];
+    const DevGroup& gr = groups[u.group];
+    const DevSlice& sl = slices[u.slice];
+    for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
+    __syncthreads();
+    const int snx = gr.snx, sny = gr.sny, snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
+    const int C = gr.C, LyP = gr.LyP, G = LyP >> 4, Cp = gr.RB * 8;
+    const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
+    const int x = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lq = lane >> 4, ln = lane & 15;
+    const int nyt = (sny + 15) >> 4;
+    const float2* __restrict__ gb = vv.gbar + (size_t)x * LyP * C;
+    for (int yt = wave; yt < nyt; yt += 4) {
+        // A rows y = 16 yt + ln (< snyP; the split planes hold zero coils past C and zero rows past sny)
+        u32x4 arh[NCS], arl[NCS], aih[NCS], ail[NCS];
+        const int ya = yt * 16 + ln;
+#pragma unroll
+        for (int s = 0; s < NCS; s++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int c = s * 16 + 4 * lq + j;
+                uint32_t h = 0u, l = 0u;
+                if (c < Cp) {
+                    const size_t so = ((size_t)x * Cp + c) * snyP + ya;
+                    h = sl.s_hi[so];
+                    l = sl.s_lo[so];
+                }
+                arh[s][j] = h;
+                arl[s][j] = l;
+                aih[s][j] = __builtin_rotateright32(h, 16) ^ 0x8000u;
+                ail[s][j] = __builtin_rotateright32(l, 16) ^ 0x8000u;
+            }
+        // the lane's voxels y0 + j (rows of its accumulator tile; the planes are padded to snyP)
+        const int y0 = yt * 16 + 4 * lq;
+        double base[4], poly[4];
+        float rate[4], nr2[Q][4], nri[Q][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const size_t o = (size_t)x * snyP + y0 + j;
+            base[j] = sl.base[o];
+            poly[j] = sl.poly[o];
+            rate[j] = sl.rate[o];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                nr2[q][j] = sl.nr2[q * PP + o];
+                nri[q][j] = sl.nri[q * PP + o];
+            }
+        }
+        const double yy0 = (double)y0 - gr.sys_yoff;
+        float ar[Q][4], ai[Q][4], br[Q][4], bi[Q][4], fr[Q][4], fi[Q][4];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) ar[q][j] = ai[q][j] = br[q][j] = bi[q][j] = fr[q][j] = fi[q][j] = 0.0f;
+        for (int g = 0; g < G; g++) {
+            const int lp = g * 16 + ln;
+            u32x4 bh[NCS], bl[NCS];
+#pragma unroll
+            for (int s = 0; s < NCS; s++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = s * 16 + 4 * lq + j;
+                    const float2 v = c < C ? gb[(size_t)lp * C + c] : make_float2(0.f, 0.f);
+                    uint32_t h, l;
+                    split2(v.x, v.y, h, l);
+                    bh[s][j] = h;
+                    bl[s][j] = l;
+                }
+            f32x4 hr = f32x4{0.f, 0.f, 0.f, 0.f}, hi = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < NCS; s++) {
+                hr = mfma3(arh[s], arl[s], bh[s], bl[s], hr);
+                hi = mfma3(aih[s], ail[s], bh[s], bl[s], hi);
+            }
+            const LineRec& L = lines_s[lp];
+            if (L.ky >= 0) {
+                const float trf = L.trf, tau = L.tau, tb0 = L.tb0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    double cyc = __builtin_fma(L.kyn, yy0 + (double)j, base[j]);
+                    cyc = __builtin_fma((double)rate[j], L.t_s, cyc);
+                    cyc = __builtin_fma(poly[j], L.d, cyc);
+                    float ps, pc;
+                    sincos_turns(cyc, ps, pc);
+                    // t = conj(P) H
+                    const float tr = pc * hr[j] + ps * hi[j], ti = pc * hi[j] - ps * hr[j];
+#pragma unroll
+                    for (int q = 0; q < Q; q++) {
+                        const float d = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, nr2[q][j], tau * nri[q][j]));
+                        ar[q][j] = __builtin_fmaf(d, tr, ar[q][j]);
+                        ai[q][j] = __builtin_fmaf(d, ti, ai[q][j]);
+                        const float dt = d * tau, df = d * tb0;
+                        br[q][j] = __builtin_fmaf(dt, tr, br[q][j]);
+                        bi[q][j] = __builtin_fmaf(dt, ti, bi[q][j]);
+                        fr[q][j] = __builtin_fmaf(df, tr, fr[q][j]);
+                        fi[q][j] = __builtin_fmaf(df, ti, fi[q][j]);
+                    }
+                }
+            }
+        }
+        // sums over the tile's 16 lines (lanes ln of one row group), then lane ln = j writes voxel y0 + j
+        auto rsum = [](float v) {
+#pragma unroll
+            for (int m = 1; m < 16; m <<= 1) v += __shfl_xor(v, m, 16);
+            return v;
+        };
+        float oar[Q], oai[Q], obr[Q], obi[Q], ofr[Q], ofi[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            oar[q] = oai[q] = obr[q] = obi[q] = ofr[q] = ofi[q] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float a = rsum(ar[q][j]), b = rsum(ai[q][j]), c = rsum(br[q][j]), d = rsum(bi[q][j]);
+                const float e = rsum(fr[q][j]), f = rsum(fi[q][j]);
+                if (ln == j) { oar[q] = a; oai[q] = b; obr[q] = c; obi[q] = d; ofr[q] = e; ofi[q] = f; }
+            }
+        }
+        const int y = y0 + ln;
+        if (ln < 4 && y < sny) {
+            const size_t i = (size_t)x + (size_t)snx * y;
+            float m_r[Q], m_i[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if constexpr (CPLX) {
+                    const float2 m = reinterpret_cast<const float2*>(u.images)[q * P + i];
+                    m_r[q] = m.x;
+                    m_i[q] = m.y;
+                } else {
+                    m_r[q] = reinterpret_cast<const float*>(u.images)[q * P + i];
+                    m_i[q] = 0.0f;
+                }
+            }
+            if (vv.gimg) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    if constexpr (CPLX) reinterpret_cast<float2*>(vv.gimg)[q * P + i] = make_float2(oar[q], oai[q]);
+                    else reinterpret_cast<float*>(vv.gimg)[q * P + i] = oar[q];
+                }
+            }
+            if (vv.gfmap) {
+                float acc = 0.f;
+#pragma unroll
+                for (int q = 0; q < Q; q++) acc += m_r[q] * ofi[q] - m_i[q] * ofr[q];
+                vv.gfmap[i] = gr.do_dist ? (float)(kTau * (double)acc) : 0.0f;
+            }
+            if (vv.gti) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const float ti_ms = sl.t_inhom ? sl.t_inhom[q * P + i]
+                                                   : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
+                    const float re = m_r[q] * obr[q] + m_i[q] * obi[q];
+                    vv.gti[q * P + i] = gr.do_relax ? re / (ti_ms * ti_ms) : 0.0f;
+                }
+            }
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------- z-stage (3D partitions)
 // Centred z-DFT across partitions with per-excitation weights, kspace3d.rs:90-100, 677-692, 716-722:
 //   forward  out[p][n] (+)= sum_c w(p, sy(n), c) sum_z zk[p][z] in_c[z][n]
@@ -1966,9 +2141,32 @@ static hipError_t yadj_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit*
     }
 }
 
-// Adjoint of units [unit0, unit0 + n) sharing one group shape.
-hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, const DevUnit* u,
-                            const DevSlice* s, const DevGroup* g, const VjpUnit* v, hipStream_t st) {
+template <int Q, bool CPLX>
+static hipError_t yadjm_c(int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                          const VjpUnit* v, int unit0) {
+    if (C <= 16) hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 1>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    else if (C <= 32) hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 2>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    else hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 4>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+static hipError_t yadjm_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                          const DevGroup* g, const VjpUnit* v, int unit0) {
+    switch (Q) {
+    case 1: return yadjm_c<1, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 2: return yadjm_c<2, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 3: return yadjm_c<3, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 4: return yadjm_c<4, CPLX>(C, grid, st, u, s, g, v, unit0);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the y-adjoint's coil
+// contraction as split-BF16 MFMA (needs the split coil planes), else FP32 on the VALU (the FP32 ones).
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, bool mfma,
+                            const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
+                            hipStream_t st) {
     // units ride on a grid dimension of at most 65535 blocks
     for (int off = 0; off < n; off += 65535) {
         const int m = std::min(65535, n - off);
@@ -1976,9 +2174,15 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, 
                            u, g, v, unit0 + off);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        dim3 grid(snx * ((sny + 255) / 256), m);
-        e = cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)
-                 : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0 + off);
+        if (mfma) {
+            dim3 grid(snx, m);
+            e = cplx ? yadjm_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)
+                     : yadjm_q<false>(Q, C, grid, st, u, s, g, v, unit0 + off);
+        } else {
+            dim3 grid(snx * ((sny + 255) / 256), m);
+            e = cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)
+                     : yadj_q<false>(Q, C, grid, st, u, s, g, v, unit0 + off);
+        }
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

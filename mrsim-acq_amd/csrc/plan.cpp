@@ -23,8 +23,9 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
                               const WorkItem* w, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
-hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, const DevUnit* u,
-                            const DevSlice* s, const DevGroup* g, const VjpUnit* v, hipStream_t st);
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, bool mfma,
+                            const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
+                            hipStream_t st);
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st);
 hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st);
@@ -1088,13 +1089,16 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
         HIP_TRY(hipMemcpy(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
         p->vjp_fresh = false;
     }
-    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, PREP_S32, st));
+    // split-BF16 plans take the y-adjoint's coil contraction on MFMA (split coil planes), FP32 plans
+    // keep it in FP32 on the VALU
+    const bool mfma = p->precision == MRS_PREC_SPLIT_BF16;
+    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, mfma ? PREP_SPLIT : PREP_S32, st));
     size_t u0 = 0;
     while (u0 < n) {
         size_t u1 = u0 + 1;
         while (u1 < n && p->units[u1].group == p->units[u0].group) u1++;
         const DevGroup& dg = p->groups[p->units[u0].group].dev;
-        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.snx, dg.sny, dg.LyP,
+        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.snx, dg.sny, dg.LyP, mfma,
                                  p->d_units, p->d_slices, p->d_groups, p->d_vjp_units, st));
         u0 = u1;
     }
