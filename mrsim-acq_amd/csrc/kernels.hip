@@ -1116,6 +1116,97 @@ __global__ __launch_bounds__(256) void xadjoint(const DevUnit* __restrict__ unit
     }
 }
 
+// x-adjoint on MFMA (split-BF16 plans): Gb[x, n] = sum_kx conj(T_tab[kx, x]) g[kx, n], n = line*C + coil,
+// the forward x-stage with the roles of x and kx exchanged.  A block stages the conjugated, transposed
+// twiddle rows of its 16*MT x (split hi and lo, both ghost tables when the group has a ghost) in LDS
+// from the FP32 tables; each wave streams its 16-column tiles of the cotangent, 16 kx per step (a lane
+// reads 4 consecutive kx of one column and splits them), and the complex product rides on B as in the
+// forward: B_re = [g_re, -g_im] gives Re Gb, B_im = [g_im, g_re] gives Im Gb.  Padded lines (ky < 0)
+// read no cotangent and write zero rows.  grid = (ceil(snx / 16 MT) * nparts, units).
+template <int MT>
+__global__ __launch_bounds__(256) void xadjoint_mfma(const DevUnit* __restrict__ units,
+                                                     const DevGroup* __restrict__ groups,
+                                                     const VjpUnit* __restrict__ vj, int unit0, int nparts) {
+    extern __shared__ uint32_t tla[];
+    const int unit = unit0 + blockIdx.y;
+    const DevUnit& u = units[unit];
+    const VjpUnit& v = vj[unit];
+    const DevGroup& gr = groups[u.group];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lq = lane >> 4, ln = lane & 15;
+    const int C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15, nx = gr.nx, ny = gr.ny;
+    const int nkP = (nx + 15) & ~15;              // kx padded to whole 16-kx steps
+    const int stride = nkP + 4;                   // u32 per LDS row (16-byte pad against bank conflicts)
+    constexpr int rows = 16 * MT;
+    const int xb = blockIdx.x / nparts, part = blockIdx.x % nparts;
+    const int x0 = xb * rows;
+    const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
+    // stage A[x][kx] = conj(T[kx][x]), split; reads run along x (coalesced), rows past snx or kx past
+    // nx are zero
+    const int per_tab = rows * nkP;
+    for (int tb = 0; tb < ntab; tb++) {
+        const float2* __restrict__ t32 = gr.t32[tb];
+        for (int i = threadIdx.x; i < per_tab; i += 256) {
+            const int r = i % rows, kx = i / rows;
+            const int x = x0 + r;
+            uint32_t h = 0u, l = 0u;
+            if (x < snx && kx < nx) {
+                const float2 t = t32[(size_t)kx * snxP + x];
+                split2(t.x, -t.y, h, l);
+            }
+            tla[((tb * 2 + 0) * rows + r) * stride + kx] = h;
+            tla[((tb * 2 + 1) * rows + r) * stride + kx] = l;
+        }
+    }
+    __syncthreads();
+    const int ntiles = (LyP * C) >> 4;
+    const int first = part * 4 + wave, step = nparts * 4;
+    const size_t cstride = (size_t)ny * nx;
+    for (int nt = first; nt < ntiles; nt += step) {
+        const int n = nt * 16 + ln;
+        const int lp = n / C, c = n % C;
+        const LineRec& L = gr.lines[lp];
+        const int tb = ntab == 2 ? gr.lines[(nt * 16) / C].table : 0;   // a tile never spans two tables
+        const uint32_t* __restrict__ th = tla + (tb * 2 + 0) * rows * stride;
+        const uint32_t* __restrict__ tlo = tla + (tb * 2 + 1) * rows * stride;
+        const bool ok = L.ky >= 0 && v.gout;
+        const f32x2* __restrict__ g = reinterpret_cast<const f32x2*>(v.gout) + (ok ? (size_t)c * cstride + (size_t)L.ky * nx : 0);
+        f32x4 accr[MT], acci[MT];
+#pragma unroll
+        for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < nkP; k0 += 16) {
+            const int kx = k0 + lq * 4;
+            u32x4 bhi, blo;
+#pragma unroll
+            for (int jj = 0; jj < 4; jj++) {
+                const f32x2 gv = (ok && kx + jj < nx) ? g[kx + jj] : f32x2{0.f, 0.f};
+                uint32_t h, l;
+                split2(gv.x, gv.y, h, l);
+                bhi[jj] = h;
+                blo[jj] = l;
+            }
+            const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+            const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+#pragma unroll
+            for (int m = 0; m < MT; m++) {
+                const int r = m * 16 + ln;
+                const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + kx);
+                const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + kx);
+                accr[m] = mfma3(h, l, brh, brl, accr[m]);
+                acci[m] = mfma3(h, l, bih, bil, acci[m]);
+            }
+        }
+        // accr[m][r]: row x0 + 16 m + 4 lq + r, column n
+#pragma unroll
+        for (int m = 0; m < MT; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = x0 + m * 16 + 4 * lq + r;
+                if (x < snx) v.gbar[((size_t)x * LyP + lp) * C + c] = make_float2(accr[m][r], acci[m][r]);
+            }
+    }
+}
+
 // y-adjoint: one lane per voxel (x, y), looping over the lines; a block owns 256 y of one column x,
 // and each 64-line pass of the column's Gb (every lane reads the same values) is staged in LDS.
 // grid = (snx * ceil(sny / 256), units).  With Eb = sum_c conj(S_c) Gb_c,
@@ -2162,16 +2253,32 @@ static hipError_t yadjm_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit
     }
 }
 
-// Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the y-adjoint's coil
-// contraction as split-BF16 MFMA (needs the split coil planes), else FP32 on the VALU (the FP32 ones).
-hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int LyP, bool mfma,
-                            const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
+// Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the x-adjoint and the
+// y-adjoint's coil contraction as split-BF16 MFMA (the y-adjoint needs the split coil planes), else
+// FP32 on the VALU (the y-adjoint reads the FP32 planes).
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int nx, int LyP, int ntab,
+                            bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
                             hipStream_t st) {
+    // MFMA x-adjoint tiling: 16*MT x rows per block with their twiddle rows within 64 KiB of LDS, the
+    // unit's column tiles split over enough parts for about 1024 blocks per launch
+    const int nkP = (nx + 15) & ~15;
+    auto lds_of = [&](int mt) { return (size_t)ntab * 2 * 16 * mt * (nkP + 4) * 4; };
+    int mt = 4;
+    while (mt > 1 && lds_of(mt) > 64 * 1024) mt /= 2;
+    const int nxb = (snx + 16 * mt - 1) / (16 * mt), ntiles = LyP * C / 16;
     // units ride on a grid dimension of at most 65535 blocks
     for (int off = 0; off < n; off += 65535) {
         const int m = std::min(65535, n - off);
-        hipLaunchKernelGGL(xadjoint, dim3((snx + 63) / 64, (LyP / 4) * ((C + XA_CG - 1) / XA_CG), m), dim3(256), 0, st,
-                           u, g, v, unit0 + off);
+        if (mfma) {
+            const int np = std::max(1, std::min(std::max(1, ntiles / 4), 1024 / std::max(1, m * nxb) + 1));
+            const dim3 grid(nxb * np, m);
+            if (mt == 4) hipLaunchKernelGGL((xadjoint_mfma<4>), grid, dim3(256), lds_of(4), st, u, g, v, unit0 + off, np);
+            else if (mt == 2) hipLaunchKernelGGL((xadjoint_mfma<2>), grid, dim3(256), lds_of(2), st, u, g, v, unit0 + off, np);
+            else hipLaunchKernelGGL((xadjoint_mfma<1>), grid, dim3(256), lds_of(1), st, u, g, v, unit0 + off, np);
+        } else {
+            hipLaunchKernelGGL(xadjoint, dim3((snx + 63) / 64, (LyP / 4) * ((C + XA_CG - 1) / XA_CG), m), dim3(256), 0,
+                               st, u, g, v, unit0 + off);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (mfma) {
