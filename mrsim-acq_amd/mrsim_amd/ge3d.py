@@ -206,8 +206,16 @@ class Ge3dForward:
                 shape = (len(self.tes), max(len(self._used[vi]), 1), v.nz, v.snx * v.sny)
                 dev = v.fmap.device
                 per_vol.append([torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3)])
-            self._vjp_buf = (gk2, per_vol)
-        gk2, per_vol = self._vjp_buf
+            # the per-unit views stay fixed with the buffers: build them once
+            views = ([], [], [], [])
+            for vi, e, ci, z in self._umeta:
+                b = per_vol[vi]
+                views[0].append(gk2[(vi, e)][ci, z])
+                views[1].append(b[0][e, ci, z])
+                views[2].append(b[1][e, ci, z])
+                views[3].append(b[2][e, ci, z])
+            self._vjp_buf = (gk2, per_vol, views)
+        gk2, per_vol, (gout, gimg, gfm, gti) = self._vjp_buf
         sets = []
         for (vi, e), (n_used, w) in self._zadj.items():
             v = self.volumes[vi]
@@ -218,13 +226,6 @@ class Ge3dForward:
             sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
                              n_comp=n_used, ky_segments=self.ky_segments, weights=w))
         zstage(sets, adjoint=True, stream=stream)
-        gout, gimg, gfm, gti = [], [], [], []
-        for vi, e, ci, z in self._umeta:
-            gout.append(gk2[(vi, e)][ci, z])
-            b = per_vol[vi]
-            gimg.append(b[0][e, ci, z])
-            gfm.append(b[1][e, ci, z])
-            gti.append(b[2][e, ci, z])
         self.plan.vjp(gout, gimg, gfm, gti, stream)
         out = []
         for vi, v in enumerate(self.volumes):
