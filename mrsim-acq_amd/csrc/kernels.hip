@@ -1359,27 +1359,47 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
 // there and adds the same sums as the VALU y-adjoint; the sums over the 16 lines of a tile sit in 16
 // lanes and are added across them at the end.  Block = 4 waves on one (column, unit), a wave owning
 // the y tiles w, w + 4, ...; NCS 16-coil steps cover C <= 16 NCS.  grid = (snx, units).
+// The block first splits the column's Gb (LyP lines x C coils) once into LDS, hi and lo planes of
+// rows of 16 NCS + 4 words (zero coils past C), so the waves read their B operands from LDS; the
+// contraction of line tile g + 1 is issued before the VALU work of tile g.
+// Dynamic LDS: yadjoint_mfma_lds(LyP, NCS) bytes.
+__host__ __device__ constexpr size_t yadjoint_mfma_lds(int LyP, int ncs) {
+    return (size_t)LyP * sizeof(LineRec) + 2 * (size_t)LyP * (16 * ncs + 4) * 4;
+}
+
 template <int Q, bool CPLX, int NCS>
 __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__ units,
                                                      const DevSlice* __restrict__ slices,
                                                      const DevGroup* __restrict__ groups,
                                                      const VjpUnit* __restrict__ vj, int unit0) {
-    __shared__ LineRec lines_s[256];
+    constexpr int CW = NCS * 16, RS = CW + 4;
+    extern __shared__ __align__(16) unsigned char yadj_smem[];
     const int unit = unit0 + blockIdx.y;
     const DevUnit& u = units[unit];
     const VjpUnit& vv = vj[unit];
     const DevGroup& gr = groups[u.group];
     const DevSlice& sl = slices[u.slice];
-    for (int i = threadIdx.x; i < gr.LyP; i += 256) lines_s[i] = gr.lines[i];
-    __syncthreads();
     const int snx = gr.snx, sny = gr.sny, snxP = (snx + 15) & ~15, snyP = (sny + 15) & ~15;
     const int C = gr.C, LyP = gr.LyP, G = LyP >> 4, Cp = gr.RB * 8;
     const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
     const int x = blockIdx.x;
+    LineRec* lines_s = reinterpret_cast<LineRec*>(yadj_smem);
+    uint32_t* bsh = reinterpret_cast<uint32_t*>(yadj_smem + (size_t)LyP * sizeof(LineRec));
+    uint32_t* bsl = bsh + (size_t)LyP * RS;
+    const float2* __restrict__ gb = vv.gbar + (size_t)x * LyP * C;
+    for (int i = threadIdx.x; i < LyP; i += 256) lines_s[i] = gr.lines[i];
+    for (int e = threadIdx.x; e < LyP * CW; e += 256) {
+        const int l = e / CW, c = e % CW;
+        const float2 v = c < C ? gb[(size_t)l * C + c] : make_float2(0.f, 0.f);
+        uint32_t h, lo;
+        split2(v.x, v.y, h, lo);
+        bsh[l * RS + c] = h;
+        bsl[l * RS + c] = lo;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lq = lane >> 4, ln = lane & 15;
     const int nyt = (sny + 15) >> 4;
-    const float2* __restrict__ gb = vv.gbar + (size_t)x * LyP * C;
     for (int yt = wave; yt < nyt; yt += 4) {
         // A rows y = 16 yt + ln (< snyP; the split planes hold zero coils past C and zero rows past sny)
         u32x4 arh[NCS], arl[NCS], aih[NCS], ail[NCS];
@@ -1422,26 +1442,25 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
         for (int q = 0; q < Q; q++)
 #pragma unroll
             for (int j = 0; j < 4; j++) ar[q][j] = ai[q][j] = br[q][j] = bi[q][j] = fr[q][j] = fi[q][j] = 0.0f;
-        for (int g = 0; g < G; g++) {
-            const int lp = g * 16 + ln;
-            u32x4 bh[NCS], bl[NCS];
-#pragma unroll
-            for (int s = 0; s < NCS; s++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int c = s * 16 + 4 * lq + j;
-                    const float2 v = c < C ? gb[(size_t)lp * C + c] : make_float2(0.f, 0.f);
-                    uint32_t h, l;
-                    split2(v.x, v.y, h, l);
-                    bh[s][j] = h;
-                    bl[s][j] = l;
-                }
-            f32x4 hr = f32x4{0.f, 0.f, 0.f, 0.f}, hi = f32x4{0.f, 0.f, 0.f, 0.f};
+        // H of line tile g: B = the split Gb rows of lines 16 g .. 16 g + 15 (lane column ln)
+        auto contract = [&](int g, f32x4& hr, f32x4& hi) {
+            const int o = (g * 16 + ln) * RS + 4 * lq;
+            hr = f32x4{0.f, 0.f, 0.f, 0.f};
+            hi = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < NCS; s++) {
-                hr = mfma3(arh[s], arl[s], bh[s], bl[s], hr);
-                hi = mfma3(aih[s], ail[s], bh[s], bl[s], hi);
+                const u32x4 bh = *reinterpret_cast<const u32x4*>(bsh + o + 16 * s);
+                const u32x4 bl = *reinterpret_cast<const u32x4*>(bsl + o + 16 * s);
+                hr = mfma3(arh[s], arl[s], bh, bl, hr);
+                hi = mfma3(aih[s], ail[s], bh, bl, hi);
             }
+        };
+        f32x4 hr, hi;
+        contract(0, hr, hi);
+        for (int g = 0; g < G; g++) {
+            const int lp = g * 16 + ln;
+            f32x4 nhr = hr, nhi = hi;
+            if (g + 1 < G) contract(g + 1, nhr, nhi);
             const LineRec& L = lines_s[lp];
             if (L.ky >= 0) {
                 const float trf = L.trf, tau = L.tau, tb0 = L.tb0;
@@ -1467,6 +1486,8 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
                     }
                 }
             }
+            hr = nhr;
+            hi = nhi;
         }
         // sums over the tile's 16 lines (lanes ln of one row group), then lane ln = j writes voxel y0 + j
         auto rsum = [](float v) {
@@ -2233,24 +2254,48 @@ static hipError_t yadj_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit*
 }
 
 template <int Q, bool CPLX>
-static hipError_t yadjm_c(int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+static hipError_t yadjm_c(int C, int LyP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
                           const VjpUnit* v, int unit0) {
-    if (C <= 16) hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 1>), grid, dim3(256), 0, st, u, s, g, v, unit0);
-    else if (C <= 32) hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 2>), grid, dim3(256), 0, st, u, s, g, v, unit0);
-    else hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 4>), grid, dim3(256), 0, st, u, s, g, v, unit0);
+    if (C <= 16)
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 1>), grid, dim3(256), yadjoint_mfma_lds(LyP, 1), st, u, s, g, v, unit0);
+    else if (C <= 32)
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 2>), grid, dim3(256), yadjoint_mfma_lds(LyP, 2), st, u, s, g, v, unit0);
+    else
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 4>), grid, dim3(256), yadjoint_mfma_lds(LyP, 4), st, u, s, g, v, unit0);
     return hipGetLastError();
 }
 
+template <int Q, bool CPLX>
+static hipError_t yadjm_lds_limit(int bytes) {
+    hipError_t e;
+    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes)))
+        return e;
+    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes)))
+        return e;
+    return hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
 template <bool CPLX>
-static hipError_t yadjm_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+static hipError_t yadjm_q(int Q, int C, int LyP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
                           const DevGroup* g, const VjpUnit* v, int unit0) {
     switch (Q) {
-    case 1: return yadjm_c<1, CPLX>(C, grid, st, u, s, g, v, unit0);
-    case 2: return yadjm_c<2, CPLX>(C, grid, st, u, s, g, v, unit0);
-    case 3: return yadjm_c<3, CPLX>(C, grid, st, u, s, g, v, unit0);
-    case 4: return yadjm_c<4, CPLX>(C, grid, st, u, s, g, v, unit0);
+    case 1: return yadjm_c<1, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 2: return yadjm_c<2, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 3: return yadjm_c<3, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 4: return yadjm_c<4, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
     default: return hipErrorInvalidValue;
     }
+}
+
+// The MFMA y-adjoint's LDS (lines + split Gb) reaches 151.5 KiB at 256 lines and 64 coils.
+hipError_t set_yadjoint_lds_limit(size_t bytes) {
+    hipError_t e;
+    const int b = (int)bytes;
+    if ((e = yadjm_lds_limit<1, false>(b)) || (e = yadjm_lds_limit<2, false>(b)) || (e = yadjm_lds_limit<3, false>(b)) ||
+        (e = yadjm_lds_limit<4, false>(b)) || (e = yadjm_lds_limit<1, true>(b)) || (e = yadjm_lds_limit<2, true>(b)) ||
+        (e = yadjm_lds_limit<3, true>(b)))
+        return e;
+    return yadjm_lds_limit<4, true>(b);
 }
 
 // Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the x-adjoint and the
@@ -2283,8 +2328,8 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, 
         if (e != hipSuccess) return e;
         if (mfma) {
             dim3 grid(snx, m);
-            e = cplx ? yadjm_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)
-                     : yadjm_q<false>(Q, C, grid, st, u, s, g, v, unit0 + off);
+            e = cplx ? yadjm_q<true>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off)
+                     : yadjm_q<false>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off);
         } else {
             dim3 grid(snx * ((sny + 255) / 256), m);
             e = cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)

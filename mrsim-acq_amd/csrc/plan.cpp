@@ -23,6 +23,7 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
                               const WorkItem* w, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
+hipError_t set_yadjoint_lds_limit(size_t bytes);
 hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int nx, int LyP, int ntab,
                             bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
                             hipStream_t st);
@@ -785,8 +786,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             g_np[gi] = np;
             g_lds[gi] = lds_of(mt);
         }
-        if (set_xstage_lds_limit(160 * 1024) != hipSuccess)
-            rc = fail(err, errlen, MRS_E_DEVICE, "cannot raise the x-stage LDS limit");
+        if (set_xstage_lds_limit(160 * 1024) != hipSuccess || set_yadjoint_lds_limit(160 * 1024) != hipSuccess)
+            rc = fail(err, errlen, MRS_E_DEVICE, "cannot raise the x-stage and y-adjoint LDS limits");
         for (size_t k = 0; k < nunits && rc == MRS_OK; k++) {
             const size_t u = order[k];
             const int gi = p->units[u].group;
