@@ -1362,13 +1362,17 @@ __global__ __launch_bounds__(256) void yadjoint(const DevUnit* __restrict__ unit
 // The block first splits the column's Gb (LyP lines x C coils) once into LDS, hi and lo planes of
 // rows of 16 NCS + 4 words (zero coils past C), so the waves read their B operands from LDS; the
 // contraction of line tile g + 1 is issued before the VALU work of tile g.
+// EDDY: the group has an eddy-current phase (the poly plane); without it that term is zero.
 // Dynamic LDS: yadjoint_mfma_lds(LyP, NCS) bytes.
 __host__ __device__ constexpr size_t yadjoint_mfma_lds(int LyP, int ncs) {
     return (size_t)LyP * sizeof(LineRec) + 2 * (size_t)LyP * (16 * ncs + 4) * 4;
 }
 
-template <int Q, bool CPLX, int NCS>
-__global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__ units,
+// Real single-compartment images with up to 32 coils fit 128 VGPRs: four waves per SIMD.
+constexpr int yadjoint_mfma_waves(int Q, bool CPLX, int NCS) { return (Q == 1 && !CPLX && NCS <= 2) ? 4 : 1; }
+
+template <int Q, bool CPLX, int NCS, bool EDDY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(yadjoint_mfma_waves(Q, CPLX, NCS)))) void yadjoint_mfma(const DevUnit* __restrict__ units,
                                                      const DevSlice* __restrict__ slices,
                                                      const DevGroup* __restrict__ groups,
                                                      const VjpUnit* __restrict__ vj, int unit0) {
@@ -1402,7 +1406,7 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
     const int nyt = (sny + 15) >> 4;
     for (int yt = wave; yt < nyt; yt += 4) {
         // A rows y = 16 yt + ln (< snyP; the split planes hold zero coils past C and zero rows past sny)
-        u32x4 arh[NCS], arl[NCS], aih[NCS], ail[NCS];
+        u32x4 arh[NCS], arl[NCS];
         const int ya = yt * 16 + ln;
 #pragma unroll
         for (int s = 0; s < NCS; s++)
@@ -1417,8 +1421,6 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
                 }
                 arh[s][j] = h;
                 arl[s][j] = l;
-                aih[s][j] = __builtin_rotateright32(h, 16) ^ 0x8000u;
-                ail[s][j] = __builtin_rotateright32(l, 16) ^ 0x8000u;
             }
         // the lane's voxels y0 + j (rows of its accumulator tile; the planes are padded to snyP)
         const int y0 = yt * 16 + 4 * lq;
@@ -1428,7 +1430,7 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
         for (int j = 0; j < 4; j++) {
             const size_t o = (size_t)x * snyP + y0 + j;
             base[j] = sl.base[o];
-            poly[j] = sl.poly[o];
+            poly[j] = EDDY ? sl.poly[o] : 0.0;
             rate[j] = sl.rate[o];
 #pragma unroll
             for (int q = 0; q < Q; q++) {
@@ -1451,16 +1453,21 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
             for (int s = 0; s < NCS; s++) {
                 const u32x4 bh = *reinterpret_cast<const u32x4*>(bsh + o + 16 * s);
                 const u32x4 bl = *reinterpret_cast<const u32x4*>(bsl + o + 16 * s);
+                // imaginary rows [-S_im, S_re], made here rather than held in registers
+                u32x4 aih, ail;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    aih[j] = __builtin_rotateright32(arh[s][j], 16) ^ 0x8000u;
+                    ail[j] = __builtin_rotateright32(arl[s][j], 16) ^ 0x8000u;
+                }
                 hr = mfma3(arh[s], arl[s], bh, bl, hr);
-                hi = mfma3(aih[s], ail[s], bh, bl, hi);
+                hi = mfma3(aih, ail, bh, bl, hi);
             }
         };
-        f32x4 hr, hi;
-        contract(0, hr, hi);
         for (int g = 0; g < G; g++) {
             const int lp = g * 16 + ln;
-            f32x4 nhr = hr, nhi = hi;
-            if (g + 1 < G) contract(g + 1, nhr, nhi);
+            f32x4 hr, hi;
+            contract(g, hr, hi);
             const LineRec& L = lines_s[lp];
             if (L.ky >= 0) {
                 const float trf = L.trf, tau = L.tau, tb0 = L.tb0;
@@ -1468,7 +1475,7 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
                 for (int j = 0; j < 4; j++) {
                     double cyc = __builtin_fma(L.kyn, yy0 + (double)j, base[j]);
                     cyc = __builtin_fma((double)rate[j], L.t_s, cyc);
-                    cyc = __builtin_fma(poly[j], L.d, cyc);
+                    if constexpr (EDDY) cyc = __builtin_fma(poly[j], L.d, cyc);
                     float ps, pc;
                     sincos_turns(cyc, ps, pc);
                     // t = conj(P) H
@@ -1476,18 +1483,19 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
 #pragma unroll
                     for (int q = 0; q < Q; q++) {
                         const float d = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, nr2[q][j], tau * nri[q][j]));
-                        ar[q][j] = __builtin_fmaf(d, tr, ar[q][j]);
-                        ai[q][j] = __builtin_fmaf(d, ti, ai[q][j]);
                         const float dt = d * tau, df = d * tb0;
+                        // real images take Re of the image and T2' sums and Im of the fieldmap sum only
+                        ar[q][j] = __builtin_fmaf(d, tr, ar[q][j]);
                         br[q][j] = __builtin_fmaf(dt, tr, br[q][j]);
-                        bi[q][j] = __builtin_fmaf(dt, ti, bi[q][j]);
-                        fr[q][j] = __builtin_fmaf(df, tr, fr[q][j]);
                         fi[q][j] = __builtin_fmaf(df, ti, fi[q][j]);
+                        if constexpr (CPLX) {
+                            ai[q][j] = __builtin_fmaf(d, ti, ai[q][j]);
+                            bi[q][j] = __builtin_fmaf(dt, ti, bi[q][j]);
+                            fr[q][j] = __builtin_fmaf(df, tr, fr[q][j]);
+                        }
                     }
                 }
             }
-            hr = nhr;
-            hi = nhi;
         }
         // sums over the tile's 16 lines (lanes ln of one row group), then lane ln = j writes voxel y0 + j
         auto rsum = [](float v) {
@@ -1531,7 +1539,7 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
             if (vv.gfmap) {
                 float acc = 0.f;
 #pragma unroll
-                for (int q = 0; q < Q; q++) acc += m_r[q] * ofi[q] - m_i[q] * ofr[q];
+                for (int q = 0; q < Q; q++) acc += CPLX ? m_r[q] * ofi[q] - m_i[q] * ofr[q] : m_r[q] * ofi[q];
                 vv.gfmap[i] = gr.do_dist ? (float)(kTau * (double)acc) : 0.0f;
             }
             if (vv.gti) {
@@ -1539,7 +1547,7 @@ __global__ __launch_bounds__(256) void yadjoint_mfma(const DevUnit* __restrict__
                 for (int q = 0; q < Q; q++) {
                     const float ti_ms = sl.t_inhom ? sl.t_inhom[q * P + i]
                                                    : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
-                    const float re = m_r[q] * obr[q] + m_i[q] * obi[q];
+                    const float re = CPLX ? m_r[q] * obr[q] + m_i[q] * obi[q] : m_r[q] * obr[q];
                     vv.gti[q * P + i] = gr.do_relax ? re / (ti_ms * ti_ms) : 0.0f;
                 }
             }
@@ -2253,36 +2261,43 @@ static hipError_t yadj_q(int Q, int C, dim3 grid, hipStream_t st, const DevUnit*
     }
 }
 
-template <int Q, bool CPLX>
-static hipError_t yadjm_c(int C, int LyP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s, const DevGroup* g,
-                          const VjpUnit* v, int unit0) {
+template <int Q, bool CPLX, bool EDDY>
+static hipError_t yadjm_c(int C, int LyP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
+                          const DevGroup* g, const VjpUnit* v, int unit0) {
     if (C <= 16)
-        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 1>), grid, dim3(256), yadjoint_mfma_lds(LyP, 1), st, u, s, g, v, unit0);
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 1, EDDY>), grid, dim3(256), yadjoint_mfma_lds(LyP, 1), st, u, s, g, v,
+                           unit0);
     else if (C <= 32)
-        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 2>), grid, dim3(256), yadjoint_mfma_lds(LyP, 2), st, u, s, g, v, unit0);
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 2, EDDY>), grid, dim3(256), yadjoint_mfma_lds(LyP, 2), st, u, s, g, v,
+                           unit0);
     else
-        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 4>), grid, dim3(256), yadjoint_mfma_lds(LyP, 4), st, u, s, g, v, unit0);
+        hipLaunchKernelGGL((yadjoint_mfma<Q, CPLX, 4, EDDY>), grid, dim3(256), yadjoint_mfma_lds(LyP, 4), st, u, s, g, v,
+                           unit0);
     return hipGetLastError();
 }
 
-template <int Q, bool CPLX>
+template <int Q, bool CPLX, bool EDDY>
 static hipError_t yadjm_lds_limit(int bytes) {
     hipError_t e;
-    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes)))
-        return e;
-    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 2>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes)))
-        return e;
-    return hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    const auto attr = hipFuncAttributeMaxDynamicSharedMemorySize;
+    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 1, EDDY>, attr, bytes))) return e;
+    if ((e = hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 2, EDDY>, attr, bytes))) return e;
+    return hipFuncSetAttribute((const void*)yadjoint_mfma<Q, CPLX, 4, EDDY>, attr, bytes);
+}
+template <int Q, bool CPLX>
+static hipError_t yadjm_lds_limit(int bytes) {
+    const hipError_t e = yadjm_lds_limit<Q, CPLX, false>(bytes);
+    return e ? e : yadjm_lds_limit<Q, CPLX, true>(bytes);
 }
 
-template <bool CPLX>
+template <bool CPLX, bool EDDY>
 static hipError_t yadjm_q(int Q, int C, int LyP, dim3 grid, hipStream_t st, const DevUnit* u, const DevSlice* s,
                           const DevGroup* g, const VjpUnit* v, int unit0) {
     switch (Q) {
-    case 1: return yadjm_c<1, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
-    case 2: return yadjm_c<2, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
-    case 3: return yadjm_c<3, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
-    case 4: return yadjm_c<4, CPLX>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 1: return yadjm_c<1, CPLX, EDDY>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 2: return yadjm_c<2, CPLX, EDDY>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 3: return yadjm_c<3, CPLX, EDDY>(C, LyP, grid, st, u, s, g, v, unit0);
+    case 4: return yadjm_c<4, CPLX, EDDY>(C, LyP, grid, st, u, s, g, v, unit0);
     default: return hipErrorInvalidValue;
     }
 }
@@ -2301,8 +2316,8 @@ hipError_t set_yadjoint_lds_limit(size_t bytes) {
 // Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the x-adjoint and the
 // y-adjoint's coil contraction as split-BF16 MFMA (the y-adjoint needs the split coil planes), else
 // FP32 on the VALU (the y-adjoint reads the FP32 planes).
-hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int nx, int LyP, int ntab,
-                            bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy, int snx, int sny, int nx, int LyP,
+                            int ntab, bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
                             hipStream_t st) {
     // MFMA x-adjoint tiling: 16*MT x rows per block with their twiddle rows within 64 KiB of LDS, the
     // unit's column tiles split over enough parts for about 1024 blocks per launch
@@ -2328,8 +2343,12 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, 
         if (e != hipSuccess) return e;
         if (mfma) {
             dim3 grid(snx, m);
-            e = cplx ? yadjm_q<true>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off)
-                     : yadjm_q<false>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off);
+            if (eddy)
+                e = cplx ? yadjm_q<true, true>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off)
+                         : yadjm_q<false, true>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off);
+            else
+                e = cplx ? yadjm_q<true, false>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off)
+                         : yadjm_q<false, false>(Q, C, LyP, grid, st, u, s, g, v, unit0 + off);
         } else {
             dim3 grid(snx * ((sny + 255) / 256), m);
             e = cplx ? yadj_q<true>(Q, C, grid, st, u, s, g, v, unit0 + off)

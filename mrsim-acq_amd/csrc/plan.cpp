@@ -24,8 +24,8 @@ hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const D
                               const WorkItem* w, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
 hipError_t set_yadjoint_lds_limit(size_t bytes);
-hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, int snx, int sny, int nx, int LyP, int ntab,
-                            bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
+hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy, int snx, int sny, int nx, int LyP,
+                            int ntab, bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
                             hipStream_t st);
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st);
@@ -1099,8 +1099,8 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
         size_t u1 = u0 + 1;
         while (u1 < n && p->units[u1].group == p->units[u0].group) u1++;
         const DevGroup& dg = p->groups[p->units[u0].group].dev;
-        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.snx, dg.sny, dg.nx, dg.LyP,
-                                 dg.t_hi[1] != dg.t_hi[0] ? 2 : 1, mfma, p->d_units, p->d_slices, p->d_groups,
+        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.eddy_strength != 0.0, dg.snx,
+                                 dg.sny, dg.nx, dg.LyP, dg.t_hi[1] != dg.t_hi[0] ? 2 : 1, mfma, p->d_units, p->d_slices, p->d_groups,
                                  p->d_vjp_units, st));
         u0 = u1;
     }
