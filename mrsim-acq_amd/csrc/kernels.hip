@@ -1122,13 +1122,21 @@ __global__ __launch_bounds__(256) void xadjoint(const DevUnit* __restrict__ unit
 // from the FP32 tables; each wave streams its 16-column tiles of the cotangent, 16 kx per step (a lane
 // reads 4 consecutive kx of one column and splits them), and the complex product rides on B as in the
 // forward: B_re = [g_re, -g_im] gives Re Gb, B_im = [g_im, g_re] gives Im Gb.  Padded lines (ky < 0)
-// read no cotangent and write zero rows.  grid = (ceil(snx / 16 MT) * nparts, units).
+// read no cotangent and write zero rows.
+// Work w = ((unit - unit0) * nparts + part) * nxb + xb, w < nwork: the nxb x blocks of one (unit, part)
+// read the same cotangent columns, so they take consecutive dispatch slots of one XCD (blocks go
+// round-robin over the 8 XCDs): block b runs w = (b % 8) * (gridDim.x / 8) + b / 8, and the grid is a
+// multiple of 8 blocks.
 template <int MT>
 __global__ __launch_bounds__(256) void xadjoint_mfma(const DevUnit* __restrict__ units,
                                                      const DevGroup* __restrict__ groups,
-                                                     const VjpUnit* __restrict__ vj, int unit0, int nparts) {
+                                                     const VjpUnit* __restrict__ vj, int unit0, int nparts, int nxb,
+                                                     int nwork) {
     extern __shared__ uint32_t tla[];
-    const int unit = unit0 + blockIdx.y;
+    const int w = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+    if (w >= nwork) return;
+    const int xb = w % nxb, part = (w / nxb) % nparts;
+    const int unit = unit0 + w / (nxb * nparts);
     const DevUnit& u = units[unit];
     const VjpUnit& v = vj[unit];
     const DevGroup& gr = groups[u.group];
@@ -1138,7 +1146,6 @@ __global__ __launch_bounds__(256) void xadjoint_mfma(const DevUnit* __restrict__
     const int nkP = (nx + 15) & ~15;              // kx padded to whole 16-kx steps
     const int stride = nkP + 4;                   // u32 per LDS row (16-byte pad against bank conflicts)
     constexpr int rows = 16 * MT;
-    const int xb = blockIdx.x / nparts, part = blockIdx.x % nparts;
     const int x0 = xb * rows;
     const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
     // stage A[x][kx] = conj(T[kx][x]), split; reads run along x (coalesced), rows past snx or kx past
@@ -2331,10 +2338,14 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy
         const int m = std::min(65535, n - off);
         if (mfma) {
             const int np = std::max(1, std::min(std::max(1, ntiles / 4), 1024 / std::max(1, m * nxb) + 1));
-            const dim3 grid(nxb * np, m);
-            if (mt == 4) hipLaunchKernelGGL((xadjoint_mfma<4>), grid, dim3(256), lds_of(4), st, u, g, v, unit0 + off, np);
-            else if (mt == 2) hipLaunchKernelGGL((xadjoint_mfma<2>), grid, dim3(256), lds_of(2), st, u, g, v, unit0 + off, np);
-            else hipLaunchKernelGGL((xadjoint_mfma<1>), grid, dim3(256), lds_of(1), st, u, g, v, unit0 + off, np);
+            const int nwork = m * np * nxb;
+            const dim3 grid((nwork + 7) / 8 * 8);
+            if (mt == 4)
+                hipLaunchKernelGGL((xadjoint_mfma<4>), grid, dim3(256), lds_of(4), st, u, g, v, unit0 + off, np, nxb, nwork);
+            else if (mt == 2)
+                hipLaunchKernelGGL((xadjoint_mfma<2>), grid, dim3(256), lds_of(2), st, u, g, v, unit0 + off, np, nxb, nwork);
+            else
+                hipLaunchKernelGGL((xadjoint_mfma<1>), grid, dim3(256), lds_of(1), st, u, g, v, unit0 + off, np, nxb, nwork);
         } else {
             hipLaunchKernelGGL(xadjoint, dim3((snx + 63) / 64, (LyP / 4) * ((C + XA_CG - 1) / XA_CG), m), dim3(256), 0,
                                st, u, g, v, unit0 + off);
