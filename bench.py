@@ -197,7 +197,9 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
     dt = (time.perf_counter() - t0) / steps
     st = fw.stage_times(iters=3)
     # forward + adjoint: the forward, then the vector-Jacobian product of a seeded complex cotangent
-    # (partition-transform adjoint, per-unit adjoints, sums over echoes)
+    # (partition-transform adjoint, per-unit adjoints, sums over echoes); the adjoint reads the slice
+    # planes the forward just built (as a backward pass reads saved activations), and the plain
+    # adjoint, which rebuilds them, is timed beside it
     rng = np.random.default_rng(5)
     gout = [[torch.from_numpy((rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
                               .astype(np.complex64)).cuda() for o in ov] for ov in fw.out]
@@ -208,9 +210,15 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
     t0 = time.perf_counter()
     for _ in range(nfa):
         fw.forward_graph()
-        fw.vjp(gout)
+        fw.vjp(gout, after_forward=True)
     torch.cuda.synchronize()
     dt_fa = (time.perf_counter() - t0) / nfa
+    t0 = time.perf_counter()
+    for _ in range(nfa):
+        fw.forward_graph()
+        fw.vjp(gout)
+    torch.cuda.synchronize()
+    dt_fa_rebuild = (time.perf_counter() - t0) / nfa
     n_e = len(w.echo_times_ms)
     parts = sum(v.nz for v in w.volumes) * n_e
     # partition transform traffic: read K2 and write K3, complex64, every (volume, echo)
@@ -230,6 +238,7 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
             "ms_per_train": round(1e3 * dt, 4), "partition_slices_per_s": round(parts / dt, 1),
             "volumes_x_echoes_per_s": round(len(w.volumes) * n_e / dt, 2), "stage_ms": st,
             "forward_adjoint_ms": round(1e3 * dt_fa, 4), "forward_adjoint_over_forward": round(dt_fa / dt, 3),
+            "forward_adjoint_rebuilding_planes_ms": round(1e3 * dt_fa_rebuild, 4),
             "zstage_gbs": round(zbytes / (st["zstage"] * 1e-3) / 1e9, 1) if st["zstage"] > 0 else None,
             "rel_l2_vs_fp64_coil7_vol0_echo0": err}
 

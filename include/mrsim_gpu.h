@@ -216,6 +216,14 @@ int mrs_plan_forward_graph(mrs_plan* plan, void* stream, char* err, size_t errle
  * Gradients are written (not accumulated). */
 int mrs_plan_vjp(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* grad_images, float* const* grad_fmap,
                  float* const* grad_t_inhom, void* stream, char* err, size_t errlen);
+/* mrs_plan_vjp for slice inputs unchanged (same pointers, same contents) since this plan's last
+ * mrs_plan_forward / mrs_plan_forward_graph on the same stream: the adjoint reads the slice planes
+ * (phase, decay rates, coil planes) that forward built instead of rebuilding them, as a backward pass
+ * reads saved activations.  Rebuilds them when no forward has run since a slice pointer changed.
+ * The caller vouches for the contents. */
+int mrs_plan_vjp_after_forward(mrs_plan* plan, const mrs_c32* const* grad_out, void* const* grad_images,
+                               float* const* grad_fmap, float* const* grad_t_inhom, void* stream, char* err,
+                               size_t errlen);
 /* Average duration (ms) of each forward stage over `iters` eager launches, timed with HIP events
  * on `stream`: ms[0] slice prep, ms[1] y-stage, ms[2] x-stage, ms[3] zero rows + spikes/noise,
  * ms[4] the attached partition transform (mrs_plan_set_zstage). */

@@ -255,6 +255,9 @@ struct mrs_plan {
     double terms = 0.0, nlines = 0.0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
+    // the slice planes were built by a forward from the slice inputs as bound now (cleared when a
+    // slice input pointer changes)
+    bool planes_current = false;
     hipStream_t graph_stream = nullptr;
     // adjoint scratch
     std::vector<VjpUnit> vjp_units;
@@ -628,7 +631,10 @@ extern "C" int mrs_plan_set_pointers(mrs_plan* p, const mrs_slice_desc* slices, 
             return fail(err, errlen, MRS_E_INVALID, "slice %zu changed group", s);
         DevSlice before = p->slices[s];
         fill_slice_inputs(p->slices[s], slices[s]);
-        changed |= std::memcmp(&before, &p->slices[s], sizeof(DevSlice)) != 0;
+        if (std::memcmp(&before, &p->slices[s], sizeof(DevSlice)) != 0) {
+            changed = true;
+            p->planes_current = false;
+        }
         p->slice_desc[s] = slices[s];
     }
     for (size_t u = 0; u < p->units.size(); u++) {
@@ -1029,7 +1035,9 @@ extern "C" int mrs_plan_set_zstage(mrs_plan* p, const mrs_zset* sets, size_t nse
 
 extern "C" int mrs_plan_forward(mrs_plan* p, void* stream, char* err, size_t errlen) {
     if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
-    return enqueue_forward(p, (hipStream_t)stream, err, errlen);
+    const int rc = enqueue_forward(p, (hipStream_t)stream, err, errlen);
+    p->planes_current = rc == MRS_OK;
+    return rc;
 }
 
 extern "C" int mrs_plan_forward_graph(mrs_plan* p, void* stream, char* err, size_t errlen) {
@@ -1053,12 +1061,15 @@ extern "C" int mrs_plan_forward_graph(mrs_plan* p, void* stream, char* err, size
         HIP_TRY(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
     }
     HIP_TRY(hipGraphLaunch(p->graph_exec, st));
+    p->planes_current = true;
     return MRS_OK;
 }
 
-extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images,
-                            float* const* grad_fmap, float* const* grad_t_inhom, void* stream, char* err,
-                            size_t errlen) {
+// reuse: skip the slice prep when a forward built the planes from the inputs as bound now (the
+// forward and the adjoint read the same plane forms: split coil planes for split-BF16 plans, FP32
+// ones otherwise)
+static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images, float* const* grad_fmap,
+                    float* const* grad_t_inhom, void* stream, bool reuse, char* err, size_t errlen) {
     if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
     if (!grad_out) return fail(err, errlen, MRS_E_INVALID, "grad_out is required");
     hipStream_t st = (hipStream_t)stream;
@@ -1093,7 +1104,10 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
     // split-BF16 plans take the y-adjoint's coil contraction on MFMA (split coil planes), FP32 plans
     // keep it in FP32 on the VALU
     const bool mfma = p->precision == MRS_PREC_SPLIT_BF16;
-    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, mfma ? PREP_SPLIT : PREP_S32, st));
+    static_assert(PREP_SPLIT != PREP_S32, "plane forms");
+    if (!(reuse && p->planes_current && fwd_prep_mode(p) == (mfma ? PREP_SPLIT : PREP_S32)))
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, mfma ? PREP_SPLIT : PREP_S32,
+                            st));
     size_t u0 = 0;
     while (u0 < n) {
         size_t u1 = u0 + 1;
@@ -1105,6 +1119,18 @@ extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* c
         u0 = u1;
     }
     return MRS_OK;
+}
+
+extern "C" int mrs_plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images,
+                            float* const* grad_fmap, float* const* grad_t_inhom, void* stream, char* err,
+                            size_t errlen) {
+    return plan_vjp(p, grad_out, grad_images, grad_fmap, grad_t_inhom, stream, false, err, errlen);
+}
+
+extern "C" int mrs_plan_vjp_after_forward(mrs_plan* p, const mrs_c32* const* grad_out, void* const* grad_images,
+                                          float* const* grad_fmap, float* const* grad_t_inhom, void* stream,
+                                          char* err, size_t errlen) {
+    return plan_vjp(p, grad_out, grad_images, grad_fmap, grad_t_inhom, stream, true, err, errlen);
 }
 
 // Average duration of each forward stage over `iters` launches, timed with HIP events recorded on

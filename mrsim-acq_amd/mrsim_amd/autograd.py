@@ -58,7 +58,9 @@ class KspaceModel:
         self.plan.forward()
         return out
 
-    def vjp_raw(self, images, fmap, t_inhom, gout, need=(True, True, True)):
+    def vjp_raw(self, images, fmap, t_inhom, gout, need=(True, True, True), after_forward=False):
+        """after_forward: the inputs are unchanged since the last forward_raw (the adjoint reuses its
+        slice planes; a changed fieldmap or T2' tensor still rebuilds them)."""
         snx, sny, nx, ny = self.geom
         self._bind(images.detach().contiguous(), fmap.detach().contiguous(), t_inhom.detach().contiguous(),
                    self._out0)
@@ -66,7 +68,7 @@ class KspaceModel:
         gf = torch.empty_like(fmap) if need[1] else None
         gt = torch.empty_like(t_inhom) if need[2] else None
         self.plan.vjp([gout.contiguous()], None if gi is None else [gi], None if gf is None else [gf],
-                      None if gt is None else [gt])
+                      None if gt is None else [gt], after_forward=after_forward)
         return gi, gf, gt
 
     def __call__(self, images, fmap, t_inhom):

@@ -183,7 +183,7 @@ class Ge3dForward:
         self.plan.forward(stream)
         return self.out
 
-    def vjp(self, grad_out, stream=None):
+    def vjp(self, grad_out, stream=None, after_forward=False):
         """Vector-Jacobian product of the train for a real loss L: grad_out[v][e] = dL/d out[v][e]
         ((nz, C, ny, nx) complex64, torch's convention dL/dRe + i dL/dIm).
 
@@ -197,6 +197,8 @@ class Ge3dForward:
           fmap:    (nz*P,) float32, per Hz
           t_inhom: per compartment (nz*P,) float32, per ms of T2' (at the uniform value where T2' is
                    uniform: sum it over the voxels for the scalar's gradient)
+        after_forward: the inputs are unchanged since this train's last forward on `stream`; the
+        per-unit adjoints then reuse that forward's slice planes.
         """
         import torch
         if not hasattr(self, "_vjp_buf"):
@@ -226,7 +228,7 @@ class Ge3dForward:
             sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
                              n_comp=n_used, ky_segments=self.ky_segments, weights=w))
         zstage(sets, adjoint=True, stream=stream)
-        self.plan.vjp(gout, gimg, gfm, gti, stream)
+        self.plan.vjp(gout, gimg, gfm, gti, stream, after_forward=after_forward)
         out = []
         for vi, v in enumerate(self.volumes):
             b = per_vol[vi]

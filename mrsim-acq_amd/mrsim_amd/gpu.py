@@ -58,6 +58,7 @@ def lib():
     L.mrs_plan_forward_graph.argtypes = [ct.c_void_p, ct.c_void_p] + E
     L.mrs_plan_vjp.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), P(ct.c_void_p), P(ct.c_void_p),
                                ct.c_void_p] + E
+    L.mrs_plan_vjp_after_forward.argtypes = L.mrs_plan_vjp.argtypes
     L.mrs_plan_stage_times.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, P(ct.c_double)] + E
     L.mrs_plan_stats.argtypes = [ct.c_void_p, P(ct.c_double), P(ct.c_double), P(ct.c_double)]
     L.mrs_plan_describe.argtypes = [ct.c_void_p, ct.c_char_p, ct.c_size_t]
@@ -611,7 +612,9 @@ class Plan:
         err = _errbuf()
         _check(lib().mrs_plan_forward_graph(self._h, self._stream(stream), err, len(err)), err)
 
-    def vjp(self, grad_out, grad_images=None, grad_fmap=None, grad_t_inhom=None, stream=None):
+    def vjp(self, grad_out, grad_images=None, grad_fmap=None, grad_t_inhom=None, stream=None, after_forward=False):
+        """after_forward: the slice inputs are unchanged since this plan's last forward on `stream`;
+        the adjoint reuses that forward's slice planes (mrs_plan_vjp_after_forward)."""
         n = len(self.units)
 
         def arr(ts):
@@ -622,8 +625,9 @@ class Plan:
             return a
 
         err = _errbuf()
-        rc = lib().mrs_plan_vjp(self._h, arr(grad_out), arr(grad_images), arr(grad_fmap), arr(grad_t_inhom),
-                                self._stream(stream), err, len(err))
+        fn = lib().mrs_plan_vjp_after_forward if after_forward else lib().mrs_plan_vjp
+        rc = fn(self._h, arr(grad_out), arr(grad_images), arr(grad_fmap), arr(grad_t_inhom), self._stream(stream), err,
+                len(err))
         _check(rc, err)
 
     def tune(self, trials=3, stream=None) -> str:

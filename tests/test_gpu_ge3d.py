@@ -188,6 +188,21 @@ def test_ge3d_vjp_adjoint_identity(grad_setup):
     assert abs(lhs - rhs) <= 1e-5 * scale
 
 
+def test_ge3d_vjp_after_forward_matches_the_plain_vjp(grad_setup):
+    # the adjoint on the slice planes of the forward just run equals the one that rebuilds them
+    import torch
+    w, fw, vols, g, grads = grad_setup
+    gd = [[torch.from_numpy(x).cuda() for x in gv] for gv in g]
+    fw.forward()
+    again = fw.vjp(gd, after_forward=True)
+    torch.cuda.synchronize()
+    for a, b in zip(again, grads):
+        assert torch.equal(a["fmap"], b["fmap"])
+        for key in ("images", "t_inhom"):
+            for x, y in zip(a[key], b[key]):
+                assert (x is None and y is None) or torch.equal(x, y)
+
+
 def _ge3d_oracle_loss(w, v, gv, images=None, fmap=None, t_inhom=None):
     loss = 0.0
     for e in range(len(w.echo_times_ms)):

@@ -59,6 +59,24 @@ def test_forward_matches_oracle(setup):
     assert np.linalg.norm(k - want) / np.linalg.norm(want) <= 1e-5
 
 
+def test_vjp_after_forward_reuses_or_rebuilds_the_slice_planes(setup):
+    # After a forward on the same inputs the adjoint may read that forward's slice planes; once a slice
+    # input changes it must rebuild them.  Both must give the plain adjoint's gradients bit for bit, and
+    # the two fieldmaps give different gradients, so stale planes could not pass.
+    import torch
+    w, model, img, fmap, ti, g = setup
+    fmap2 = fmap + 3.0
+    want1 = model.vjp_raw(img, fmap, ti, g)
+    want2 = model.vjp_raw(img, fmap2, ti, g)
+    model.forward_raw(img, fmap, ti)
+    got1 = model.vjp_raw(img, fmap, ti, g, after_forward=True)
+    got2 = model.vjp_raw(img, fmap2, ti, g, after_forward=True)
+    torch.cuda.synchronize()
+    for a, b in zip(got1 + got2, want1 + want2):
+        assert torch.equal(a, b)
+    assert not torch.equal(want1[1], want2[1])
+
+
 def test_adjoint_identity(setup):
     w, model, img, fmap, ti, g = setup
     k, gi, gf, gt = _grads(model, img, fmap, ti, g)
