@@ -14,7 +14,8 @@ namespace mrs {
 // x-stage table (the Nyquist-ghost polarity), then ky; each table segment padded to a multiple of
 // 16 lines with ky = -1 records.
 struct LineRec {
-    double t_s;    // fieldmap phase time t_ms/1000 (s), src/kspace.rs:844
+    double t_s;    // fieldmap phase time t_ms/1000 (s), src/kspace.rs:844, plus TE/1000 for a gradient
+                   // echo with distortions (its f*TE phase, :717-721)
     double kyn;    // (ky - ny/2) / sny, src/kspace.rs:729
     double d;      // eddy clock exp(-tread/tau_eddy) * t (s), src/kspace.rs:910; 0 without eddy
     float trf;     // ms from the RF (T2 decay), src/kspace.rs:845
@@ -29,7 +30,7 @@ static_assert(sizeof(LineRec) == 48, "LineRec layout");
 // Static per-slice planes, written by the prep kernel, all in [x][y] (y fastest) order so a lane's
 // four consecutive y values are one 16-byte load.
 struct DevSlice {
-    double* base;      // P: object phase in cycles (phase0 + eddy ramp)/2pi [+ fmap*TE/1000 for GE]
+    double* base;      // P: object phase in cycles (phase0 + eddy ramp)/2pi
     float* rate;       // P: fieldmap (Hz) if do_distortions else 0
     double* poly;      // P: eddy polynomial (cycles per second of eddy clock), 0 without eddy
     float* nr2;        // Q*P: -log2(e)/T2 (1/ms), 0 without relaxation
@@ -55,6 +56,7 @@ struct DevSlice {
     int32_t has_eddy;
     double eddy[3];
     int32_t s_owner;   // this slice writes the coil planes; otherwise they alias an earlier slice's
+    int32_t p_owner;   // this slice writes the static planes (base .. rq); otherwise the same
 };
 
 // Slice prep outputs: the split coil planes (MFMA forward) and/or the FP32 ones (plain, adjoint).

@@ -151,6 +151,7 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
     if ((int)blockIdx.x >= tiles_x * tiles_y) return;
     const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 32;
     const int C = gr.C, Q = gr.Q;
+    const bool p_own = sl.p_owner != 0;   // read here, where its latency hides behind the read phase
     const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // read phase: x fastest
     const double zc = (double)sl.z - (double)(sl.nz / 2);
@@ -169,8 +170,7 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
             double p0 = sl.phase0 ? gld(sl.phase0)[i] : 0.0;
             if (eddy_ph) p0 += gr.eddy_phase * (g0 * xc + g1 * yc + g2 * zc_c);
             const float f = gld(sl.fmap)[i];
-            base = p0 / kTau;
-            if (gr.echo == MRS_ECHO_GRADIENT && gr.do_dist) base += (double)f * (gr.t_echo / 1000.0);
+            base = p0 / kTau;   // a gradient echo's f*TE/1000 is on the lines' times
             rate = gr.do_dist ? f : 0.0f;
             if (eddy)
                 poly = gr.eddy_strength * (g0 * xc + g1 * yc + g2 * zc) +
@@ -196,28 +196,31 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
     }
     __syncthreads();
     const int wy = threadIdx.x & 31, wx = threadIdx.x >> 5;   // write phase: y fastest
-    for (int k = 0; k < 4; k++) {
-        const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
-        if (x >= snxP || y >= snyP) continue;
-        const size_t o = (size_t)x * snyP + y;
-        gst(sl.base)[o] = sd[0][xl][wy];
-        gst(sl.poly)[o] = sd[1][xl][wy];
-        const float rate = sf[0][xl][wy];
-        gst(sl.rate)[o] = rate;
-        for (int q = 0; q < Q; q++) {
-            gst(sl.nr2)[q * PP + o] = sf[1 + 2 * q][xl][wy];
-            gst(sl.nri)[q * PP + o] = sf[2 + 2 * q][xl][wy];
-        }
-        if (gr.rot_shared) {
-            // chain rotors shared by every lane of a chain pass (FP64 angle, FP64 reduction)
-            const double yy = (double)y - gr.sys_yoff;
-            float rs, rc;
-            sincos_turns(__builtin_fma(gr.rot_dk, yy, (double)rate * gr.rot_dt), rs, rc);
-            reinterpret_cast<GAS f32x2*>(gst(sl.rot))[o] = f32x2{rc, rs};
-            for (int c = 0; c < 2; c++)
-                for (int q = 0; q < Q; q++)
-                    gst(sl.rq)[(c * Q + q) * PP + o] = __builtin_amdgcn_exp2f(
-                        __builtin_fmaf(gr.rot_dtrf, sf[1 + 2 * q][xl][wy], gr.rot_dtau[c] * sf[2 + 2 * q][xl][wy]));
+    // static planes: written once per set of slices that would write identical ones
+    if (p_own) {
+        for (int k = 0; k < 4; k++) {
+            const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
+            if (x >= snxP || y >= snyP) continue;
+            const size_t o = (size_t)x * snyP + y;
+            gst(sl.base)[o] = sd[0][xl][wy];
+            gst(sl.poly)[o] = sd[1][xl][wy];
+            const float rate = sf[0][xl][wy];
+            gst(sl.rate)[o] = rate;
+            for (int q = 0; q < Q; q++) {
+                gst(sl.nr2)[q * PP + o] = sf[1 + 2 * q][xl][wy];
+                gst(sl.nri)[q * PP + o] = sf[2 + 2 * q][xl][wy];
+            }
+            if (gr.rot_shared) {
+                // chain rotors shared by every lane of a chain pass (FP64 angle, FP64 reduction)
+                const double yy = (double)y - gr.sys_yoff;
+                float rs, rc;
+                sincos_turns(__builtin_fma(gr.rot_dk, yy, (double)rate * gr.rot_dt), rs, rc);
+                reinterpret_cast<GAS f32x2*>(gst(sl.rot))[o] = f32x2{rc, rs};
+                for (int c = 0; c < 2; c++)
+                    for (int q = 0; q < Q; q++)
+                        gst(sl.rq)[(c * Q + q) * PP + o] = __builtin_amdgcn_exp2f(
+                            __builtin_fmaf(gr.rot_dtrf, sf[1 + 2 * q][xl][wy], gr.rot_dtau[c] * sf[2 + 2 * q][xl][wy]));
+            }
         }
     }
     // coil planes: written once per distinct (maps, amplitude) set, only in the forms asked for

@@ -266,6 +266,9 @@ struct mrs_plan {
     // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
     std::vector<std::array<void*, 3>> s_alloc;
     std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
+    // per-slice allocations of the static planes (base, rate, poly, nr2, nri, rot, rq); slices whose
+    // prep would write identical ones (the echoes of one partition) share one set
+    std::vector<std::array<void*, 7>> p_alloc;
     // partition transform run after the 2D forward (3D trains)
     std::vector<ZSet> zsets;
     void* d_zmem = nullptr;
@@ -351,7 +354,7 @@ static int order_segment(std::vector<LineRec>& seg) {
     std::vector<LineRec> best_rest = seg;
     for (int stride : {2, 1, 4, 3, 8}) {
         std::map<std::pair<int, int>, std::vector<LineRec>> cls;
-        for (const LineRec& L : seg) cls[std::make_pair(L.t_s >= 0.0 ? 0 : 1, L.ky % stride)].push_back(L);
+        for (const LineRec& L : seg) cls[std::make_pair((int)L.cls, L.ky % stride)].push_back(L);
         std::vector<std::array<LineRec, 4>> chains;
         std::vector<LineRec> rest;
         for (auto& kv : cls) {
@@ -426,14 +429,17 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
         }
         g.acq_rows.push_back((int32_t)k);
         LineRec L{};
-        L.t_s = t[k] / 1000.0;
+        const double ts = t[k] / 1000.0;
+        // a gradient echo's fieldmap phase f*TE/1000 (object phase, src/kspace.rs:717-721) plus f*t
+        // (:844) rides whole on the line's time, so the static phase plane does not depend on the echo
+        L.t_s = a.echo == MRS_ECHO_GRADIENT && a.do_distortions ? ts + a.t_echo / 1000.0 : ts;
         L.kyn = ((double)k - (double)ys) / (double)sny;
-        L.d = eddy_possible ? std::exp(-tread[k] / a.eddy_tau) * L.t_s : 0.0;
+        L.d = eddy_possible ? std::exp(-tread[k] / a.eddy_tau) * ts : 0.0;
         L.trf = (float)trf[k];
-        L.tau = a.echo == MRS_ECHO_SPIN ? (float)(std::fabs(L.t_s) * 1000.0) : (float)trf[k];
-        L.tb0 = a.echo == MRS_ECHO_SPIN ? (float)L.t_s : (float)(L.t_s + a.t_echo / 1000.0);
+        L.tau = a.echo == MRS_ECHO_SPIN ? (float)(std::fabs(ts) * 1000.0) : (float)trf[k];
+        L.tb0 = a.echo == MRS_ECHO_SPIN ? (float)ts : (float)(ts + a.t_echo / 1000.0);
         L.ky = (int32_t)k;
-        L.cls = L.t_s >= 0.0 ? 0 : 1;
+        L.cls = ts >= 0.0 ? 0 : 1;
         // src/kspace.rs:987 and :1109: table 1 iff the line's ghost shift is negative
         const double shift = pol[k] < 0 ? -a.ghost_offset : a.ghost_offset;
         L.table = (a.ghost_offset != 0.0 && shift < 0.0) ? 1 : 0;
@@ -595,8 +601,42 @@ static void share_coil_planes(mrs_plan* p) {
     }
 }
 
+// Slices whose static planes would be identical share the first such slice's: the same inputs of
+// the static planes, the same geometry and the same protocol terms that enter them (the echo time
+// does not: a gradient echo's fmap*TE phase rides on the lines' times).  The echoes of one 3D
+// partition are such slices.
+static void share_static_planes(mrs_plan* p) {
+    std::map<std::string, size_t> owner;
+    for (size_t s = 0; s < p->slices.size(); s++) {
+        DevSlice& sl = p->slices[s];
+        const DevGroup& dg = p->groups[sl.group].dev;
+        std::string key;
+        auto put = [&](const auto& v) { key.append(reinterpret_cast<const char*>(&v), sizeof v); };
+        put(sl.fmap); put(sl.phase0); put(sl.t2); put(sl.t_inhom); put(sl.has_t_inhom);
+        for (int q = 0; q < dg.Q; q++) { put(sl.t2_uniform[q]); put(sl.ti_uniform[q]); }
+        put(sl.z); put(sl.nz); put(sl.has_eddy); put(sl.eddy);
+        put(dg.snx); put(dg.sny); put(dg.Q); put(dg.ox); put(dg.oy); put(dg.sxs_xoff); put(dg.sys_yoff);
+        put(dg.do_relax); put(dg.do_dist); put(dg.t_inhom); put(dg.eddy_phase); put(dg.eddy_strength);
+        put(dg.eddy_quad); put(dg.rot_shared); put(dg.rot_dt); put(dg.rot_dk); put(dg.rot_dtrf);
+        put(dg.rot_dtau);
+        auto it = owner.find(key);
+        const size_t o = it == owner.end() ? s : it->second;
+        if (it == owner.end()) owner.emplace(std::move(key), s);
+        sl.p_owner = o == s;
+        const auto& a = p->p_alloc[o];
+        sl.base = (double*)a[0];
+        sl.rate = (float*)a[1];
+        sl.poly = (double*)a[2];
+        sl.nr2 = (float*)a[3];
+        sl.nri = (float*)a[4];
+        sl.rot = (float2*)a[5];
+        sl.rq = (float*)a[6];
+    }
+}
+
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     share_coil_planes(p);
+    share_static_planes(p);
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -706,6 +746,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         p->s_alloc.push_back({(void*)sl.s_hi, (void*)sl.s_lo, (void*)sl.s32});
         if ((rc = plan_alloc(p, &sl.rot, dg.rot_shared ? PP : 1, err, errlen))) break;
         if ((rc = plan_alloc(p, &sl.rq, dg.rot_shared ? 2 * PP * dg.Q : 1, err, errlen))) break;
+        p->p_alloc.push_back({(void*)sl.base, (void*)sl.rate, (void*)sl.poly, (void*)sl.nr2, (void*)sl.nri,
+                              (void*)sl.rot, (void*)sl.rq});
     }
     // per-unit scratch and work lists
     std::map<std::tuple<int, int, int, int>, size_t> cls;
