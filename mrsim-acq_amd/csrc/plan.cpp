@@ -263,6 +263,14 @@ struct mrs_plan {
     std::vector<VjpUnit> vjp_units;
     VjpUnit* d_vjp_units = nullptr;
     bool vjp_fresh = true;
+    // Adjoint launch order: units of one launch shape run in one launch, and within it the units that
+    // read one coil-plane set (the echoes of a 3D partition) sit side by side, so their blocks for a
+    // column x meet on one XCD and share the coil planes in L2.  d_units_vjp and d_vjp_units hold the
+    // units' descriptors in this order; vjp_runs are its launches.
+    std::vector<size_t> vjp_order;
+    std::vector<std::pair<size_t, size_t>> vjp_runs;
+    DevUnit* d_units_vjp = nullptr;
+    uint64_t units_version = 0, units_version_vjp = ~0ull;
     // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
     std::vector<std::array<void*, 3>> s_alloc;
     std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
@@ -637,6 +645,7 @@ static void share_static_planes(mrs_plan* p) {
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     share_coil_planes(p);
     share_static_planes(p);
+    p->units_version++;
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -1125,8 +1134,35 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
             if (rc) { p->vjp_units.clear(); return rc; }
         }
         int rc = plan_alloc(p, &p->d_vjp_units, n, err, errlen);
+        if (!rc) rc = plan_alloc(p, &p->d_units_vjp, n, err, errlen);
         if (rc) { p->vjp_units.clear(); return rc; }
         p->vjp_fresh = true;
+        // launch shape, then coil-plane set, then unit
+        auto shape = [&](size_t u) {
+            const DevGroup& dg = p->groups[p->units[u].group].dev;
+            return std::make_tuple(dg.Q, dg.C, dg.cplx, dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP,
+                                   dg.t_hi[1] != dg.t_hi[0]);
+        };
+        p->vjp_order.resize(n);
+        for (size_t u = 0; u < n; u++) p->vjp_order[u] = u;
+        std::stable_sort(p->vjp_order.begin(), p->vjp_order.end(), [&](size_t a, size_t b) {
+            const auto ka = shape(a), kb = shape(b);
+            if (ka != kb) return ka < kb;
+            return p->s_owner[p->units[a].slice] < p->s_owner[p->units[b].slice];
+        });
+        p->vjp_runs.clear();
+        for (size_t k = 0; k < n;) {
+            size_t e = k + 1;
+            while (e < n && shape(p->vjp_order[e]) == shape(p->vjp_order[k])) e++;
+            p->vjp_runs.push_back({k, e});
+            k = e;
+        }
+    }
+    if (p->units_version_vjp != p->units_version) {  // blocking copy, as below
+        std::vector<DevUnit> du(n);
+        for (size_t k = 0; k < n; k++) du[k] = p->units[p->vjp_order[k]];
+        HIP_TRY(hipMemcpy(p->d_units_vjp, du.data(), n * sizeof(DevUnit), hipMemcpyHostToDevice));
+        p->units_version_vjp = p->units_version;
     }
     bool changed = p->vjp_fresh;
     for (size_t u = 0; u < n; u++) {
@@ -1140,7 +1176,9 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
         changed |= std::memcmp(&before, &v, sizeof(VjpUnit)) != 0;
     }
     if (changed) {  // blocking copy: the host vector may change as soon as this call returns
-        HIP_TRY(hipMemcpy(p->d_vjp_units, p->vjp_units.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
+        std::vector<VjpUnit> vo(n);
+        for (size_t k = 0; k < n; k++) vo[k] = p->vjp_units[p->vjp_order[k]];
+        HIP_TRY(hipMemcpy(p->d_vjp_units, vo.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
         p->vjp_fresh = false;
     }
     // split-BF16 plans take the y-adjoint's coil contraction on MFMA (split coil planes), FP32 plans
@@ -1150,15 +1188,11 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
     if (!(reuse && p->planes_current && fwd_prep_mode(p) == (mfma ? PREP_SPLIT : PREP_S32)))
         HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, mfma ? PREP_SPLIT : PREP_S32,
                             st));
-    size_t u0 = 0;
-    while (u0 < n) {
-        size_t u1 = u0 + 1;
-        while (u1 < n && p->units[u1].group == p->units[u0].group) u1++;
-        const DevGroup& dg = p->groups[p->units[u0].group].dev;
-        HIP_TRY(launch_vjp_units((int)u0, (int)(u1 - u0), dg.Q, dg.C, dg.cplx != 0, dg.eddy_strength != 0.0, dg.snx,
-                                 dg.sny, dg.nx, dg.LyP, dg.t_hi[1] != dg.t_hi[0] ? 2 : 1, mfma, p->d_units, p->d_slices, p->d_groups,
-                                 p->d_vjp_units, st));
-        u0 = u1;
+    for (const auto& run : p->vjp_runs) {
+        const DevGroup& dg = p->groups[p->units[p->vjp_order[run.first]].group].dev;
+        HIP_TRY(launch_vjp_units((int)run.first, (int)(run.second - run.first), dg.Q, dg.C, dg.cplx != 0,
+                                 dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP, dg.t_hi[1] != dg.t_hi[0] ? 2 : 1,
+                                 mfma, p->d_units_vjp, p->d_slices, p->d_groups, p->d_vjp_units, st));
     }
     return MRS_OK;
 }
