@@ -1184,26 +1184,41 @@ __global__ __launch_bounds__(256) void xadjoint_mfma(const DevUnit* __restrict__
         f32x4 accr[MT], acci[MT];
 #pragma unroll
         for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int k0 = 0; k0 < nkP; k0 += 16) {
-            const int kx = k0 + lq * 4;
-            u32x4 bhi, blo;
+        // batches of KB 16-kx steps: the batch's cotangent loads are all in flight before the first
+        // is split, one memory latency per batch rather than per step
+        constexpr int KB = 4;
+        for (int kb = 0; kb < nkP; kb += 16 * KB) {
+            f32x2 gv[KB][4];
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
-                const f32x2 gv = (ok && kx + jj < nx) ? g[kx + jj] : f32x2{0.f, 0.f};
-                uint32_t h, l;
-                split2(gv.x, gv.y, h, l);
-                bhi[jj] = h;
-                blo[jj] = l;
-            }
-            const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
-            const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+            for (int t = 0; t < KB; t++)
 #pragma unroll
-            for (int m = 0; m < MT; m++) {
-                const int r = m * 16 + ln;
-                const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + kx);
-                const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + kx);
-                accr[m] = mfma3(h, l, brh, brl, accr[m]);
-                acci[m] = mfma3(h, l, bih, bil, acci[m]);
+                for (int jj = 0; jj < 4; jj++) {
+                    const int kx = kb + 16 * t + lq * 4 + jj;
+                    gv[t][jj] = (ok && kx < nx) ? g[kx] : f32x2{0.f, 0.f};
+                }
+#pragma unroll
+            for (int t = 0; t < KB; t++) {
+                const int k0 = kb + 16 * t;
+                if (k0 >= nkP) break;
+                const int kx = k0 + lq * 4;
+                u32x4 bhi, blo;
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++) {
+                    uint32_t h, l;
+                    split2(gv[t][jj].x, gv[t][jj].y, h, l);
+                    bhi[jj] = h;
+                    blo[jj] = l;
+                }
+                const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
+                const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    const int r = m * 16 + ln;
+                    const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + kx);
+                    const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + kx);
+                    accr[m] = mfma3(h, l, brh, brl, accr[m]);
+                    acci[m] = mfma3(h, l, bih, bil, acci[m]);
+                }
             }
         }
         // accr[m][r]: row x0 + 16 m + 4 lq + r, column n
