@@ -190,11 +190,16 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
     for _ in range(2):
         fw.forward_graph()
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        fw.forward_graph()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / steps
+
+    def fwd_loop(n):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fw.forward_graph()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    # best of three repetitions, as for forward + adjoint below
+    dt = min(fwd_loop(max(4, steps)) for _ in range(3))
     st = fw.stage_times(iters=3)
     # forward + adjoint: the forward, then the vector-Jacobian product of a seeded complex cotangent
     # (partition-transform adjoint, per-unit adjoints, sums over echoes); the adjoint reads the slice
@@ -203,22 +208,22 @@ def ge3d_pass(steps: int, check: bool = True, tiles: str = "", stages_only: bool
     rng = np.random.default_rng(5)
     gout = [[torch.from_numpy((rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
                               .astype(np.complex64)).cuda() for o in ov] for ov in fw.out]
-    fw.forward_graph()
-    fw.vjp(gout)
-    torch.cuda.synchronize()
-    nfa = max(2, steps // 2)
-    t0 = time.perf_counter()
-    for _ in range(nfa):
-        fw.forward_graph()
-        fw.vjp(gout, after_forward=True)
-    torch.cuda.synchronize()
-    dt_fa = (time.perf_counter() - t0) / nfa
-    t0 = time.perf_counter()
-    for _ in range(nfa):
-        fw.forward_graph()
-        fw.vjp(gout)
-    torch.cuda.synchronize()
-    dt_fa_rebuild = (time.perf_counter() - t0) / nfa
+
+    def fa_loop(reuse, n):
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fw.forward_graph()
+            fw.vjp(gout, after_forward=reuse)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    fa_loop(True, 2)
+    fa_loop(False, 2)
+    # three interleaved repetitions of each path, the best of each (a step is a few ms)
+    nfa = max(4, steps)
+    reps = [(fa_loop(True, nfa), fa_loop(False, nfa)) for _ in range(3)]
+    dt_fa = min(r[0] for r in reps)
+    dt_fa_rebuild = min(r[1] for r in reps)
     n_e = len(w.echo_times_ms)
     parts = sum(v.nz for v in w.volumes) * n_e
     # partition transform traffic: read K2 and write K3, complex64, every (volume, echo)
