@@ -270,7 +270,7 @@ struct mrs_plan {
     std::vector<size_t> vjp_order;
     std::vector<std::pair<size_t, size_t>> vjp_runs;
     DevUnit* d_units_vjp = nullptr;
-    uint64_t units_version = 0, units_version_vjp = ~0ull;
+    std::vector<DevUnit> units_vjp;   // what d_units_vjp holds (the fields the adjoint reads)
     // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
     std::vector<std::array<void*, 3>> s_alloc;
     std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
@@ -645,7 +645,6 @@ static void share_static_planes(mrs_plan* p) {
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     share_coil_planes(p);
     share_static_planes(p);
-    p->units_version++;
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -1158,11 +1157,20 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
             k = e;
         }
     }
-    if (p->units_version_vjp != p->units_version) {  // blocking copy, as below
+    {
+        // the adjoint reads a unit's images, slice and group: a new output pointer (each forward of
+        // the autograd wrapper binds one) needs no upload
         std::vector<DevUnit> du(n);
-        for (size_t k = 0; k < n; k++) du[k] = p->units[p->vjp_order[k]];
-        HIP_TRY(hipMemcpy(p->d_units_vjp, du.data(), n * sizeof(DevUnit), hipMemcpyHostToDevice));
-        p->units_version_vjp = p->units_version;
+        for (size_t k = 0; k < n; k++) {
+            du[k] = p->units[p->vjp_order[k]];
+            du[k].out = nullptr;
+            du[k].seed = du[k].coil0 = 0;
+        }
+        if (du.size() != p->units_vjp.size() ||
+            std::memcmp(du.data(), p->units_vjp.data(), n * sizeof(DevUnit)) != 0) {  // blocking copy, as below
+            HIP_TRY(hipMemcpy(p->d_units_vjp, du.data(), n * sizeof(DevUnit), hipMemcpyHostToDevice));
+            p->units_vjp.swap(du);
+        }
     }
     bool changed = p->vjp_fresh;
     for (size_t u = 0; u < n; u++) {
