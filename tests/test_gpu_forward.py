@@ -276,3 +276,47 @@ def test_chunked_streams_batch_matches_oracle_and_replays():
         torch.cuda.synchronize()
         assert all(torch.equal(o, r) for o, r in zip(outs, ref)), f"replay {rep}"
     plan.close()
+
+
+def test_echoes_share_static_planes_until_an_input_changes():
+    # Two gradient-echo groups (TE 20 and 32 ms, distortions on) whose slices bind the same fieldmap,
+    # coil maps and T2: the plan gives them one set of static planes, with each echo's f*TE phase on
+    # its lines.  Rebinding one slice's fieldmap must split the set again.  Both states against the
+    # FP64 oracle at the split-BF16 tolerance.
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    rng = np.random.default_rng(12)
+    snx = sny = 32
+    nx = ny = 16
+    C, Q = 4, 1
+    P = snx * sny
+    acqs = [abi.Acquisition.default(echo=abi.ECHO_GRADIENT, t_echo=te, t_line=0.5, signal_scale=1.0, n_coils=C)
+            for te in (20.0, 32.0)]
+    fmap = (rng.standard_normal(P) * 20.0).astype(np.float32)
+    fmap2 = (rng.standard_normal(P) * 20.0).astype(np.float32)
+    maps = (rng.standard_normal((C, P)) + 1j * rng.standard_normal((C, P))).astype(np.complex64)
+    t2 = rng.uniform(60.0, 90.0, size=(Q, P)).astype(np.float32)
+    img = rng.uniform(0.0, 1.0, size=(Q, P)).astype(np.float32)
+    d = lambda a: torch.from_numpy(a).cuda()
+    f_d, f2_d, m_d, t2_d, img_d = d(fmap), d(fmap2), d(maps), d(t2), d(img)
+    outs = [torch.empty((C, ny, nx), dtype=torch.complex64, device="cuda") for _ in range(2)]
+    groups = [gpu.Group(snx, sny, nx, ny, C, Q, a) for a in acqs]
+    slices = [gpu.Slice(g, f_d, m_d, t2=t2_d) for g in range(2)]
+    units = [gpu.Unit(g, img_d, outs[g]) for g in range(2)]
+    plan = gpu.Plan(groups, slices, units)
+
+    def check(fmaps):
+        plan.forward()
+        torch.cuda.synchronize()
+        for g in range(2):
+            want = O.forward_direct(snx, sny, nx, ny, C, Q, acqs[g], img, fmaps[g], maps, t2=t2)
+            got = outs[g].cpu().numpy()
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            print(f"echo {g}: rel-L2 {err:.2e}", flush=True)
+            assert err <= 1e-5
+
+    check([fmap, fmap])
+    plan.set_pointers(slices=[slices[0], gpu.Slice(1, f2_d, m_d, t2=t2_d)])
+    check([fmap, fmap2])
+    plan.close()
