@@ -216,8 +216,9 @@ class Ge3dForward:
                 views[1].append(b[0][e, ci, z])
                 views[2].append(b[1][e, ci, z])
                 views[3].append(b[2][e, ci, z])
-            self._vjp_buf = (gk2, per_vol, views)
-        gk2, per_vol, (gout, gimg, gfm, gti) = self._vjp_buf
+            # the views never move either: their pointer arrays are built once too
+            self._vjp_buf = (gk2, per_vol, self.plan.pointer_arrays(*views))
+        gk2, per_vol, ptrs = self._vjp_buf
         sets = []
         for (vi, e), (n_used, w) in self._zadj.items():
             v = self.volumes[vi]
@@ -228,7 +229,7 @@ class Ge3dForward:
             sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
                              n_comp=n_used, ky_segments=self.ky_segments, weights=w))
         zstage(sets, adjoint=True, stream=stream)
-        self.plan.vjp(gout, gimg, gfm, gti, stream, after_forward=after_forward)
+        self.plan.vjp_arrays(ptrs, stream, after_forward=after_forward)
         out = []
         for vi, v in enumerate(self.volumes):
             b = per_vol[vi]

@@ -615,6 +615,10 @@ class Plan:
     def vjp(self, grad_out, grad_images=None, grad_fmap=None, grad_t_inhom=None, stream=None, after_forward=False):
         """after_forward: the slice inputs are unchanged since this plan's last forward on `stream`;
         the adjoint reuses that forward's slice planes (mrs_plan_vjp_after_forward)."""
+        self.vjp_arrays(self.pointer_arrays(grad_out, grad_images, grad_fmap, grad_t_inhom), stream, after_forward)
+
+    def pointer_arrays(self, grad_out, grad_images=None, grad_fmap=None, grad_t_inhom=None):
+        """The per-unit device pointers of vjp's tensors as ctypes arrays (for vjp_arrays)."""
         n = len(self.units)
 
         def arr(ts):
@@ -624,10 +628,13 @@ class Plan:
                     a[i] = None if t is None else t.data_ptr()
             return a
 
+        return arr(grad_out), arr(grad_images), arr(grad_fmap), arr(grad_t_inhom)
+
+    def vjp_arrays(self, arrays, stream=None, after_forward=False):
+        """vjp on pointer arrays built once by pointer_arrays, for a caller whose tensors stay put."""
         err = _errbuf()
         fn = lib().mrs_plan_vjp_after_forward if after_forward else lib().mrs_plan_vjp
-        rc = fn(self._h, arr(grad_out), arr(grad_images), arr(grad_fmap), arr(grad_t_inhom), self._stream(stream), err,
-                len(err))
+        rc = fn(self._h, *arrays, self._stream(stream), err, len(err))
         _check(rc, err)
 
     def tune(self, trials=3, stream=None) -> str:
