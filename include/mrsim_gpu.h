@@ -292,6 +292,15 @@ typedef struct mrs_zset {
 /* Run sets on `stream` and wait for them (the call allocates and frees its descriptor copy). */
 int mrs_zstage(const mrs_zset* sets, size_t nsets, int32_t adjoint, void* stream, char* err, size_t errlen);
 
+/* The same partition transform with its descriptors kept on the device, for a caller that runs one
+ * set list many times (the adjoint of a 3D train, src/kspace3d.rs:90-100, 716-722): create uploads
+ * them once (blocking), run enqueues the sets on `stream` and returns at once, destroy frees them.
+ * The sets' buffers must stay allocated while the z-plan runs. */
+typedef struct mrs_zplan mrs_zplan;
+int mrs_zplan_create(const mrs_zset* sets, size_t nsets, int32_t adjoint, mrs_zplan** zplan, char* err, size_t errlen);
+int mrs_zplan_run(mrs_zplan* zplan, void* stream, char* err, size_t errlen);
+void mrs_zplan_destroy(mrs_zplan* zplan);
+
 /* Spikes and receiver noise of one volume's acquired 3D k-space (device, nz x n_coils x ny x nx complex64,
  * in place): partitions_to_slices (src/kspace3d.rs:235-264) before its inverse z-DFT.  `volume` is the
  * volume index g, `seed` the echo's seed (series seed ^ echo_salt(e)); acq gives n_spikes,

@@ -248,6 +248,64 @@ extern "C" int mrs_zstage(const mrs_zset* in, size_t nsets, int32_t adjoint, voi
     return MRS_OK;
 }
 
+// A z-stage whose descriptors live on the device: created once (one blocking upload), then run
+// asynchronously as often as needed, with no allocation or wait per run.
+struct mrs_zplan {
+    std::vector<mrs::ZSet> sets;
+    int off[4];
+    bool adjoint;
+    void* mem = nullptr;
+    mrs::ZSet* d_sets = nullptr;
+    mrs::ZBlock* d_blocks = nullptr;
+};
+
+extern "C" int mrs_zplan_create(const mrs_zset* in, size_t nsets, int32_t adjoint, mrs_zplan** out, char* err,
+                                size_t errlen) {
+    if (!out) return fail(err, errlen, MRS_E_INVALID, "null output handle");
+    *out = nullptr;
+    std::vector<mrs::ZSet> sets;
+    std::vector<mrs::ZBlock> blocks;
+    int off[4];
+    int rc = mrs::build_zsets(in, nsets, adjoint != 0, sets, blocks, off, err, errlen);
+    if (rc) return rc;
+    mrs_zplan* z = new mrs_zplan;
+    z->adjoint = adjoint != 0;
+    std::copy(off, off + 4, z->off);
+    const size_t sb = sets.size() * sizeof(mrs::ZSet), bb = blocks.size() * sizeof(mrs::ZBlock);
+    if (sb + bb > 0) {
+        hipError_t e = hipMalloc(&z->mem, sb + bb);
+        if (e != hipSuccess) {
+            delete z;
+            return fail(err, errlen, MRS_E_NOMEM, "hipMalloc: %s", hipGetErrorString(e));
+        }
+        z->d_sets = (mrs::ZSet*)z->mem;
+        z->d_blocks = (mrs::ZBlock*)((char*)z->mem + sb);
+        if ((e = hipMemcpy(z->d_sets, sets.data(), sb, hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(z->d_blocks, blocks.data(), bb, hipMemcpyHostToDevice)) != hipSuccess) {
+            hipFree(z->mem);
+            delete z;
+            return fail(err, errlen, MRS_E_DEVICE, "z-plan upload: %s", hipGetErrorString(e));
+        }
+    }
+    z->sets.swap(sets);
+    *out = z;
+    return MRS_OK;
+}
+
+extern "C" int mrs_zplan_run(mrs_zplan* z, void* stream, char* err, size_t errlen) {
+    if (!z) return fail(err, errlen, MRS_E_INVALID, "null z-plan");
+    if (z->sets.empty()) return MRS_OK;
+    const hipError_t e = mrs::enqueue_zsets(z->d_sets, z->d_blocks, z->off, z->adjoint, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "z-stage: %s", hipGetErrorString(e));
+    return MRS_OK;
+}
+
+extern "C" void mrs_zplan_destroy(mrs_zplan* z) {
+    if (!z) return;
+    if (z->mem) hipFree(z->mem);
+    delete z;
+}
+
 // partitions_to_slices' spikes and noise (src/kspace3d.rs:235-264) on one volume's acquired 3D k-space
 // [p][coil][ky][kx] (device), for volume index `volume` on `seed` (the echo's salted seed).
 extern "C" int mrs_partition_noise(mrs_c32* kspace, size_t nz, size_t n_coils, size_t nx, size_t ny,

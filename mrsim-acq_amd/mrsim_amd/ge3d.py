@@ -23,7 +23,7 @@ from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 from . import abi
-from .gpu import (Group, MrsError, Plan, Recon, Slice, Unit, ZSet, grappa_apply, grappa_fit, grase_block,
+from .gpu import (Group, MrsError, Plan, Recon, Slice, Unit, ZPlan, ZSet, grappa_apply, grappa_fit, grase_block,
                   partition_noise, sampling_mask, zstage)
 
 SEED_SALT_GRAPPA_CAL = 0x47525043414C
@@ -219,16 +219,15 @@ class Ge3dForward:
             # the views never move either: their pointer arrays are built once too
             self._vjp_buf = (gk2, per_vol, self.plan.pointer_arrays(*views))
         gk2, per_vol, ptrs = self._vjp_buf
-        sets = []
-        for (vi, e), (n_used, w) in self._zadj.items():
-            v = self.volumes[vi]
-            g = grad_out[vi][e]
-            if g.dtype != torch.complex64 or not g.is_contiguous() or g.shape != self.out[vi][e].shape:
-                raise ValueError(f"grad_out[{vi}][{e}] must be a contiguous complex64 tensor of shape "
-                                 f"{tuple(self.out[vi][e].shape)}")
-            sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
-                             n_comp=n_used, ky_segments=self.ky_segments, weights=w))
-        zstage(sets, adjoint=True, stream=stream)
+        # the adjoint partition transform's sets stay on the device while the cotangents do
+        sets = self._zadj_sets(grad_out, gk2)
+        key = tuple(z.inp.data_ptr() for z in sets)
+        zp = getattr(self, "_zadj_plan", None)
+        if zp is None or zp[0] != key:
+            if zp is not None:
+                zp[1].close()
+            self._zadj_plan = zp = (key, ZPlan(sets, adjoint=True))
+        zp[1].run(stream)
         self.plan.vjp_arrays(ptrs, stream, after_forward=after_forward)
         out = []
         for vi, v in enumerate(self.volumes):
@@ -241,6 +240,19 @@ class Ge3dForward:
             fm = b[1].sum(dim=(0, 1)).reshape(-1) if used else torch.zeros_like(v.fmap)
             out.append({"images": imgs, "fmap": fm, "t_inhom": tis})
         return out
+
+    def _zadj_sets(self, grad_out, gk2):
+        import torch
+        sets = []
+        for (vi, e), (n_used, w) in self._zadj.items():
+            v = self.volumes[vi]
+            g = grad_out[vi][e]
+            if g.dtype != torch.complex64 or not g.is_contiguous() or g.shape != self.out[vi][e].shape:
+                raise ValueError(f"grad_out[{vi}][{e}] must be a contiguous complex64 tensor of shape "
+                                 f"{tuple(self.out[vi][e].shape)}")
+            sets.append(ZSet(inp=g, out=gk2[(vi, e)], nz=v.nz, n_coils=v.coil_maps.shape[1], ny=v.ny, nx=v.nx,
+                             n_comp=n_used, ky_segments=self.ky_segments, weights=w))
+        return sets
 
     def _slices(self, vi, e, g, seed, acq):
         """Spikes and noise on a copy of the (volume, echo) 3D k-space, then the inverse z-DFT / nz:
@@ -317,6 +329,10 @@ class Ge3dForward:
         return self.plan.stage_times(iters)
 
     def close(self):
+        zp = getattr(self, "_zadj_plan", None)
+        if zp is not None:
+            zp[1].close()
+            self._zadj_plan = None
         self.plan.close()
 
 

@@ -78,6 +78,10 @@ def lib():
                                         ct.c_size_t, ct.c_double, ct.c_int32, ct.c_size_t, ct.c_double,
                                         ct.c_double] + E
     L.mrs_zstage.argtypes = [P(abi.ZSetDesc), ct.c_size_t, ct.c_int32, ct.c_void_p] + E
+    L.mrs_zplan_create.argtypes = [P(abi.ZSetDesc), ct.c_size_t, ct.c_int32, P(ct.c_void_p)] + E
+    L.mrs_zplan_run.argtypes = [ct.c_void_p, ct.c_void_p] + E
+    L.mrs_zplan_destroy.argtypes = [ct.c_void_p]
+    L.mrs_zplan_destroy.restype = None
     L.mrs_recon_create.argtypes = [ct.c_size_t, ct.c_size_t, ct.c_size_t, P(abi.Acquisition), P(ct.c_float),
                                    P(ct.c_void_p)] + E
     L.mrs_recon_run.argtypes = [ct.c_void_p, P(ct.c_void_p), P(ct.c_void_p), ct.c_size_t, ct.c_void_p] + E
@@ -440,8 +444,8 @@ def _zset_array(sets):
     return arr
 
 
-def zstage(sets: Sequence[ZSet], adjoint: bool = False, stream=None):
-    """mrs_zstage: run the sets on `stream` (torch's current stream by default) and wait."""
+def zstage_check(sets: Sequence[ZSet], adjoint: bool = False):
+    """Raise ValueError unless every set's tensors are device tensors of the set's shape."""
     import torch
     for i, z in enumerate(sets):
         ncols = z.n_coils * z.ny * z.nx
@@ -454,11 +458,49 @@ def zstage(sets: Sequence[ZSet], adjoint: bool = False, stream=None):
             raise ValueError(f"zset {i}: buffers smaller than the set's shape")
         if z.weights is not None:
             _check_dev(f"zset {i} weights", z.weights, torch.float32, z.nz * z.ky_segments * z.n_comp)
+
+
+def zstage(sets: Sequence[ZSet], adjoint: bool = False, stream=None):
+    """mrs_zstage: run the sets on `stream` (torch's current stream by default) and wait."""
+    import torch
+    zstage_check(sets, adjoint)
     arr = _zset_array(sets)
     if stream is None:
         stream = torch.cuda.current_stream()
     err = _errbuf()
     _check(lib().mrs_zstage(arr, len(sets), 1 if adjoint else 0, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+
+class ZPlan:
+    """mrs_zplan: a set list whose descriptors stay on the device; run() enqueues it without waiting.
+    The sets' tensors are kept referenced for the z-plan's life."""
+
+    def __init__(self, sets: Sequence[ZSet], adjoint: bool = False):
+        zstage_check(sets, adjoint)
+        self._keep = list(sets)
+        arr = _zset_array(sets)
+        h = ct.c_void_p()
+        err = _errbuf()
+        _check(lib().mrs_zplan_create(arr, len(sets), 1 if adjoint else 0, ct.byref(h), err, len(err)), err)
+        self._h = h
+
+    def run(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        err = _errbuf()
+        _check(lib().mrs_zplan_run(self._h, ct.c_void_p(stream.cuda_stream), err, len(err)), err)
+
+    def close(self):
+        if self._h:
+            lib().mrs_zplan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
