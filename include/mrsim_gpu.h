@@ -246,6 +246,16 @@ int mrs_plan_set_tiles(mrs_plan* plan, int32_t x_tile, int32_t line_tiles, int32
                        size_t errlen);
 int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tiles, int32_t* units_per_item);
 
+/* Slice-basis y-stage on (1, the default) or off (0).  With it on, the units of a slice that has
+ * four or more of them (its volumes; up to 8 coils, no eddy polynomial, at most 64 acquired lines
+ * and 256 padded sim rows) run in one kernel that generates the slice's per-line relaxation and
+ * phase factors once for all of them and puts every unit's coil-weighted images on the MFMA (the
+ * per-volume loop of acquire_volumes_complex::per_vol, src/kspace.rs:2012-2030, around
+ * build_coil_kspace_timed, :607-1042).  Other units keep the per-unit y-stage.  get_basis reports
+ * the setting and the number of eligible slices. */
+int mrs_plan_set_basis(mrs_plan* plan, int32_t on, char* err, size_t errlen);
+int mrs_plan_get_basis(const mrs_plan* plan, int32_t* on, int32_t* eligible_slices);
+
 /* ---- 3D gradient-echo train (stack of EPI blocks over nz partitions) ----------------------- */
 
 /* Within-echo timing of a GRASE/3D-GE block on ny lines: t_ms[ky] from the echo centre and the

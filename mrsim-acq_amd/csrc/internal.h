@@ -57,6 +57,8 @@ struct DevSlice {
     double eddy[3];
     int32_t s_owner;   // this slice writes the coil planes; otherwise they alias an earlier slice's
     int32_t p_owner;   // this slice writes the static planes (base .. rq); otherwise the same
+    int32_t prep_extra;   // plane forms written in addition to the plan's (PREP_S32 for the coil-plane
+                          // owner of slices whose volumes run in the slice-basis y-stage)
 };
 
 // Slice prep outputs: the split coil planes (MFMA forward) and/or the FP32 ones (plain, adjoint).
@@ -99,6 +101,13 @@ struct DevUnit {
     uint64_t seed;
     uint64_t coil0;
 };
+
+// A slice-basis set: up to kBasisMax units (volumes) of one slice whose y-stage shares the slice's
+// per-line basis; stored as [count, unit 0, ..., unit kBasisMax-1].
+static constexpr int kBasisMax = 20;
+static constexpr int kBasisStride = kBasisMax + 1;
+static constexpr int kBasisMin = 4;      // fewer units of a slice run in the per-unit y-stage
+static constexpr int kBasisMaxY = 256;   // largest padded sim-grid height (snyP) of the basis y-stage
 
 // Work item of the batched kernels.
 struct WorkItem {

@@ -22,6 +22,8 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
                               hipStream_t st);
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
                               const WorkItem* w, hipStream_t st);
+hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
+                               const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
 hipError_t set_yadjoint_lds_limit(size_t bytes);
 hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy, int snx, int sny, int nx, int LyP,
@@ -230,6 +232,18 @@ struct mrs_plan {
         std::vector<int32_t> usets;
         int32_t* d_usets = nullptr;
         size_t cap_items = 0, cap_vitems = 0, cap_usets = 0;
+        // slice-basis sets (kBasisStride ints each) and their (set, column) items, one list per
+        // pairs-per-wave count npw = 1..5 (index npw - 1)
+        std::vector<int32_t> bsets;
+        int32_t* d_bsets = nullptr;
+        size_t cap_bsets = 0;
+        struct BList {
+            std::vector<WorkItem> items;
+            WorkItem* d_items = nullptr;
+            size_t cap = 0;
+            std::vector<size_t> off;
+        };
+        BList bl[5];
     };
     std::vector<YClass> yclasses;
     struct XClass {
@@ -249,6 +263,9 @@ struct mrs_plan {
     hipEvent_t ev_prep = nullptr;
     std::vector<XClass> xclasses;
     int xt = 8, gp = 4, nv = 1;
+    // slice-basis y-stage for the volumes of a slice (on by default where a slice qualifies)
+    int basis = 1;
+    std::vector<uint8_t> basis_slice;   // the slice's units can run in the slice-basis y-stage
     int maxvox = 0;
     int maxC = 1;
     bool any_zero_rows = false, any_epilogue = false;
@@ -645,6 +662,10 @@ static void share_static_planes(mrs_plan* p) {
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     share_coil_planes(p);
     share_static_planes(p);
+    // the slice-basis y-stage reads the FP32 coil planes: their owner writes them in the forward too
+    for (DevSlice& sl : p->slices) sl.prep_extra = 0;
+    for (size_t s = 0; s < p->slices.size(); s++)
+        if (p->basis && p->basis_slice[s]) p->slices[p->s_owner[s]].prep_extra |= PREP_S32;
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -787,6 +808,18 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             p->yclasses.push_back({dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0, {}, nullptr, {}});
         }
     }
+    // slices whose volumes can share one slice-basis y-stage: up to 8 coils, no eddy polynomial, at
+    // most 64 acquired lines and kBasisMaxY padded rows, four or more units
+    p->basis_slice.assign(nslices, 0);
+    if (rc == MRS_OK && precision == MRS_PREC_SPLIT_BF16) {
+        std::vector<int> cnt(nslices, 0);
+        for (size_t u = 0; u < nunits; u++) cnt[units[u].slice]++;
+        for (size_t s = 0; s < nslices; s++) {
+            const DevGroup& dg = p->groups[slices[s].group].dev;
+            p->basis_slice[s] = cnt[s] >= kBasisMin && dg.RB == 1 && dg.eddy_strength == 0.0 && dg.Q <= 4 &&
+                                dg.LyP <= 64 && ((dg.sny + 15) & ~15) <= kBasisMaxY;
+        }
+    }
     if (rc == MRS_OK) {
         rc = plan_alloc(p, &p->d_groups, ngroups, err, errlen);
         if (!rc) rc = plan_alloc(p, &p->d_slices, nslices, err, errlen);
@@ -911,12 +944,18 @@ static bool set_eligible(const mrs_plan* p, const mrs_plan::YClass& c, int gp) {
     return p->nv > 1 && c.RB == 1 && gp >= 4 && !c.eddy && (p->nv == 2 || planes <= 3);
 }
 
+// Units of slices that qualify for the slice-basis y-stage (with basis on) go to sets of up to
+// kBasisMax units of one slice and chunk, near-equal in size, each listed as `count, units...`; a
+// set of n units runs in the kernel variant with ceil(ceil(n / 2) / 2) unit pairs per wave.
 static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs_plan::YClass& c, int xt, int gp,
                   std::vector<std::vector<WorkItem>>& single, std::vector<std::vector<WorkItem>>& multi,
-                  std::vector<int32_t>& usets) {
+                  std::vector<int32_t>& usets, std::vector<int32_t>& bsets,
+                  std::vector<std::vector<WorkItem>> (&bper)[5]) {
     single.assign(p->nchunks, {});
     multi.assign(p->nchunks, {});
     usets.clear();
+    bsets.clear();
+    for (auto& v : bper) v.assign(p->nchunks, {});
     const int nv = set_eligible(p, c, gp) ? p->nv : 1;
     size_t i = 0;
     while (i < order.size()) {
@@ -937,6 +976,23 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
         while (a < mine.size()) {
             size_t b = a;
             while (b < mine.size() && p->units[mine[b]].slice == p->units[mine[a]].slice) b++;
+            const int sl = p->units[mine[a]].slice;
+            if (p->basis && p->basis_slice[sl]) {
+                const size_t n = b - a, nsets = (n + kBasisMax - 1) / kBasisMax;
+                const int snx = p->groups[p->units[mine[a]].group].dev.snx;
+                size_t k = a;
+                for (size_t si = 0; si < nsets; si++) {
+                    const size_t cnt = n / nsets + (si < n % nsets ? 1 : 0);
+                    const int npw = (int)(((cnt + 1) / 2 + 1) / 2);
+                    const int32_t sidx = (int32_t)(bsets.size() / kBasisStride);
+                    bsets.push_back((int32_t)cnt);
+                    for (int v = 0; v < kBasisMax; v++) bsets.push_back((int32_t)mine[k + std::min<size_t>(v, cnt - 1)]);
+                    k += cnt;
+                    for (int x = 0; x < snx; x++) bper[npw - 1][ch].push_back({sidx, x});
+                }
+                a = b;
+                continue;
+            }
             size_t k = a;
             for (; nv > 1 && k + nv <= b; k += nv) {
                 sets.push_back((int32_t)(usets.size() / nv));
@@ -964,13 +1020,15 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
 // Rebuilds and uploads a class's work lists for the current tiles.
 static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<size_t>& order, char* err,
                          size_t errlen) {
-    std::vector<std::vector<WorkItem>> single, multi;
-    std::vector<int32_t> usets;
-    ylist(p, order, c, p->xt, p->gp, single, multi, usets);
+    std::vector<std::vector<WorkItem>> single, multi, bper[5];
+    std::vector<int32_t> usets, bsets;
+    ylist(p, order, c, p->xt, p->gp, single, multi, usets, bsets, bper);
     c.nv = set_eligible(p, c, p->gp) ? p->nv : 1;
     c.items = chunked(single, c.off);
     c.vitems = chunked(multi, c.voff);
     c.usets = usets;
+    c.bsets = bsets;
+    for (int k = 0; k < 5; k++) c.bl[k].items = chunked(bper[k], c.bl[k].off);
     auto put = [&](auto*& dptr, size_t& cap, const auto& host) -> int {
         using T = typename std::decay_t<decltype(host)>::value_type;
         const size_t n = std::max<size_t>(host.size(), 1);
@@ -985,11 +1043,20 @@ static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<siz
     int rc = put(c.d_items, c.cap_items, c.items);
     if (!rc) rc = put(c.d_vitems, c.cap_vitems, c.vitems);
     if (!rc) rc = put(c.d_usets, c.cap_usets, c.usets);
+    if (!rc) rc = put(c.d_bsets, c.cap_bsets, c.bsets);
+    for (int k = 0; k < 5 && !rc; k++) rc = put(c.bl[k].d_items, c.bl[k].cap, c.bl[k].items);
     return rc;
 }
 
 static hipError_t launch_yclass(const mrs_plan* p, const mrs_plan::YClass& c, int k, hipStream_t st) {
     hipError_t e = hipSuccess;
+    for (int n = 0; n < 5 && e == hipSuccess; n++) {
+        const auto& b = c.bl[n];
+        if (b.off.size() > (size_t)k + 1 && b.off[k + 1] > b.off[k])
+            e = launch_ystage_basis(c.Q, c.cplx != 0, n + 1, (int)(b.off[k + 1] - b.off[k]), p->d_units, p->d_slices,
+                                    p->d_groups, b.d_items + b.off[k], c.d_bsets, st);
+    }
+    if (e != hipSuccess) return e;
     const size_t va = c.voff[k], vb = c.voff[k + 1];
     if (vb > va)
         e = launch_ystage_mfma(c.Q, c.RB, p->gp, c.nv, c.cplx != 0, c.eddy != 0, (int)(vb - va), p->xt, p->d_units,
@@ -1000,10 +1067,18 @@ static hipError_t launch_yclass(const mrs_plan* p, const mrs_plan::YClass& c, in
                               p->d_slices, p->d_groups, c.d_items + a, c.d_usets, st);
 }
 
-static int set_tiles(mrs_plan* p, int xt, int gp, int nv, char* err, size_t errlen) {
+static int set_tiles(mrs_plan* p, int xt, int gp, int nv, int basis, char* err, size_t errlen) {
+    // the work lists and descriptors below may still be read by launches in flight
+    HIP_TRY(hipDeviceSynchronize());
     p->xt = xt;
     p->gp = gp;
     p->nv = nv;
+    if (basis != p->basis) {
+        p->basis = basis;
+        p->planes_current = false;   // the FP32 coil planes are written only with basis on
+        int rc = upload_descs(p, err, errlen);
+        if (rc) return rc;
+    }
     const std::vector<size_t> order = unit_order(p);
     for (auto& c : p->yclasses) {
         int rc = upload_ylists(p, c, order, err, errlen);
@@ -1276,6 +1351,14 @@ extern "C" int mrs_plan_describe(const mrs_plan* p, char* out, size_t outlen) {
              p->groups.size(), p->slices.size(), p->units.size(), p->nchunks, p->xt, p->gp, p->nv,
              p->precision == MRS_PREC_SPLIT_BF16 ? "split-bf16" : "fp32");
     r += line;
+    {
+        size_t nb = 0, sets = 0;
+        for (uint8_t b : p->basis_slice) nb += b ? 1 : 0;
+        for (const auto& c : p->yclasses) sets += c.bsets.size() / kBasisStride;
+        snprintf(line, sizeof line, "slice basis %s: %zu eligible slices, %zu unit sets\n", p->basis ? "on" : "off", nb,
+                 sets);
+        r += line;
+    }
     size_t owners = 0;
     for (const DevSlice& s : p->slices) owners += s.s_owner ? 1 : 0;
     snprintf(line, sizeof line, "coil planes: %zu distinct sets\n", owners);
@@ -1339,13 +1422,17 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
-    int bxt = p->xt, bgp = p->gp, bnv = p->nv;
+    int bxt = p->xt, bgp = p->gp, bnv = p->nv, bbs = p->basis;
     std::string rep;
+    std::vector<int> bss = {0};
+    for (uint8_t b : p->basis_slice)
+        if (b) { bss.push_back(1); break; }
+    for (int bs : bss)
     for (int xt : xts)
         for (int gp : gps)
             for (int nv : nvs) {
                 if (nv > 1 && gp != 4) continue;
-                int rc = set_tiles(p, xt, gp, nv, err, errlen);
+                int rc = set_tiles(p, xt, gp, nv, bs, err, errlen);
                 if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
                 rc = enqueue_forward(p, st, err, errlen);  // warm-up
                 if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
@@ -1357,15 +1444,15 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
                 ms /= (float)trials;
                 char line[96];
-                snprintf(line, sizeof line, "xt=%d gp=%d nv=%d %.4f ms\n", xt, gp, nv, ms);
+                snprintf(line, sizeof line, "basis=%d xt=%d gp=%d nv=%d %.4f ms\n", bs, xt, gp, nv, ms);
                 rep += line;
-                if (ms < best) { best = ms; bxt = xt; bgp = gp; bnv = nv; }
+                if (ms < best) { best = ms; bxt = xt; bgp = gp; bnv = nv; bbs = bs; }
             }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    int rc = set_tiles(p, bxt, bgp, bnv, err, errlen);
+    int rc = set_tiles(p, bxt, bgp, bnv, bbs, err, errlen);
     char line[96];
-    snprintf(line, sizeof line, "best xt=%d gp=%d nv=%d %.4f ms\n", bxt, bgp, bnv, best);
+    snprintf(line, sizeof line, "best basis=%d xt=%d gp=%d nv=%d %.4f ms\n", bbs, bxt, bgp, bnv, best);
     rep += line;
     if (report && reportlen) {
         std::strncpy(report, rep.c_str(), reportlen - 1);
@@ -1383,7 +1470,24 @@ extern "C" int mrs_plan_set_tiles(mrs_plan* p, int32_t xt, int32_t gp, int32_t u
         return fail(err, errlen, MRS_E_INVALID, "units per item: 1, 2 or 4");
     if (units_per_item > 1 && gp != 4)
         return fail(err, errlen, MRS_E_INVALID, "units per item above 1 need 4 line tiles per pass");
-    return set_tiles(p, xt, gp, units_per_item, err, errlen);
+    return set_tiles(p, xt, gp, units_per_item, p->basis, err, errlen);
+}
+
+extern "C" int mrs_plan_set_basis(mrs_plan* p, int32_t on, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    if (on != 0 && on != 1) return fail(err, errlen, MRS_E_INVALID, "slice basis: 0 or 1");
+    return set_tiles(p, p->xt, p->gp, p->nv, on, err, errlen);
+}
+
+extern "C" int mrs_plan_get_basis(const mrs_plan* p, int32_t* on, int32_t* slices) {
+    if (!p) return MRS_E_INVALID;
+    if (on) *on = p->basis;
+    if (slices) {
+        int32_t n = 0;
+        for (uint8_t b : p->basis_slice) n += b ? 1 : 0;
+        *slices = n;
+    }
+    return MRS_OK;
 }
 
 extern "C" int mrs_plan_get_tiles(const mrs_plan* p, int32_t* xt, int32_t* gp, int32_t* units_per_item) {

@@ -1,0 +1,342 @@
+// ybasis.hip -- the y-stage of the volumes of one slice, evaluated together (src/kspace.rs:940-975 per
+// coil and volume in the reference).
+//
+// The volumes of a slice share every per-voxel quantity but their compartment images m_q, so the
+// y-contraction of unit u, coil c, line l at column x is written as one product of a per-unit and a
+// per-slice factor:
+//   G_u[c, l] = sum_{q, y} A_u[(c), (q, y)] * B[(q, y), l]
+//   A_u(c, q, y) = S_c(x, y) * m_{u,q}(x, y)                      (line-independent)
+//   B(q, y, l)   = 2^{trf_l nr2_q + tau_l nri_q} * cis(2 pi cyc)  (unit-independent)
+//   cyc          = base + rate * t_l + kyn_l * (y - sys - yoff)   (formed and reduced in FP64)
+// B is generated once per (slice, column, chunk of 16 y) into LDS and read by every unit of the set;
+// A is generated in registers once per (unit, column, y chunk, compartment) and used by all lines.
+// Both are split into bf16 hi + lo and multiplied on v_mfma_f32_16x16x32_bf16 as hi*hi + hi*lo +
+// lo*hi with FP32 accumulation.
+//
+// Tile layout.  A 16-row tile holds two units x 8 coils, row = 8 * (unit of the pair) + coil, so
+// every lane forms its own row's natural complex A (re | im << 16 per y) with no duplicated work.
+// The complex product rides on B: with K = (y, re/im),
+//   B_re[(y, re), l] = B_re,  B_re[(y, im), l] = -B_im   ->  Re G
+//   B_im[(y, re), l] = B_im,  B_im[(y, im), l] =  B_re   ->  Im G
+// and the B_im form is the B_re form with its halves swapped and the low half's sign flipped.
+//
+// Work decomposition.  Block = 4 waves = one (set, column x).  A set holds up to 4 * NPW units of one
+// slice (kBasisMax at most).  Wave w takes line tiles 2 * (w & 1) and 2 * (w & 1) + 1 (64 acquired
+// lines at most) and the pairs NPW * (w >> 1) .. NPW * (w >> 1) + NPW - 1.  Per 16-y chunk:
+//   1. barrier; B (generated in registers after the previous chunk's contraction) is written to LDS
+//      in the MFMA B-operand layout (both forms, hi and lo), with the units' images of the chunk
+//      (loaded into registers one chunk ahead);
+//   2. barrier; each wave contracts its pairs against its line tiles, with the coil maps (FP32,
+//      [c][x][y]) in registers loaded one chunk ahead;
+//   3. each wave generates B of the next chunk for 4 of its y (one per lane's line) from the
+//      column's static planes, staged in LDS once per block.
+// Two blocks share a CU (at most 256 VGPRs per wave), so one block's B generation overlaps the
+// other's matrix work.
+#include "devcommon.h"
+
+namespace mrs {
+
+template <int Q, bool CPLX, int NPW>
+struct BasisSmem {
+    static constexpr int NU = 4 * NPW;        // units of a set
+    static constexpr int CM = CPLX ? 2 : 1;   // floats per image value
+    static constexpr int IMG_DW = NU * Q * 16 * CM;   // image dwords of one chunk
+    u32x4 b[Q][4][2][2][64];                  // [q][line tile][form re/im][hi/lo][lane]
+    float img[IMG_DW];                        // [unit][q][y 16][part] of the current chunk
+    double base[kBasisMaxY];                  // the column's static planes, y = 0 .. snyP-1
+    float rate[kBasisMaxY];
+    float nr2[Q][kBasisMaxY];
+    float nri[Q][kBasisMaxY];
+};
+
+template <int Q, bool CPLX, int NPW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
+                  const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work,
+                  const int32_t* __restrict__ bsets) {
+    using L = BasisSmem<Q, CPLX, NPW>;
+    constexpr int NU = L::NU, CM = L::CM;
+    constexpr int NSLOT = (L::IMG_DW + 255) / 256;   // image values per thread and chunk
+    __shared__ __attribute__((aligned(16))) L sm;
+
+    const WorkItem wi = work[blockIdx.x];
+    const int32_t* set = bsets + (size_t)wi.unit * kBasisStride;
+    const int nu = set[0];
+    const int x = wi.tile;
+    const DevUnit& u0 = units[set[1]];
+    const DevGroup& gr = groups[u0.group];
+    const DevSlice& sl = slices[u0.slice];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lq = lane >> 4, ln = lane & 15;
+    const int snx = gr.snx, sny = gr.sny;
+    const int snyP = (sny + 15) & ~15, snxP = (snx + 15) & ~15;
+    const int C = gr.C, LyP = gr.LyP, nt = LyP >> 4;
+    const int nch = snyP >> 4;
+    const uint32_t P = (uint32_t)(snx * sny);
+    const double sys_yoff = gr.sys_yoff;
+
+    // this lane's line of the B generation (lane = line; LyP <= 64)
+    double t_s = 0.0, kyn = 0.0;
+    float trf = 0.0f, tau = 0.0f;
+    bool ok = false;
+    if (lane < LyP) {
+        const LineRec& Lr = gr.lines[lane];
+        t_s = Lr.t_s;
+        kyn = Lr.kyn;
+        trf = Lr.trf;
+        tau = Lr.tau;
+        ok = Lr.ky >= 0;
+    }
+
+    // Image values: thread slot k holds dword d = 256 k + tid of a chunk in the LDS order
+    // (unit, q, y, part); units past the set's count read its last unit (their rows are never
+    // stored).  Loaded into registers one chunk ahead, written to LDS after the chunk's first barrier.
+    const GAS float* isrc[NSLOT];
+    int iy[NSLOT];
+#pragma unroll
+    for (int k = 0; k < NSLOT; k++) {
+        const int d = 256 * k + tid;
+        isrc[k] = nullptr;
+        iy[k] = 0;
+        if (d < L::IMG_DW) {
+            const int uu = d / (Q * 16 * CM), r = d % (Q * 16 * CM);
+            const int q = r / (16 * CM), yy = (r / CM) & 15, pt = r % CM;
+            const float* img = (const float*)units[set[1 + min(uu, nu - 1)]].images;
+            isrc[k] = gld(img) + ((size_t)q * P + x) * CM + pt;
+            iy[k] = yy;
+        }
+    }
+    auto load_images = [&](int ch, float (&v)[NSLOT]) {
+#pragma unroll
+        for (int k = 0; k < NSLOT; k++) {
+            v[k] = 0.0f;
+            if (isrc[k]) v[k] = isrc[k][(size_t)snx * CM * min(ch * 16 + iy[k], sny - 1)];
+        }
+    };
+
+    // coil maps of this lane's row (coil ln & 7) at the chunk's 4 y of the lane: FP32, [c][x][y]
+    const int coil = ln & 7;
+    const bool has_coil = coil < C;
+    const GAS float2* sbase = gld(sl.s32) + ((size_t)(has_coil ? coil : 0) * snxP + x) * snyP + 4 * lq;
+    auto load_s = [&](int ch, f32x4& s01, f32x4& s23) {
+        s01 = s23 = f32x4{0.f, 0.f, 0.f, 0.f};
+        if (has_coil) {
+            const GAS f32x4* p = (const GAS f32x4*)(sbase + ch * 16);
+            s01 = p[0];
+            s23 = p[1];
+        }
+    };
+
+    // B of this wave's quad of y (ch * 16 + 4 wave .. +3) at the lane's line, split, in the B_re form,
+    // from the column's static planes in LDS (wave-uniform addresses: broadcast reads)
+    auto gen_b = [&](int ch, u32x4 (&hre)[Q], u32x4 (&lre)[Q]) {
+        const int yq = ch * 16 + 4 * wave;
+        const f64x2 b01 = *reinterpret_cast<const f64x2*>(&sm.base[yq]);
+        const f64x2 b23 = *reinterpret_cast<const f64x2*>(&sm.base[yq + 2]);
+        const f32x4 r4 = *reinterpret_cast<const f32x4*>(&sm.rate[yq]);
+        const double bb[4] = {b01.x, b01.y, b23.x, b23.y};
+        f32x4 a2[Q], ai[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            a2[q] = *reinterpret_cast<const f32x4*>(&sm.nr2[q][yq]);
+            ai[q] = *reinterpret_cast<const f32x4*>(&sm.nri[q][yq]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const double yy = (double)(yq + j) - sys_yoff;
+            double cyc = __builtin_fma(kyn, yy, bb[j]);
+            cyc = __builtin_fma((double)r4[j], t_s, cyc);
+            float sn, cs;
+            sincos_turns(cyc, sn, cs);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(trf, a2[q][j], tau * ai[q][j]));
+                dq = ok ? dq : 0.0f;
+                uint32_t hh, ll;
+                split2(dq * cs, -(dq * sn), hh, ll);
+                hre[q][j] = hh;
+                lre[q][j] = ll;
+            }
+        }
+    };
+
+    const int h = wave & 1, ps = wave >> 1;
+    f32x4 acc[NPW][2][2];
+#pragma unroll
+    for (int i = 0; i < NPW; i++)
+#pragma unroll
+        for (int t = 0; t < 2; t++) acc[i][t][0] = acc[i][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // prologue: the column's static planes into LDS (once per block), chunk 0's operands
+    {
+        const uint32_t o = (uint32_t)(x * snyP), PPl = (uint32_t)(snxP * snyP);
+        for (int i = tid; i < snyP; i += 256) {
+            sm.base[i] = gld(sl.base)[o + i];
+            sm.rate[i] = gld(sl.rate)[o + i];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                sm.nr2[q][i] = gld(sl.nr2)[q * PPl + o + i];
+                sm.nri[q][i] = gld(sl.nri)[q * PPl + o + i];
+            }
+        }
+    }
+    f32x4 sn01, sn23;
+    float inext[NSLOT];
+    load_s(0, sn01, sn23);
+    load_images(0, inext);
+    __syncthreads();
+    u32x4 hre[Q], lre[Q];
+    gen_b(0, hre, lre);
+
+    for (int ch = 0; ch < nch; ch++) {
+        // this chunk's operands (loaded one chunk ahead); the next chunk's load during this one
+        const f32x4 s01 = sn01, s23 = sn23;
+        float icur[NSLOT];
+#pragma unroll
+        for (int k = 0; k < NSLOT; k++) icur[k] = inext[k];
+        if (ch + 1 < nch) {
+            load_s(ch + 1, sn01, sn23);
+            load_images(ch + 1, inext);
+        }
+        __syncthreads();   // every wave is done reading the previous chunk's B and images
+        // B of the chunk (both forms) and the units' images into LDS.  Lanes past the acquired
+        // lines write zero B into tiles no wave reads.
+        {
+            const int lt = lane >> 4, slot = wave * 16 + (lane & 15);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                u32x4 him, lim;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    him[j] = __builtin_amdgcn_alignbit(hre[q][j], hre[q][j], 16) ^ 0x8000u;
+                    lim[j] = __builtin_amdgcn_alignbit(lre[q][j], lre[q][j], 16) ^ 0x8000u;
+                }
+                sm.b[q][lt][0][0][slot] = hre[q];
+                sm.b[q][lt][0][1][slot] = lre[q];
+                sm.b[q][lt][1][0][slot] = him;
+                sm.b[q][lt][1][1][slot] = lim;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NSLOT; k++)
+            if (256 * k + tid < L::IMG_DW) sm.img[256 * k + tid] = icur[k];
+        __syncthreads();
+
+        // contraction of this wave's pairs against its line tiles (a wave whose tiles lie past the
+        // acquired lines multiplies unused data; its rows are not stored)
+        const float sr[4] = {s01.x, s01.z, s23.x, s23.z};
+        const float si[4] = {s01.y, s01.w, s23.y, s23.w};
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            u32x4 bh[2][2], bl[2][2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    bh[t][f] = sm.b[q][2 * h + t][f][0][lane];
+                    bl[t][f] = sm.b[q][2 * h + t][f][1][lane];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NPW; i++) {
+                const int ul = 2 * (ps * NPW + i) + (ln >> 3);
+                const float* mp = sm.img + ((ul * Q + q) * 16 + 4 * lq) * CM;
+                u32x4 ah, al;
+                if constexpr (CPLX) {
+                    const f32x4 m01 = *reinterpret_cast<const f32x4*>(mp);
+                    const f32x4 m23 = *reinterpret_cast<const f32x4*>(mp + 4);
+                    const float mr[4] = {m01.x, m01.z, m23.x, m23.z};
+                    const float mi[4] = {m01.y, m01.w, m23.y, m23.w};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float ar = __builtin_fmaf(sr[j], mr[j], -(si[j] * mi[j]));
+                        const float aim = __builtin_fmaf(sr[j], mi[j], si[j] * mr[j]);
+                        uint32_t hh, ll;
+                        split2(ar, aim, hh, ll);
+                        ah[j] = hh;
+                        al[j] = ll;
+                    }
+                } else {
+                    const f32x4 m = *reinterpret_cast<const f32x4*>(mp);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        uint32_t hh, ll;
+                        split2(sr[j] * m[j], si[j] * m[j], hh, ll);
+                        ah[j] = hh;
+                        al[j] = ll;
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    acc[i][t][0] = mfma3(ah, al, bh[t][0], bl[t][0], acc[i][t][0]);
+                    acc[i][t][1] = mfma3(ah, al, bh[t][1], bl[t][1], acc[i][t][1]);
+                }
+            }
+        }
+        // B of the next chunk, while other waves still contract this one
+        if (ch + 1 < nch) gen_b(ch + 1, hre, lre);
+    }
+
+    // G_u[x][line][coil]: lane holds rows 4 lq .. 4 lq + 3 of its tiles: unit 2 * pair + (lq >> 1) of
+    // the set, coils 4 (lq & 1) .. +3, line 16 * tile + ln
+#pragma unroll
+    for (int i = 0; i < NPW; i++) {
+        const int ul = 2 * (ps * NPW + i) + (lq >> 1);
+        if (ul >= nu) continue;
+        float2* gb = units[set[1 + ul]].gbuf;
+        const int c0 = 4 * (lq & 1);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            const int lt = 2 * h + t;
+            if (lt >= nt) continue;
+            const f32x4 re = acc[i][t][0], im = acc[i][t][1];
+            GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gb) + ((size_t)x * LyP + lt * 16 + ln) * C);
+            if (C == 8) {
+                // 16-byte aligned: (x*LyP + line)*8 + c0 is a multiple of 4 complex values
+                GAS f32x4* d4 = reinterpret_cast<GAS f32x4*>(row + c0);
+                d4[0] = f32x4{re[0], im[0], re[1], im[1]};
+                d4[1] = f32x4{re[2], im[2], re[3], im[3]};
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    if (c0 + r < C) row[c0 + r] = f32x2{re[r], im[r]};
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------- launch
+template <int Q, bool CPLX>
+static hipError_t blaunch_npw(int npw, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                              const WorkItem* w, const int32_t* sets, hipStream_t st) {
+    dim3 grid(nwork), block(256);
+    switch (npw) {
+    case 1: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 1>), grid, block, 0, st, u, s, g, w, sets); break;
+    case 2: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 2>), grid, block, 0, st, u, s, g, w, sets); break;
+    case 3: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 3>), grid, block, 0, st, u, s, g, w, sets); break;
+    case 4: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 4>), grid, block, 0, st, u, s, g, w, sets); break;
+    case 5: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 5>), grid, block, 0, st, u, s, g, w, sets); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// npw = pairs of units per wave (a set holds up to 4 * npw units); Q = 1..4 compartments.
+hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
+                               const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    switch (Q) {
+    case 1: return cplx ? blaunch_npw<1, true>(npw, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<1, false>(npw, nwork, u, s, g, w, sets, st);
+    case 2: return cplx ? blaunch_npw<2, true>(npw, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<2, false>(npw, nwork, u, s, g, w, sets, st);
+    case 3: return cplx ? blaunch_npw<3, true>(npw, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<3, false>(npw, nwork, u, s, g, w, sets, st);
+    case 4: return cplx ? blaunch_npw<4, true>(npw, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<4, false>(npw, nwork, u, s, g, w, sets, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mrs

@@ -1,34 +1,43 @@
-"""Builds libmrsim_gpu.so (the HIP kernels + C ABI) for gfx950 with hipcc, in-tree."""
+"""Builds libmrsim_gpu.so (the HIP kernels + C ABI) for gfx950 with hipcc, in-tree.  Each source is
+compiled to an object in parallel, then the objects are linked into the shared library."""
 from __future__ import annotations
 
 import os
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(ROOT, "build")
 LIB = os.path.join(OUT_DIR, "libmrsim_gpu.so")
-SOURCES = ["kernels.hip", "plan.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip", "nufft.hip"]
+SOURCES = ["kernels.hip", "ybasis.hip", "plan.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip", "nufft.hip"]
 
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # no packed FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-command-line-argument",
 ]
 
 
-def build(verbose: bool = False, extra=None) -> str:
-    os.makedirs(OUT_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = ["hipcc", *FLAGS, *(extra or []), *srcs, "-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
+
+
+def build(verbose: bool = False, extra=None) -> str:
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objs = [os.path.join(OUT_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+    cmds = [["hipcc", *FLAGS, *(extra or []), "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = LIB + ".tmp"
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -66,6 +66,8 @@ def lib():
     L.mrs_plan_tune.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int32, ct.c_char_p, ct.c_size_t]
     L.mrs_plan_set_tiles.argtypes = [ct.c_void_p, ct.c_int32, ct.c_int32, ct.c_int32] + E
     L.mrs_plan_get_tiles.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32), P(ct.c_int32)]
+    L.mrs_plan_set_basis.argtypes = [ct.c_void_p, ct.c_int32] + E
+    L.mrs_plan_get_basis.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32)]
     L.mrs_version.restype = ct.c_char_p
     L.mrs_sizeof.argtypes = [ct.c_char_p]
     L.mrs_sizeof.restype = ct.c_size_t
@@ -694,6 +696,17 @@ class Plan:
         a, b, c = ct.c_int32(), ct.c_int32(), ct.c_int32()
         lib().mrs_plan_get_tiles(self._h, ct.byref(a), ct.byref(b), ct.byref(c))
         return a.value, b.value, c.value
+
+    def set_basis(self, on: bool):
+        """Run the volumes of qualifying slices in the slice-basis y-stage (True) or per unit (False)."""
+        err = _errbuf()
+        _check(lib().mrs_plan_set_basis(self._h, 1 if on else 0, err, len(err)), err)
+
+    def basis(self):
+        """(on, number of slices that qualify for the slice-basis y-stage)."""
+        a, b = ct.c_int32(), ct.c_int32()
+        lib().mrs_plan_get_basis(self._h, ct.byref(a), ct.byref(b))
+        return bool(a.value), b.value
 
     def set_zstage(self, sets: Sequence["ZSet"]):
         """Attach partition transforms that run after the 2D forward (and inside its graph)."""

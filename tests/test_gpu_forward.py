@@ -209,9 +209,10 @@ def test_graph_replay_tuning_and_determinism():
     plan.close()
 
 
-def _set_plan_outputs(w, tiles):
+def _set_plan_outputs(w, tiles, basis=False):
     import torch
     plan, outs, keep = workloads.to_plan(w)
+    plan.set_basis(basis)
     plan.set_tiles(*tiles)
     assert plan.tiles() == tuple(tiles)
     plan.forward()
@@ -320,3 +321,55 @@ def test_echoes_share_static_planes_until_an_input_changes():
     plan.set_pointers(slices=[slices[0], gpu.Slice(1, f2_d, m_d, t2=t2_d)])
     check([fmap, fmap2])
     plan.close()
+
+
+def _basis_workload(name):
+    """Workloads of the slice-basis y-stage: the volumes of each slice in one kernel."""
+    from mrsim_amd.workloads import Stream, Workload, _slice, _volume_images
+    if name == "batch_q3_v20":
+        return workloads.make("batch_128", n_slices=2, n_volumes=20)
+    if name == "batch_q3_v7":      # an odd set: the last pair has one unit
+        return workloads.make("batch_128", n_slices=2, n_volumes=7)
+    if name == "batch_q3_v45":     # three sets of 15 units per slice
+        return workloads.make("batch_128", n_slices=1, n_volumes=45)
+    # small grids: 40^2 -> 20^2 (three 16-row chunks, the last one ragged; 20 lines padded to 32: the
+    # upper line tiles are idle),
+    # 3 coils (rows of absent coils), Q = 1, 2 or 4, real or complex images
+    Q = {"small_q1": 1, "small_q2_cplx": 2, "small_q4": 4}[name]
+    cplx = name.endswith("cplx")
+    C = 3
+    acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=C)
+    w = Workload(name, 40, 40, 20, 20, C, Q, acq, complex_images=cplx)
+    for s in range(2):
+        st = Stream(0x5EED0B00 + 31 * s + Q)
+        sd = _slice(st, 40, 40, Q, C, 16.0, 24.0)
+        w.slices.append(sd)
+        for g in range(5 + s):
+            img = _volume_images(st, sd, g)
+            if cplx:
+                img = (img * np.exp(1j * sd.phase0)[None, :]).astype(np.complex64)
+            w.units.append((s, img))
+    return w
+
+
+@pytest.mark.parametrize("name", ["batch_q3_v20", "batch_q3_v7", "batch_q3_v45", "small_q1", "small_q2_cplx",
+                                  "small_q4"])
+def test_slice_basis_matches_per_unit_and_oracle(name):
+    # The volumes of a slice in the slice-basis y-stage (per-slice phase and decay factors shared by
+    # every unit, per-unit coil-weighted images on the MFMA) against the per-unit y-stage and the
+    # FP64 oracle, both at the split-BF16 tolerance.
+    w = _basis_workload(name)
+    per_unit, _ = _set_plan_outputs(w, (8, 4, 1), basis=False)
+    basis, desc = _set_plan_outputs(w, (8, 4, 1), basis=True)
+    assert "slice basis on" in desc and " 0 unit sets" not in desc, desc
+    worst = max(rel(a, b) for a, b in zip(basis, per_unit))
+    print(f"{name}: max rel diff basis vs per-unit {worst:.2e}", flush=True)
+    assert worst <= TOL
+    for u in sorted({0, 1, len(w.units) // 2, len(w.units) - 1}):
+        e = rel(basis[u], O.forward_workload_unit(w, u))
+        print(f"  unit {u}: rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e <= TOL
+    nx, ny = w.nx, w.ny
+    mask = O.sampling_mask(nx, ny, w.acq).reshape(ny, nx)
+    for k in basis:
+        assert np.all(k[:, ~mask] == 0)
