@@ -63,6 +63,12 @@ __device__ __forceinline__ u32x4 rows_of(u32x4 p, int part) {
     return r;
 }
 
+// One bf16 MFMA 16x16x32 with FP32 accumulation of packed operands.
+__device__ __forceinline__ f32x4 mfma_bf16(u32x4 a, u32x4 b, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                   0, 0, 0);
+}
+
 __device__ __forceinline__ f32x4 mfma3(u32x4 ahi, u32x4 alo, u32x4 bhi, u32x4 blo, f32x4 acc) {
     bf16x8 Ah = __builtin_bit_cast(bf16x8, ahi), Al = __builtin_bit_cast(bf16x8, alo);
     bf16x8 Bh = __builtin_bit_cast(bf16x8, bhi), Bl = __builtin_bit_cast(bf16x8, blo);

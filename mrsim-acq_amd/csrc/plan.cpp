@@ -233,7 +233,7 @@ struct mrs_plan {
         int32_t* d_usets = nullptr;
         size_t cap_items = 0, cap_vitems = 0, cap_usets = 0;
         // slice-basis sets (kBasisStride ints each) and their (set, column) items, one list per
-        // pairs-per-wave count npw = 1..5 (index npw - 1)
+        // pairs-per-wave count npf = 1..3 (index npf - 1)
         std::vector<int32_t> bsets;
         int32_t* d_bsets = nullptr;
         size_t cap_bsets = 0;
@@ -243,7 +243,7 @@ struct mrs_plan {
             size_t cap = 0;
             std::vector<size_t> off;
         };
-        BList bl[5];
+        BList bl[3];
     };
     std::vector<YClass> yclasses;
     struct XClass {
@@ -946,11 +946,11 @@ static bool set_eligible(const mrs_plan* p, const mrs_plan::YClass& c, int gp) {
 
 // Units of slices that qualify for the slice-basis y-stage (with basis on) go to sets of up to
 // kBasisMax units of one slice and chunk, near-equal in size, each listed as `count, units...`; a
-// set of n units runs in the kernel variant with ceil(ceil(n / 2) / 2) unit pairs per wave.
+// set of n units runs in the kernel variant with ceil(ceil(n / 2) / 4) unit pairs per wave.
 static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs_plan::YClass& c, int xt, int gp,
                   std::vector<std::vector<WorkItem>>& single, std::vector<std::vector<WorkItem>>& multi,
                   std::vector<int32_t>& usets, std::vector<int32_t>& bsets,
-                  std::vector<std::vector<WorkItem>> (&bper)[5]) {
+                  std::vector<std::vector<WorkItem>> (&bper)[3]) {
     single.assign(p->nchunks, {});
     multi.assign(p->nchunks, {});
     usets.clear();
@@ -983,12 +983,12 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
                 size_t k = a;
                 for (size_t si = 0; si < nsets; si++) {
                     const size_t cnt = n / nsets + (si < n % nsets ? 1 : 0);
-                    const int npw = (int)(((cnt + 1) / 2 + 1) / 2);
+                    const int npf = (int)(((cnt + 1) / 2 + 3) / 4);   // unit pairs per wave
                     const int32_t sidx = (int32_t)(bsets.size() / kBasisStride);
                     bsets.push_back((int32_t)cnt);
                     for (int v = 0; v < kBasisMax; v++) bsets.push_back((int32_t)mine[k + std::min<size_t>(v, cnt - 1)]);
                     k += cnt;
-                    for (int x = 0; x < snx; x++) bper[npw - 1][ch].push_back({sidx, x});
+                    for (int x = 0; x < snx; x++) bper[npf - 1][ch].push_back({sidx, x});
                 }
                 a = b;
                 continue;
@@ -1020,7 +1020,7 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
 // Rebuilds and uploads a class's work lists for the current tiles.
 static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<size_t>& order, char* err,
                          size_t errlen) {
-    std::vector<std::vector<WorkItem>> single, multi, bper[5];
+    std::vector<std::vector<WorkItem>> single, multi, bper[3];
     std::vector<int32_t> usets, bsets;
     ylist(p, order, c, p->xt, p->gp, single, multi, usets, bsets, bper);
     c.nv = set_eligible(p, c, p->gp) ? p->nv : 1;
@@ -1028,7 +1028,7 @@ static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<siz
     c.vitems = chunked(multi, c.voff);
     c.usets = usets;
     c.bsets = bsets;
-    for (int k = 0; k < 5; k++) c.bl[k].items = chunked(bper[k], c.bl[k].off);
+    for (int k = 0; k < 3; k++) c.bl[k].items = chunked(bper[k], c.bl[k].off);
     auto put = [&](auto*& dptr, size_t& cap, const auto& host) -> int {
         using T = typename std::decay_t<decltype(host)>::value_type;
         const size_t n = std::max<size_t>(host.size(), 1);
@@ -1044,13 +1044,13 @@ static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<siz
     if (!rc) rc = put(c.d_vitems, c.cap_vitems, c.vitems);
     if (!rc) rc = put(c.d_usets, c.cap_usets, c.usets);
     if (!rc) rc = put(c.d_bsets, c.cap_bsets, c.bsets);
-    for (int k = 0; k < 5 && !rc; k++) rc = put(c.bl[k].d_items, c.bl[k].cap, c.bl[k].items);
+    for (int k = 0; k < 3 && !rc; k++) rc = put(c.bl[k].d_items, c.bl[k].cap, c.bl[k].items);
     return rc;
 }
 
 static hipError_t launch_yclass(const mrs_plan* p, const mrs_plan::YClass& c, int k, hipStream_t st) {
     hipError_t e = hipSuccess;
-    for (int n = 0; n < 5 && e == hipSuccess; n++) {
+    for (int n = 0; n < 3 && e == hipSuccess; n++) {
         const auto& b = c.bl[n];
         if (b.off.size() > (size_t)k + 1 && b.off[k + 1] > b.off[k])
             e = launch_ystage_basis(c.Q, c.cplx != 0, n + 1, (int)(b.off[k + 1] - b.off[k]), p->d_units, p->d_slices,

@@ -20,9 +20,10 @@
 //   B_im[(y, re), l] = B_im,  B_im[(y, im), l] =  B_re   ->  Im G
 // and the B_im form is the B_re form with its halves swapped and the low half's sign flipped.
 //
-// Work decomposition.  Block = 4 waves = one (set, column x).  A set holds up to 4 * NPW units of one
-// slice (kBasisMax at most).  Wave w takes line tiles 2 * (w & 1) and 2 * (w & 1) + 1 (64 acquired
-// lines at most) and the pairs NPW * (w >> 1) .. NPW * (w >> 1) + NPW - 1.  Per 16-y chunk:
+// Work decomposition.  Block = 4 waves = one (set, column x).  A set holds up to kBasisMax units of
+// one slice, in pairs (2p, 2p + 1); wave w takes the pairs w, w + 4, w + 8 (NPF of them) and all four
+// 16-line tiles (64 acquired lines at most), so each A is formed once and used by every line tile.
+// Per 16-y chunk:
 //   1. barrier; B (generated in registers after the previous chunk's contraction) is written to LDS
 //      in the MFMA B-operand layout (both forms, hi and lo), with the units' images of the chunk
 //      (loaded into registers one chunk ahead);
@@ -36,9 +37,9 @@
 
 namespace mrs {
 
-template <int Q, bool CPLX, int NPW>
+template <int Q, bool CPLX, int NPF>
 struct BasisSmem {
-    static constexpr int NU = 4 * NPW;        // units of a set
+    static constexpr int NU = 8 * NPF;        // units of a set (pairs wave + 4 i of 4 waves)
     static constexpr int CM = CPLX ? 2 : 1;   // floats per image value
     static constexpr int IMG_DW = NU * Q * 16 * CM;   // image dwords of one chunk
     u32x4 b[Q][4][2][2][64];                  // [q][line tile][form re/im][hi/lo][lane]
@@ -49,12 +50,12 @@ struct BasisSmem {
     float nri[Q][kBasisMaxY];
 };
 
-template <int Q, bool CPLX, int NPW>
+template <int Q, bool CPLX, int NPF>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
                   const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work,
                   const int32_t* __restrict__ bsets) {
-    using L = BasisSmem<Q, CPLX, NPW>;
+    using L = BasisSmem<Q, CPLX, NPF>;
     constexpr int NU = L::NU, CM = L::CM;
     constexpr int NSLOT = (L::IMG_DW + 255) / 256;   // image values per thread and chunk
     __shared__ __attribute__((aligned(16))) L sm;
@@ -90,8 +91,8 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
     }
 
     // Image values: thread slot k holds dword d = 256 k + tid of a chunk in the LDS order
-    // (unit, q, y, part); units past the set's count read its last unit (their rows are never
-    // stored).  Loaded into registers one chunk ahead, written to LDS after the chunk's first barrier.
+    // (unit, q, y, part).  Loaded into registers one chunk ahead, written to LDS after the chunk's
+    // first barrier.
     const GAS float* isrc[NSLOT];
     int iy[NSLOT];
 #pragma unroll
@@ -99,10 +100,11 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
         const int d = 256 * k + tid;
         isrc[k] = nullptr;
         iy[k] = 0;
-        if (d < L::IMG_DW) {
-            const int uu = d / (Q * 16 * CM), r = d % (Q * 16 * CM);
+        const int uu = d / (Q * 16 * CM);
+        if (d < L::IMG_DW && uu < nu) {   // slots past the set's units stay unset (rows never stored)
+            const int r = d % (Q * 16 * CM);
             const int q = r / (16 * CM), yy = (r / CM) & 15, pt = r % CM;
-            const float* img = (const float*)units[set[1 + min(uu, nu - 1)]].images;
+            const float* img = (const float*)units[set[1 + uu]].images;
             isrc[k] = gld(img) + ((size_t)q * P + x) * CM + pt;
             iy[k] = yy;
         }
@@ -161,12 +163,11 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
         }
     };
 
-    const int h = wave & 1, ps = wave >> 1;
-    f32x4 acc[NPW][2][2];
+    f32x4 acc[NPF][4][2];
 #pragma unroll
-    for (int i = 0; i < NPW; i++)
+    for (int i = 0; i < NPF; i++)
 #pragma unroll
-        for (int t = 0; t < 2; t++) acc[i][t][0] = acc[i][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int t = 0; t < 4; t++) acc[i][t][0] = acc[i][t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // prologue: the column's static planes into LDS (once per block), chunk 0's operands
     {
@@ -220,29 +221,21 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
         }
 #pragma unroll
         for (int k = 0; k < NSLOT; k++)
-            if (256 * k + tid < L::IMG_DW) sm.img[256 * k + tid] = icur[k];
+            if (isrc[k]) sm.img[256 * k + tid] = icur[k];
         __syncthreads();
 
-        // contraction of this wave's pairs against its line tiles (a wave whose tiles lie past the
-        // acquired lines multiplies unused data; its rows are not stored)
+        // contraction: per compartment, A of each of the wave's pairs (pair wave + 4 i, units 2 pair
+        // and 2 pair + 1) once, then every line tile against it (tiles past the acquired lines hold
+        // zero B; their rows are not stored)
         const float sr[4] = {s01.x, s01.z, s23.x, s23.z};
         const float si[4] = {s01.y, s01.w, s23.y, s23.w};
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            u32x4 bh[2][2], bl[2][2];
+            u32x4 ah[NPF], al[NPF];
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-#pragma unroll
-                for (int f = 0; f < 2; f++) {
-                    bh[t][f] = sm.b[q][2 * h + t][f][0][lane];
-                    bl[t][f] = sm.b[q][2 * h + t][f][1][lane];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < NPW; i++) {
-                const int ul = 2 * (ps * NPW + i) + (ln >> 3);
-                const float* mp = sm.img + ((ul * Q + q) * 16 + 4 * lq) * CM;
-                u32x4 ah, al;
+            for (int i = 0; i < NPF; i++) {
+                const int ul = 2 * (wave + 4 * i) + (ln >> 3);
+                const float* mp = sm.img + ((min(ul, NU - 1) * Q + q) * 16 + 4 * lq) * CM;
                 if constexpr (CPLX) {
                     const f32x4 m01 = *reinterpret_cast<const f32x4*>(mp);
                     const f32x4 m23 = *reinterpret_cast<const f32x4*>(mp + 4);
@@ -254,8 +247,8 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
                         const float aim = __builtin_fmaf(sr[j], mi[j], si[j] * mr[j]);
                         uint32_t hh, ll;
                         split2(ar, aim, hh, ll);
-                        ah[j] = hh;
-                        al[j] = ll;
+                        ah[i][j] = hh;
+                        al[i][j] = ll;
                     }
                 } else {
                     const f32x4 m = *reinterpret_cast<const f32x4*>(mp);
@@ -263,15 +256,45 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
                     for (int j = 0; j < 4; j++) {
                         uint32_t hh, ll;
                         split2(sr[j] * m[j], si[j] * m[j], hh, ll);
-                        ah[j] = hh;
-                        al[j] = ll;
+                        ah[i][j] = hh;
+                        al[i][j] = ll;
+                    }
+                }
+            }
+            // line tile t's B is read while tile t - 1 is multiplied; the three split products run in
+            // phases over all 2 NPF accumulators of the tile, so dependent MFMAs are 2 NPF apart
+            u32x4 bh[2], bl[2], nbh[2], nbl[2];
+#pragma unroll
+            for (int f = 0; f < 2; f++) {
+                nbh[f] = sm.b[q][0][f][0][lane];
+                nbl[f] = sm.b[q][0][f][1][lane];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    bh[f] = nbh[f];
+                    bl[f] = nbl[f];
+                }
+                if (t + 1 < 4) {
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        nbh[f] = sm.b[q][t + 1][f][0][lane];
+                        nbl[f] = sm.b[q][t + 1][f][1][lane];
                     }
                 }
 #pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    acc[i][t][0] = mfma3(ah, al, bh[t][0], bl[t][0], acc[i][t][0]);
-                    acc[i][t][1] = mfma3(ah, al, bh[t][1], bl[t][1], acc[i][t][1]);
-                }
+                for (int i = 0; i < NPF; i++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(al[i], bh[f], acc[i][t][f]);
+#pragma unroll
+                for (int i = 0; i < NPF; i++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bl[f], acc[i][t][f]);
+#pragma unroll
+                for (int i = 0; i < NPF; i++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bh[f], acc[i][t][f]);
             }
         }
         // B of the next chunk, while other waves still contract this one
@@ -281,17 +304,16 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
     // G_u[x][line][coil]: lane holds rows 4 lq .. 4 lq + 3 of its tiles: unit 2 * pair + (lq >> 1) of
     // the set, coils 4 (lq & 1) .. +3, line 16 * tile + ln
 #pragma unroll
-    for (int i = 0; i < NPW; i++) {
-        const int ul = 2 * (ps * NPW + i) + (lq >> 1);
+    for (int i = 0; i < NPF; i++) {
+        const int ul = 2 * (wave + 4 * i) + (lq >> 1);
         if (ul >= nu) continue;
         float2* gb = units[set[1 + ul]].gbuf;
         const int c0 = 4 * (lq & 1);
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int lt = 2 * h + t;
-            if (lt >= nt) continue;
+        for (int t = 0; t < 4; t++) {
+            if (t >= nt) continue;
             const f32x4 re = acc[i][t][0], im = acc[i][t][1];
-            GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gb) + ((size_t)x * LyP + lt * 16 + ln) * C);
+            GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gb) + ((size_t)x * LyP + t * 16 + ln) * C);
             if (C == 8) {
                 // 16-byte aligned: (x*LyP + line)*8 + c0 is a multiple of 4 complex values
                 GAS f32x4* d4 = reinterpret_cast<GAS f32x4*>(row + c0);
@@ -308,21 +330,19 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
 
 // ----------------------------------------------------------------------------- launch
 template <int Q, bool CPLX>
-static hipError_t blaunch_npw(int npw, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+static hipError_t blaunch_npw(int npf, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
                               const WorkItem* w, const int32_t* sets, hipStream_t st) {
     dim3 grid(nwork), block(256);
-    switch (npw) {
+    switch (npf) {
     case 1: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 1>), grid, block, 0, st, u, s, g, w, sets); break;
     case 2: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 2>), grid, block, 0, st, u, s, g, w, sets); break;
     case 3: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 3>), grid, block, 0, st, u, s, g, w, sets); break;
-    case 4: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 4>), grid, block, 0, st, u, s, g, w, sets); break;
-    case 5: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 5>), grid, block, 0, st, u, s, g, w, sets); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-// npw = pairs of units per wave (a set holds up to 4 * npw units); Q = 1..4 compartments.
+// npf = unit pairs per wave (a set holds up to 8 * npf units); Q = 1..4 compartments.
 hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
                                const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
