@@ -206,6 +206,20 @@ struct GroupHost {
     int32_t* d_zero = nullptr;
 };
 
+// Device copies of per-call descriptors in a ring of slots.  A call fills the next slot from pinned
+// host staging with an asynchronous copy on its own stream, launches the kernels that read it, and
+// records the slot's event after them; a slot is refilled only once its event has completed, so no
+// launch still in flight on any stream reads descriptors that a later call overwrote.
+struct DescRing {
+    static constexpr int N = 4;
+    void* dev[N] = {};
+    void* host[N] = {};
+    hipEvent_t ev[N] = {};
+    bool armed[N] = {};
+    size_t bytes = 0;
+    int cur = -1;
+};
+
 struct mrs_plan {
     int precision = MRS_PREC_SPLIT_BF16;
     std::vector<GroupHost> groups;
@@ -278,16 +292,17 @@ struct mrs_plan {
     hipStream_t graph_stream = nullptr;
     // adjoint scratch
     std::vector<VjpUnit> vjp_units;
-    VjpUnit* d_vjp_units = nullptr;
     bool vjp_fresh = true;
     // Adjoint launch order: units of one launch shape run in one launch, and within it the units that
     // read one coil-plane set (the echoes of a 3D partition) sit side by side, so their blocks for a
-    // column x meet on one XCD and share the coil planes in L2.  d_units_vjp and d_vjp_units hold the
-    // units' descriptors in this order; vjp_runs are its launches.
+    // column x meet on one XCD and share the coil planes in L2.  The adjoint's descriptor slots hold
+    // the units in this order; vjp_runs are its launches.
     std::vector<size_t> vjp_order;
     std::vector<std::pair<size_t, size_t>> vjp_runs;
-    DevUnit* d_units_vjp = nullptr;
-    std::vector<DevUnit> units_vjp;   // what d_units_vjp holds (the fields the adjoint reads)
+    // the adjoint's descriptors (units in vjp order, then their VjpUnit records) and what the current
+    // ring slot holds
+    DescRing vjp_ring;
+    std::vector<char> vjp_desc;
     // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
     std::vector<std::array<void*, 3>> s_alloc;
     std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
@@ -303,6 +318,15 @@ struct mrs_plan {
 };
 
 static void free_plan_memory(mrs_plan* p) {
+    for (int k = 0; k < DescRing::N; k++) {
+        if (p->vjp_ring.ev[k]) {
+            hipEventSynchronize(p->vjp_ring.ev[k]);
+            hipEventDestroy(p->vjp_ring.ev[k]);
+        }
+        if (p->vjp_ring.host[k]) hipHostFree(p->vjp_ring.host[k]);
+        p->vjp_ring.ev[k] = nullptr;
+        p->vjp_ring.host[k] = nullptr;
+    }
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
     if (p->graph) hipGraphDestroy(p->graph);
     for (hipStream_t s : p->streams) hipStreamDestroy(s);
@@ -718,6 +742,8 @@ extern "C" int mrs_plan_set_pointers(mrs_plan* p, const mrs_slice_desc* slices, 
         p->unit_desc[u] = units[u];
     }
     if (!changed) return MRS_OK;  // descriptors and any recorded graph stay valid
+    // launches in flight (on any stream) may still read the descriptors about to be overwritten
+    HIP_TRY(hipDeviceSynchronize());
     if (p->graph_exec) {
         hipGraphExecDestroy(p->graph_exec);
         hipGraphDestroy(p->graph);
@@ -1207,9 +1233,16 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
             int rc = plan_alloc(p, &p->vjp_units[u].gbar, snxP * dg.LyP * dg.C, err, errlen);
             if (rc) { p->vjp_units.clear(); return rc; }
         }
-        int rc = plan_alloc(p, &p->d_vjp_units, n, err, errlen);
-        if (!rc) rc = plan_alloc(p, &p->d_units_vjp, n, err, errlen);
-        if (rc) { p->vjp_units.clear(); return rc; }
+        DescRing& R = p->vjp_ring;
+        R.bytes = n * (sizeof(DevUnit) + sizeof(VjpUnit));
+        for (int k = 0; k < DescRing::N; k++) {
+            int rc = plan_alloc(p, (char**)&R.dev[k], R.bytes, err, errlen);
+            if (!rc && hipHostMalloc(&R.host[k], R.bytes, hipHostMallocDefault) != hipSuccess)
+                rc = fail(err, errlen, MRS_E_DEVICE, "pinned staging of the adjoint descriptors failed");
+            if (!rc && hipEventCreateWithFlags(&R.ev[k], hipEventDisableTiming) != hipSuccess)
+                rc = fail(err, errlen, MRS_E_DEVICE, "event creation failed");
+            if (rc) { p->vjp_units.clear(); return rc; }
+        }
         p->vjp_fresh = true;
         // launch shape, then coil-plane set, then unit
         auto shape = [&](size_t u) {
@@ -1232,38 +1265,43 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
             k = e;
         }
     }
+    // The adjoint reads a unit's images, slice and group (a new output pointer, which each forward of
+    // the autograd wrapper binds, changes nothing it reads) and the unit's cotangent and gradient
+    // pointers.  New descriptors go to the ring's next slot (asynchronous copy on `st`).
+    std::vector<char> desc(p->vjp_ring.bytes);
     {
-        // the adjoint reads a unit's images, slice and group: a new output pointer (each forward of
-        // the autograd wrapper binds one) needs no upload
-        std::vector<DevUnit> du(n);
+        DevUnit* du = reinterpret_cast<DevUnit*>(desc.data());
+        VjpUnit* vo = reinterpret_cast<VjpUnit*>(desc.data() + n * sizeof(DevUnit));
         for (size_t k = 0; k < n; k++) {
-            du[k] = p->units[p->vjp_order[k]];
+            const size_t u = p->vjp_order[k];
+            std::memset(&du[k], 0, sizeof(DevUnit));
+            du[k] = p->units[u];
             du[k].out = nullptr;
             du[k].seed = du[k].coil0 = 0;
-        }
-        if (du.size() != p->units_vjp.size() ||
-            std::memcmp(du.data(), p->units_vjp.data(), n * sizeof(DevUnit)) != 0) {  // blocking copy, as below
-            HIP_TRY(hipMemcpy(p->d_units_vjp, du.data(), n * sizeof(DevUnit), hipMemcpyHostToDevice));
-            p->units_vjp.swap(du);
+            VjpUnit& v = p->vjp_units[u];
+            v.gout = reinterpret_cast<const float2*>(grad_out[u]);
+            v.gimg = grad_images ? reinterpret_cast<float2*>(grad_images[u]) : nullptr;
+            v.gfmap = grad_fmap ? grad_fmap[u] : nullptr;
+            v.gti = grad_t_inhom ? grad_t_inhom[u] : nullptr;
+            if (!v.gout) return fail(err, errlen, MRS_E_INVALID, "unit %zu has no cotangent", u);
+            vo[k] = v;
         }
     }
-    bool changed = p->vjp_fresh;
-    for (size_t u = 0; u < n; u++) {
-        VjpUnit& v = p->vjp_units[u];
-        const VjpUnit before = v;
-        v.gout = reinterpret_cast<const float2*>(grad_out[u]);
-        v.gimg = grad_images ? reinterpret_cast<float2*>(grad_images[u]) : nullptr;
-        v.gfmap = grad_fmap ? grad_fmap[u] : nullptr;
-        v.gti = grad_t_inhom ? grad_t_inhom[u] : nullptr;
-        if (!v.gout) return fail(err, errlen, MRS_E_INVALID, "unit %zu has no cotangent", u);
-        changed |= std::memcmp(&before, &v, sizeof(VjpUnit)) != 0;
-    }
-    if (changed) {  // blocking copy: the host vector may change as soon as this call returns
-        std::vector<VjpUnit> vo(n);
-        for (size_t k = 0; k < n; k++) vo[k] = p->vjp_units[p->vjp_order[k]];
-        HIP_TRY(hipMemcpy(p->d_vjp_units, vo.data(), n * sizeof(VjpUnit), hipMemcpyHostToDevice));
+    DescRing& R = p->vjp_ring;
+    if (p->vjp_fresh || R.cur < 0 || desc != p->vjp_desc) {
+        const int k = (R.cur + 1) % DescRing::N;
+        if (R.armed[k]) HIP_TRY(hipEventSynchronize(R.ev[k]));   // launches reading slot k are done
+        std::memcpy(R.host[k], desc.data(), R.bytes);
+        HIP_TRY(hipMemcpyAsync(R.dev[k], R.host[k], R.bytes, hipMemcpyHostToDevice, st));
+        R.cur = k;
+        p->vjp_desc.swap(desc);
         p->vjp_fresh = false;
+    } else if (R.armed[R.cur]) {
+        // the current slot's copy may have been issued on another stream: order after its last use
+        HIP_TRY(hipStreamWaitEvent(st, R.ev[R.cur], 0));
     }
+    const DevUnit* d_units_vjp = reinterpret_cast<const DevUnit*>(R.dev[R.cur]);
+    const VjpUnit* d_vjp_units = reinterpret_cast<const VjpUnit*>((const char*)R.dev[R.cur] + n * sizeof(DevUnit));
     // split-BF16 plans take the y-adjoint's coil contraction on MFMA (split coil planes), FP32 plans
     // keep it in FP32 on the VALU
     const bool mfma = p->precision == MRS_PREC_SPLIT_BF16;
@@ -1275,8 +1313,10 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
         const DevGroup& dg = p->groups[p->units[p->vjp_order[run.first]].group].dev;
         HIP_TRY(launch_vjp_units((int)run.first, (int)(run.second - run.first), dg.Q, dg.C, dg.cplx != 0,
                                  dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP, dg.t_hi[1] != dg.t_hi[0] ? 2 : 1,
-                                 mfma, p->d_units_vjp, p->d_slices, p->d_groups, p->d_vjp_units, st));
+                                 mfma, d_units_vjp, p->d_slices, p->d_groups, d_vjp_units, st));
     }
+    HIP_TRY(hipEventRecord(R.ev[R.cur], st));
+    R.armed[R.cur] = true;
     return MRS_OK;
 }
 

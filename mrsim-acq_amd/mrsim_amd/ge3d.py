@@ -219,15 +219,28 @@ class Ge3dForward:
             # the views never move either: their pointer arrays are built once too
             self._vjp_buf = (gk2, per_vol, self.plan.pointer_arrays(*views))
         gk2, per_vol, ptrs = self._vjp_buf
-        # the adjoint partition transform's sets stay on the device while the cotangents do
+        # The adjoint partition transform runs from an mrs_zplan whose sets stay on the device.  The
+        # first caller's cotangent tensors get a plan of their own, reused while they are passed
+        # again; cotangents in other tensors are copied (asynchronously, on `stream`) into cotangent
+        # buffers this train owns, whose plan is built once: no allocation and no wait per call.
         sets = self._zadj_sets(grad_out, gk2)
         key = tuple(z.inp.data_ptr() for z in sets)
         zp = getattr(self, "_zadj_plan", None)
-        if zp is None or zp[0] != key:
-            if zp is not None:
-                zp[1].close()
+        if zp is None:
             self._zadj_plan = zp = (key, ZPlan(sets, adjoint=True))
-        zp[1].run(stream)
+        if zp[0] == key:
+            zp[1].run(stream)
+        else:
+            own = getattr(self, "_zadj_own", None)
+            if own is None:
+                bufs = [torch.empty_like(z.inp) for z in sets]
+                own_sets = self._zadj_sets(self._nest(bufs), gk2)
+                self._zadj_own = own = (bufs, ZPlan(own_sets, adjoint=True))
+            cur = torch.cuda.current_stream() if stream is None else stream
+            with torch.cuda.stream(cur):
+                for b, z in zip(own[0], sets):
+                    b.copy_(z.inp, non_blocking=True)
+            own[1].run(stream)
         self.plan.vjp_arrays(ptrs, stream, after_forward=after_forward)
         out = []
         for vi, v in enumerate(self.volumes):
@@ -240,6 +253,13 @@ class Ge3dForward:
             fm = b[1].sum(dim=(0, 1)).reshape(-1) if used else torch.zeros_like(v.fmap)
             out.append({"images": imgs, "fmap": fm, "t_inhom": tis})
         return out
+
+    def _nest(self, flat):
+        """A flat list in the order of the adjoint sets as grad_out[volume][echo]."""
+        nested = [[None] * len(self.tes) for _ in self.volumes]
+        for (vi, e), b in zip(self._zadj.keys(), flat):
+            nested[vi][e] = b
+        return nested
 
     def _zadj_sets(self, grad_out, gk2):
         import torch
@@ -329,10 +349,11 @@ class Ge3dForward:
         return self.plan.stage_times(iters)
 
     def close(self):
-        zp = getattr(self, "_zadj_plan", None)
-        if zp is not None:
-            zp[1].close()
-            self._zadj_plan = None
+        for attr in ("_zadj_plan", "_zadj_own"):
+            zp = getattr(self, attr, None)
+            if zp is not None:
+                zp[1].close()
+                setattr(self, attr, None)
         self.plan.close()
 
 

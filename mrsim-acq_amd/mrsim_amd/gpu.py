@@ -710,6 +710,7 @@ class Plan:
 
     def set_zstage(self, sets: Sequence["ZSet"]):
         """Attach partition transforms that run after the 2D forward (and inside its graph)."""
+        zstage_check(sets, adjoint=False)
         arr = _zset_array(sets)
         self._zkeep = list(sets)
         err = _errbuf()

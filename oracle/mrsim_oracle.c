@@ -188,12 +188,49 @@ double orc_coil_sensitivity(size_t coil, size_t n_coils, double x, double y, siz
 /* T2Slice value at voxel i */
 static double t2_at(const mrs_t2_slice* s, size_t i) { return s->map ? (double)s->map[i] : (double)s->value; }
 
-/* ---------------------------------------------------------------- XStage, default build (:1052-1140) */
-typedef struct { size_t snx, nx; C* tw[2]; } XStage;
+/* ---------------------------------------------------------------- XStage (:1052-1140)
+ * Default build: twiddle tables tw[pol][x + snx*kxi] (:1091-1101, :1126-1137).  With the FFT x-stage
+ * switched on (the `kspace` feature's form, :1077-1088, :1110-1124): the input ghost ramp
+ * e^{i2pi(+-ghost)(x - c)/snx}, an unnormalised length-snx e^{+i2pi} transform (rustfft's inverse;
+ * here an iterative radix-2 FFT when snx is a power of two, the direct sum over x otherwise), the bins
+ * (kxi - xs) mod snx, and the output centre phase e^{-i2pi m c/snx}.  The FFT's last bits differ from
+ * the twiddle sums (as rustfft's do); it is the CPU baseline's `kspace` leg. */
+static int g_fft_xstage = 0;
+void orc_set_fft_xstage(int on) { g_fft_xstage = on ? 1 : 0; }
+typedef struct {
+    size_t snx, nx;
+    C* tw[2];
+    int fft;
+    size_t xs;
+    C* ramp[2];
+    C* centre;
+    C* buf;
+    C* w;        /* e^{+i2pi k/snx}, k < snx/2 (radix-2) */
+} XStage;
+static int is_pow2(size_t n) { return n && !(n & (n - 1)); }
 static void xstage_new(XStage* xs_, size_t snx, size_t nx, size_t xs, size_t sxs, double xoff, double ghost) {
     double c = (double)sxs + xoff;
     xs_->snx = snx;
     xs_->nx = nx;
+    xs_->fft = g_fft_xstage;
+    xs_->xs = xs;
+    xs_->tw[0] = xs_->tw[1] = NULL;
+    xs_->ramp[0] = xs_->ramp[1] = NULL;
+    xs_->centre = xs_->buf = xs_->w = NULL;
+    if (xs_->fft) {
+        for (int p = 0; p < 2; p++) {
+            double gh = p == 0 ? ghost : -ghost;
+            xs_->ramp[p] = malloc(sizeof(C) * snx);
+            for (size_t x = 0; x < snx; x++) xs_->ramp[p][x] = c_cis(TAU * gh * ((double)x - c) / (double)snx);
+        }
+        xs_->centre = malloc(sizeof(C) * nx);
+        for (size_t kxi = 0; kxi < nx; kxi++)
+            xs_->centre[kxi] = c_cis(-TAU * ((double)kxi - (double)xs) * c / (double)snx);
+        xs_->buf = malloc(sizeof(C) * snx);
+        xs_->w = malloc(sizeof(C) * (snx / 2 + 1));
+        for (size_t k = 0; k <= snx / 2; k++) xs_->w[k] = c_cis(TAU * (double)k / (double)snx);
+        return;
+    }
     for (int p = 0; p < 2; p++) {
         double gh = p == 0 ? ghost : -ghost;
         C* t = malloc(sizeof(C) * snx * nx);
@@ -207,9 +244,57 @@ static void xstage_new(XStage* xs_, size_t snx, size_t nx, size_t xs, size_t sxs
         xs_->tw[p] = t;
     }
 }
-static void xstage_free(XStage* x) { free(x->tw[0]); free(x->tw[1]); }
+static void xstage_free(XStage* x) {
+    free(x->tw[0]); free(x->tw[1]);
+    if (x->fft) { free(x->ramp[0]); free(x->ramp[1]); free(x->centre); free(x->buf); free(x->w); }
+}
+/* in-place unnormalised e^{+i2pi kx/n} transform of a[0..n) (radix-2 decimation in time) */
+static void fft_inverse_pow2(C* a, size_t n, const C* w) {
+    for (size_t i = 1, j = 0; i < n; i++) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { C t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        size_t step = n / len;
+        for (size_t i = 0; i < n; i += len) {
+            for (size_t k = 0; k < len / 2; k++) {
+                C u = a[i + k], v = c_mul(a[i + k + len / 2], w[k * step]);
+                a[i + k] = c_add(u, v);
+                a[i + k + len / 2].re = u.re - v.re;
+                a[i + k + len / 2].im = u.im - v.im;
+            }
+        }
+    }
+}
 static void xstage_run(const XStage* xs_, const double* g_re, const double* g_im, double ghost_shift, C* out) {
     int pol = ghost_shift < 0.0 ? 1 : 0;
+    if (xs_->fft) {
+        const size_t n = xs_->snx;
+        C* b = xs_->buf;
+        for (size_t x = 0; x < n; x++) {
+            C g = {g_re[x], g_im[x]};
+            b[x] = c_mul(g, xs_->ramp[pol][x]);
+        }
+        if (is_pow2(n)) {
+            fft_inverse_pow2(b, n, xs_->w);
+            for (size_t kxi = 0; kxi < xs_->nx; kxi++) {
+                long m = (long)kxi - (long)xs_->xs;
+                size_t bin = (size_t)(((m % (long)n) + (long)n) % (long)n);
+                out[kxi] = c_mul(b[bin], xs_->centre[kxi]);
+            }
+        } else {
+            for (size_t kxi = 0; kxi < xs_->nx; kxi++) {
+                long m = (long)kxi - (long)xs_->xs;
+                size_t bin = (size_t)(((m % (long)n) + (long)n) % (long)n);
+                C acc = {0.0, 0.0};
+                for (size_t x = 0; x < n; x++) acc = c_add(acc, c_mul(b[x], c_cis(TAU * (double)((bin * x) % n) / (double)n)));
+                out[kxi] = c_mul(acc, xs_->centre[kxi]);
+            }
+        }
+        return;
+    }
     const C* tw = xs_->tw[pol];
     for (size_t kxi = 0; kxi < xs_->nx; kxi++) {
         const C* row = tw + xs_->snx * kxi;

@@ -81,6 +81,7 @@ def lib():
         L.orc_build_coil_kspace_nufft.argtypes = [P(SliceInput), P(Acquisition), ct.c_size_t, ct.c_size_t,
                                                   P(ct.c_int), P(C64)]
         L.orc_set_kspace_feature.argtypes = [ct.c_int]
+        L.orc_set_fft_xstage.argtypes = [ct.c_int]
         _lib = L
     return _lib
 
@@ -217,6 +218,20 @@ class kspace_feature:
 
     def __exit__(self, *exc):
         lib().orc_set_kspace_feature(0)
+        return False
+
+
+class fft_xstage:
+    """Context manager: inside it the per-slice forward runs the `kspace` feature's FFT x-stage
+    (ramp, unnormalised inverse FFT, centre phase; src/kspace.rs:1077-1088, 1110-1124) in place of
+    the twiddle tables (:1091-1101, 1126-1137)."""
+
+    def __enter__(self):
+        lib().orc_set_fft_xstage(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_fft_xstage(0)
         return False
 
 

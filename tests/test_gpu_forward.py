@@ -373,3 +373,28 @@ def test_slice_basis_matches_per_unit_and_oracle(name):
     mask = O.sampling_mask(nx, ny, w.acq).reshape(ny, nx)
     for k in basis:
         assert np.all(k[:, ~mask] == 0)
+
+
+def test_log_amp_closed_forms_on_the_gpu():
+    # src/kspace.rs:959-964: log_amp adds to the exponent of every voxel's relaxation weight.  On the
+    # GPU drop-in: log_amp = c everywhere gives e^c times the plain k-space, and log_amp = ln s gives
+    # the k-space of the compartments scaled by s (s a power of two: the scaled f32 inputs are exact).
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(1)
+    nx, ny, o = O.GOLDEN_SHAPES[1]
+    P = nx * o * ny * o
+    s = 2.0 ** np.random.default_rng(5).integers(-2, 3, size=P)
+
+    def mk(scale=None):
+        cs = [(c * scale).astype(np.float32) for c in comps] if scale is not None else list(comps)
+        return abi.SliceInputBuilder(cs, [t2m] * len(cs), fmap, (nx * o, ny * o), (nx, ny), z=3, nz=9,
+                                     t_inhom=[tim] * len(cs), phase0=phase0, slice_seed=7)
+
+    acq = abi.Acquisition.default(n_coils=3)
+    k0 = gpu.build_coil_kspace(mk(), acq, coil=1, ncoils=3)
+    kc = gpu.build_coil_kspace(mk(), acq, coil=1, ncoils=3, log_amp=np.full(P, 0.37))
+    assert rel(kc, np.exp(0.37) * k0) <= TOL
+    k_la = gpu.build_coil_kspace(mk(), acq, coil=2, ncoils=3, log_amp=np.log(s))
+    k_sc = gpu.build_coil_kspace(mk(s), acq, coil=2, ncoils=3)
+    assert rel(k_la, k_sc) <= TOL
+    # and both against the oracle's forward with the same log-amplitude map
+    assert rel(k_la, O.coil_kspace(mk(), acq, 2, 3, log_amp=np.log(s))) <= TOL

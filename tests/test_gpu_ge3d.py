@@ -226,6 +226,37 @@ def test_ge3d_vjp_after_forward_matches_the_plain_vjp(grad_setup):
                 assert (x is None and y is None) or torch.equal(x, y)
 
 
+def test_ge3d_vjp_again_with_the_same_cotangent_tensors_changed_in_place(torch_dev):
+    # The adjoint partition transform keeps a plan over the first cotangent tensors it was given; a
+    # second call with the same tensors, scaled by 2 in place, must read the new values (the
+    # gradients double), and cotangents in new tensors (copied into the train's own buffers) give
+    # the same gradients as the first call.
+    torch = torch_dev
+    w = workloads.make_ge3d("ge3d_small")
+    fw, vols = workloads.to_ge3d(w)
+    fw.forward()
+    rng = np.random.default_rng(32)
+    gd = [[torch.from_numpy((rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)).astype(np.complex64))
+           .cuda() for o in ov] for ov in fw.out]
+    first = fw.vjp(gd)
+    for gv in gd:
+        for t in gv:
+            t.mul_(2.0)
+    second = fw.vjp(gd)
+    fresh = fw.vjp([[(t * 0.5).contiguous() for t in gv] for gv in gd])
+    torch.cuda.synchronize()
+    for a, b, c in zip(first, second, fresh):
+        assert torch.allclose(b["fmap"], 2 * a["fmap"], rtol=1e-5, atol=1e-6 * float(a["fmap"].abs().max()))
+        assert torch.equal(c["fmap"], a["fmap"])
+        for key in ("images", "t_inhom"):
+            for x, y, z in zip(a[key], b[key], c[key]):
+                if x is None:
+                    continue
+                assert torch.allclose(y, 2 * x, rtol=1e-5, atol=1e-6 * float(x.abs().max()))
+                assert torch.equal(z, x)
+    fw.close()
+
+
 def _ge3d_oracle_loss(w, v, gv, images=None, fmap=None, t_inhom=None):
     loss = 0.0
     for e in range(len(w.echo_times_ms)):

@@ -77,6 +77,30 @@ def test_vjp_after_forward_reuses_or_rebuilds_the_slice_planes(setup):
     assert not torch.equal(want1[1], want2[1])
 
 
+def test_vjps_queued_on_a_side_stream_match_null_stream_results(setup):
+    # Five adjoints with alternating cotangents queued back to back on a non-blocking stream, with no
+    # synchronisation between them: each call's descriptors (cotangent and gradient pointers) go to
+    # their own slot of the plan's descriptor ring, so no earlier call's kernels read a later call's
+    # pointers.  Each result equals the same adjoint run alone on the current stream, bit for bit.
+    import torch
+    w, model, img, fmap, ti, g = setup
+    g2 = (g * (0.5 - 1.5j)).contiguous()
+    want1 = model.vjp_raw(img, fmap, ti, g)
+    torch.cuda.synchronize()
+    want2 = model.vjp_raw(img, fmap, ti, g2)
+    torch.cuda.synchronize()
+    assert not torch.equal(want1[0], want2[0])
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        got = [model.vjp_raw(img, fmap, ti, gg) for gg in (g, g2, g, g2, g)]
+    s.synchronize()
+    for k, r in enumerate(got):
+        want = want1 if k % 2 == 0 else want2
+        for a, b in zip(r, want):
+            assert torch.equal(a, b), f"queued adjoint {k} differs"
+
+
 def test_adjoint_identity(setup):
     w, model, img, fmap, ti, g = setup
     k, gi, gf, gt = _grads(model, img, fmap, ti, g)

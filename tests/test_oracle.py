@@ -281,3 +281,60 @@ def test_direct_forward_maps_complex_images_and_coil_phase():
     got = O.forward_direct(nx, ny, nx, ny, 1, 1, acq, comps[:1], fmap, maps, phase0=phase0, t2=t2m[None],
                            t_inhom=tim[None])[0]
     assert np.linalg.norm(got - want) <= 3e-7 * np.linalg.norm(want)
+
+
+def _maps_slice(shape, scale=None):
+    """A golden-shape slice with T2 and T2' maps on every compartment (the per-voxel relaxation path,
+    where the reference adds log_amp in the exponent, src/kspace.rs:959-964)."""
+    comps, fmap, phase0, t2m, tim = O.golden_inputs(shape)
+    nx, ny, o = O.GOLDEN_SHAPES[shape]
+    if scale is not None:
+        comps = [(c * scale).astype(np.float32) for c in comps]
+    return abi.SliceInputBuilder(list(comps), [t2m] * len(comps), fmap, (nx * o, ny * o), (nx, ny), z=3, nz=9,
+                                 t_inhom=[tim] * len(comps), phase0=phase0, slice_seed=7)
+
+
+@pytest.mark.parametrize("echo", [abi.ECHO_SPIN, abi.ECHO_GRADIENT])
+def test_log_amp_constant_scales_the_kspace(echo):
+    # log_amp = c everywhere: every voxel weight exp(c - trf/T2 - tau/T2') = e^c times the plain one
+    b = _maps_slice(1)
+    acq = abi.Acquisition.default(echo=echo, n_coils=2)
+    nx, ny, o = O.GOLDEN_SHAPES[1]
+    c = 0.37
+    k0 = O.coil_kspace(b, acq, 1, 2)
+    kc = O.coil_kspace(b, acq, 1, 2, log_amp=np.full(nx * o * ny * o, c))
+    assert np.abs(kc - np.exp(c) * k0).max() <= 1e-12 * np.abs(k0).max()
+
+
+def test_log_amp_map_equals_scaled_compartments():
+    # log_amp = ln s(x, y) scales every compartment of voxel (x, y) by s; s a power of two, so the scaled
+    # f32 compartments are exact and only exp's rounding separates the two
+    shape = 0
+    nx, ny, o = O.GOLDEN_SHAPES[shape]
+    P = nx * o * ny * o
+    s = 2.0 ** np.random.default_rng(4).integers(-2, 3, size=P)
+    acq = abi.Acquisition.default(n_coils=3)
+    k_la = O.coil_kspace(_maps_slice(shape), acq, 2, 3, log_amp=np.log(s))
+    k_sc = O.coil_kspace(_maps_slice(shape, scale=s), acq, 2, 3)
+    assert np.abs(k_la - k_sc).max() <= 1e-12 * np.abs(k_sc).max()
+
+
+@pytest.mark.parametrize("shape,ghost", [(0, 0.0), (1, 0.015), (2, 0.015), ("128", 0.01)])
+def test_fft_x_stage_matches_twiddle_x_stage(shape, ghost):
+    # The `kspace` feature's x-stage (ramp, unnormalised inverse FFT, centre phase; src/kspace.rs:1077-1124)
+    # is the twiddle sum of the default build (:1091-1137) reordered (the reference's own pin of the two
+    # builds: fft_inverse_matches_the_direct_dft, :3437-3451).  Power-of-two and other readout lengths.
+    if shape == "128":
+        rng = np.random.default_rng(9)
+        P = 128 * 128
+        b = abi.SliceInputBuilder([rng.uniform(0, 1, P).astype(np.float32)], [rng.uniform(40, 120, P).astype(np.float32)],
+                                  (40 * (rng.uniform(size=P) - 0.5)).astype(np.float32), (128, 128), (64, 64),
+                                  t_inhom=[rng.uniform(20, 80, P).astype(np.float32)], phase0=rng.uniform(-3, 3, P))
+    else:
+        b = _maps_slice(shape)
+    acq = abi.Acquisition.default(ghost_offset=ghost, n_coils=2)
+    k_tw = O.coil_kspace(b, acq, 0, 2)
+    with O.fft_xstage():
+        k_fft = O.coil_kspace(b, acq, 0, 2)
+    assert np.abs(k_fft - k_tw).max() <= 1e-12 * np.abs(k_tw).max()
+    assert not np.array_equal(k_fft, k_tw) or ghost == 0.0
