@@ -19,43 +19,76 @@ sys.path.insert(0, ROOT)
 
 # MI355X published peaks
 PEAK_HBM_GBS = 8000.0
-PEAK_TRANS_TS = 256 * 16 * 2.4e9 / 1e12   # quarter-rate transcendental lanes: 9.83 T/s
-PEAK_FP32_VALU_TF = 157.3
-PEAK_BF16_MFMA_TF = 2500.0
+PEAK_FP32_TF = 157.3        # FP32 (vector and dense matrix)
+PEAK_BF16_MFMA_TF = 2500.0  # dense bf16 MFMA
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(budget_s: float = 20.0):
-    """The oracle's restatement of the reference's default exact-DFT path (rotor y-stage, twiddle
-    x-stage), one process per unit on the host cores as the reference's `par` runs volumes, on a
-    bounded sample of units of the same workload shape."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _pool_rate(fn, n, cores):
+    """Units per second of fn(i) for i < n over `cores` forked worker processes."""
     import multiprocessing as mp
-
-    cores = min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    one = _cpu_unit(0)  # single unit timing on one core decides the sample size
-    t1 = time.perf_counter() - t0
-    n = max(cores, int(budget_s / max(t1, 1e-3)) // cores * cores)
-    n = min(n, 4 * cores)
     ctx = mp.get_context("fork")
-    t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
-        pool.map(_cpu_unit, range(n))
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "slices/s", "cores": cores, "kind": "port",
-            "sample": f"{n} slice units (128^2->64^2, 8 coils, 3 compartments, T2/T2' maps) of the oracle's "
-                      f"restatement of build_coil_kspace_timed x 8 coils, {cores} processes; "
-                      f"1 unit on 1 core {t1:.2f} s"}
+        pool.map(fn, range(cores))   # warm-up: every worker loads the oracle
+        t0 = time.perf_counter()
+        pool.map(fn, range(n))
+        return n / (time.perf_counter() - t0)
 
 
-def _cpu_unit(i):
+def cpu_baseline(budget_s: float = 8.0):
+    """The oracle's FP64 restatement of the reference's CPU path, timed on the host cores:
+      default_exact      config-1 units (rotor y-stage, twiddle-table x-stage: the default build)
+      kspace_fft_xstage  the same units with the `kspace` feature's FFT x-stage (its NUFFT y-stage needs
+                         uniform relaxation, which T2/T2' maps rule out: the rotor y-stage stays)
+      config0_1core / config0_all_cores  one 128^2 -> 64^2 slice, 1 coil, 1 compartment, T2/T2' maps
+    One process per unit, as the reference's `par` runs volumes, on min(nproc, 16) processes (the
+    CPU share of one GPU on the benchmark machines).  Each leg is a bounded sample of about
+    `budget_s` seconds."""
+    nproc = os.cpu_count() or 1
+    cores = min(nproc, 16)
+    legs = {}
+    for name, fn in (("default_exact", _cpu_unit), ("kspace_fft_xstage", _cpu_unit_fft)):
+        t0 = time.perf_counter()
+        fn(0)
+        t1 = time.perf_counter() - t0
+        n = max(cores, min(8 * cores, int(budget_s * cores / max(t1, 1e-3)) // cores * cores))
+        legs[name] = {"slices_per_s": round(_pool_rate(fn, n, cores), 2), "units": n,
+                      "one_unit_one_core_s": round(t1, 3)}
+    _cpu_config0(0)
+    t0 = time.perf_counter()
+    k = 0
+    while time.perf_counter() - t0 < budget_s / 4 or k < 3:
+        _cpu_config0(k)
+        k += 1
+    c0_one = k / (time.perf_counter() - t0)
+    n0 = max(cores, min(64 * cores, int(c0_one * cores * budget_s / 2) // cores * cores))
+    legs["config0_1core"] = {"slices_per_s": round(c0_one, 2), "slices": k}
+    legs["config0_all_cores"] = {"slices_per_s": round(_pool_rate(_cpu_config0, n0, cores), 2), "slices": n0}
+    d = legs["default_exact"]
+    return {"value": d["slices_per_s"], "unit": "slices/s", "cores": cores, "kind": "port",
+            "sample": f"{d['units']} slice units of the batch workload (128^2->64^2, 8 coils, 3 compartments, "
+                      f"T2/T2' maps): the oracle's FP64 restatement of build_coil_kspace_timed x 8 coils, "
+                      f"default exact-DFT build, {cores} processes",
+            "nproc": nproc, "cpu_model": _cpu_model(), "legs": legs}
+
+
+def _cpu_inputs(i):
     import numpy as np
 
     from mrsim_amd import abi
-    from oracle import oracle as O
     from mrsim_amd import workloads as W
     st = W.Stream(0xC0DE + i)
     snx = sny = 128
@@ -66,9 +99,111 @@ def _cpu_unit(i):
     fmap = (100 * (st.unit(P) - 0.5)).astype(np.float32)
     b = abi.SliceInputBuilder(comps, t2, fmap, (snx, sny), (64, 64), t_inhom=ti, phase0=st.unit(P) * 6.28)
     acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, n_coils=8)
+    return b, acq
+
+
+def _cpu_unit(i):
+    from oracle import oracle as O
+    b, acq = _cpu_inputs(i)
     for c in range(8):
         O.coil_kspace(b, acq, c, 8)
     return i
+
+
+def _cpu_unit_fft(i):
+    from oracle import oracle as O
+    b, acq = _cpu_inputs(i)
+    with O.fft_xstage():
+        for c in range(8):
+            O.coil_kspace(b, acq, c, 8)
+    return i
+
+
+_C0 = None
+
+
+def _cpu_config0(i):
+    """Config 0 on the CPU: one slice, 128^2 -> 64^2, 1 coil, 1 compartment, T2/T2' maps, complex128."""
+    global _C0
+    from mrsim_amd import abi, workloads
+    from oracle import oracle as O
+    if _C0 is None:
+        w = workloads.make("slice_128")
+        sd = w.slices[0]
+        _C0 = (abi.SliceInputBuilder([w.units[0][1][0]], [sd.t2[0]], sd.fmap, (128, 128), (64, 64),
+                                     t_inhom=[sd.t_inhom[0]], phase0=sd.phase0), w.acq)
+    O.coil_kspace(_C0[0], _C0[1], 0, 1)
+    return i
+
+
+_PF = {}
+
+
+def _oracle_pf256_unit(args):
+    """FP64 oracle k-space of one unit of the pf_256 workload (for the per-slice error of config 2)."""
+    from mrsim_amd import workloads
+    from oracle import oracle as O
+    n_slices, u = args
+    if n_slices not in _PF:
+        _PF[n_slices] = workloads.make("pf_256", n_slices=n_slices)
+    return O.forward_workload_unit(_PF[n_slices], u)
+
+
+def config2_pass(steps: int, check: bool = True):
+    """Config 2: 24 slices of 256^2 -> 128^2, 16 coils, partial Fourier 6/8 (97 of 128 lines), Hann
+    recon window.  The batched split-BF16 forward (graph replay) and the plain FP32 per-slice forward,
+    with the relative L2 error of every slice's k-space against the FP64 oracle for both; the window is
+    a reconstruction factor (src/kspace.rs:1327-1340), applied by the reconstruction, timed here too."""
+    import multiprocessing as mp
+
+    import numpy as np
+    import torch
+
+    from mrsim_amd import abi, workloads
+    from mrsim_amd import gpu as G
+    w = workloads.make("pf_256")
+    ns = len(w.units)
+    res = {"workload": "pf_256: 24 slices, 256^2->128^2, 16 coils, 1 compartment, PF 6/8 (97 lines), Hann window"}
+    outs_by = {}
+    for prec, name in ((abi.PREC_SPLIT_BF16, "split_bf16"), (abi.PREC_FP32, "plain_fp32")):
+        plan, outs, keep = workloads.to_plan(w, prec)
+        run = plan.forward_graph if prec == abi.PREC_SPLIT_BF16 else plan.forward
+        if prec == abi.PREC_SPLIT_BF16:
+            plan.tune(trials=3)
+        for _ in range(2):
+            run()
+        torch.cuda.synchronize()
+        n = max(3, steps)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            run()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / n
+        res[name] = {"ms_per_batch": round(1e3 * dt, 4), "slices_per_s": round(ns / dt, 1)}
+        outs_by[name] = [o.cpu().numpy() for o in outs]
+        plan.close()
+    rec = G.Recon(w.nx, w.ny, w.n_coils, w.acq.copy(window=abi.WINDOW_HANN))
+    plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
+    plan.forward()
+    imgs = [torch.empty((w.ny, w.nx), dtype=torch.complex64, device="cuda") for _ in outs]
+    rec.run(outs, imgs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(max(3, steps)):
+        rec.launch()
+    torch.cuda.synchronize()
+    res["recon_hann_ms_per_batch"] = round(1e3 * (time.perf_counter() - t0) / max(3, steps), 4)
+    rec.close()
+    plan.close()
+    if check:
+        cores = min(os.cpu_count() or 1, 16)
+        with mp.get_context("fork").Pool(cores) as pool:
+            want = pool.map(_oracle_pf256_unit, [(ns, u) for u in range(ns)])
+        for name in ("split_bf16", "plain_fp32"):
+            e = [float(np.linalg.norm(g - r) / np.linalg.norm(r)) for g, r in zip(outs_by[name], want)]
+            res[name]["rel_l2_vs_fp64_per_slice"] = [float(f"{x:.3e}") for x in e]
+            res[name]["rel_l2_max"] = float(f"{max(e):.3e}")
+    return res
 
 
 def nufft_pass(iters: int):
@@ -273,7 +408,7 @@ def measure_traffic(tiles: str, kernel: str, timeout: float = 300.0):
         return None
     me = [sys.executable, "-u", os.path.abspath(__file__)]
     child = me + ["--steps", "2", "--warmup", "1", "--no-cpu", "--no-plain", "--no-grad", "--no-ge3d",
-                  "--check-units", "0", "--no-traffic", "--tiles", tiles]
+                  "--no-nufft", "--no-config2", "--check-units", "0", "--no-traffic", "--tiles", tiles]
     nbytes = 256 << 20
     copy = me + ["--calibrate-copy", str(nbytes)]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
@@ -315,8 +450,9 @@ def main():
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-nufft", action="store_true")
     ap.add_argument("--no-ge3d", action="store_true")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 256^2 16-coil partial-Fourier leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
-    ap.add_argument("--traffic-tiles", type=str, default="8,4,2", help="tiles of the traffic passes")
+    ap.add_argument("--traffic-tiles", type=str, default="8,4,1", help="tiles of the traffic passes")
     ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
     ap.add_argument("--ge3d-tiles", type=str, default="", help="tiles of the 3D train (skips its autotuner)")
     ap.add_argument("--stages-only", action="store_true",
@@ -343,7 +479,7 @@ def main():
     # initialises the GPU
     traffic = None
     if not args.no_traffic and not args.only_ge3d:
-        traffic = measure_traffic(args.traffic_tiles, "ystage_mfma")
+        traffic = measure_traffic(args.traffic_tiles, "ystage_basis")
         log(f"y-stage traffic per dispatch: {traffic}")
     import numpy as np
     import torch
@@ -400,6 +536,7 @@ def main():
 
     # the plain FP32 per-slice path (one launch pair per unit) on the same batch
     plain = None
+    errs_plain = []
     if not args.no_plain:
         pplan, pouts, pkeep = workloads.to_plan(w, abi.PREC_FP32)
         pplan.forward()
@@ -412,6 +549,12 @@ def main():
         pdt = time.perf_counter() - t0
         plain = nunits * ksteps / pdt
         log(f"plain fp32 per-slice: {1e3 * pdt / ksteps:.3f} ms/step, {plain:.0f} slices/s")
+        if args.check_units > 0:
+            from oracle import oracle as O
+            for u in np.linspace(0, nunits - 1, args.check_units).astype(int):
+                want = O.forward_workload_unit(w, int(u))
+                errs_plain.append(float(np.linalg.norm(pouts[int(u)].cpu().numpy() - want) / np.linalg.norm(want)))
+            log(f"plain fp32 rel-L2 vs FP64 oracle: {errs_plain}")
         pplan.close()
 
     # reconstruction of the batch's k-spaces (window, inverse 2D DFT per coil, Roemer combine)
@@ -444,45 +587,63 @@ def main():
     if ge3d:
         log(f"3D gradient-echo train: {ge3d}")
 
+    c2 = None if args.no_config2 else config2_pass(max(3, args.steps // 4), check=args.check_units > 0)
+    if c2:
+        log(f"config 2: {c2}")
+
     stats = plan.stats()
     terms = stats["terms"]
     lines = stats["lines"]
     P = w.snx * w.sny
     Q, C = w.n_comp, w.n_coils
-    # algorithmic work of the y-stage per launch (whole batch): transcendentals of the literal sum
-    trans = lines * P * (Q + 2)
-    y_ms = stage_ms["ystage"]
-    achieved = trans / (y_ms * 1e-3) / 1e12
-    # HBM bytes of one y-stage pass over the batch (its chunk dispatches), from the counter passes
+    Ly = lines / nunits
+    # The dominant kernel is the y-stage, launched once per unit chunk; stage_ms["ystage"] times the
+    # chunks' launches back to back with HIP events on their stream, so a launch averages
+    # y_ms / nchunks over nunits / nchunks units.
     import re
     nchunks = int(re.search(r"(\d+) chunks", plan.describe()).group(1))
+    y_ms = stage_ms["ystage"]
+    t_launch = y_ms / nchunks * 1e-3
+    upl = nunits / nchunks
+    # algorithmic work per unit: F_y = 8 C P Ly real flops of the FP32-accurate complex y-contraction
+    f_y = 8.0 * C * P * Ly
+    achieved = f_y * upl / t_launch / 1e12
+    # algorithmic HBM bytes per unit: images in, G out, plus each slice's static planes and coil maps
+    alg_bytes = (Q * P * 4 + C * w.snx * Ly * 8) + len(w.slices) / nunits * P * (4 + 8 + 8 * Q + 8 * C)
     traffic_bytes = None
     if traffic:
-        traffic_bytes = round((traffic["fetch_bytes"] + traffic["write_bytes"]) * nchunks)
-    # algorithmic bytes of the y-stage pass: images, coil maps and static maps in, G out
-    alg_bytes = (nunits * Q * P * 4 + len(w.slices) * P * (4 + 8 + 8 * Q + 8 * C) +
-                 nunits * C * w.snx * (lines / nunits) * 8)
+        traffic_bytes = round(traffic["fetch_bytes"] + traffic["write_bytes"])
+    x_ms = stage_ms["xstage"]
+    x_bytes = nunits * C * Ly * (w.snx + w.nx) * 8
     roofline = {
-        "bound": "trans",
-        "achieved": round(achieved, 4),
-        "peak": round(PEAK_TRANS_TS, 3),
-        "unit": "Ttrans/s",
-        "frac": round(achieved / PEAK_TRANS_TS, 4),
+        "bound": "mfma",
+        "achieved": round(achieved, 2),
+        "peak": PEAK_FP32_TF,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TF, 4),
         "traffic": traffic_bytes,
-        "kernel": "ystage_mfma",
-        "kernel_ms": round(y_ms, 4),
-        "per_unit": f"Ly*P*(Q+2) = {int(lines / nunits)}*{P}*{Q + 2} transcendentals per slice unit, "
-                    f"{nunits} units per pass",
+        "kernel": "ystage_basis",
+        "kernel_ms_per_launch": round(y_ms / nchunks, 4),
+        "launches_per_step": nchunks,
+        "per_unit": f"F_y = 8*C*P*Ly = 8*{C}*{P}*{Ly:.0f} flops per slice unit, {upl:.0f} units per launch; "
+                    f"peak: the published FP32 rate (the contraction is FP32-accurate, run as split-BF16 "
+                    f"products on the bf16 MFMA)",
         "traffic_detail": traffic,
-        "contract_view": {
-            "hbm": {"achieved": round(alg_bytes / (y_ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(alg_bytes / (y_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
-                    "algorithmic_bytes": round(alg_bytes)},
-            "mfma": {"achieved": round(3 * 8.0 * C * P * lines / (y_ms * 1e-3) / 1e12, 2), "peak": PEAK_BF16_MFMA_TF,
-                     "unit": "TFLOP/s", "frac": round(3 * 8.0 * C * P * lines / (y_ms * 1e-3) / 1e12 / PEAK_BF16_MFMA_TF, 4),
-                     "note": "three split-BF16 products of F_y = 8*C*P*Ly per unit"},
+        "views": {
+            "hbm": {"achieved": round(alg_bytes * upl / t_launch / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(alg_bytes * upl / t_launch / 1e9 / PEAK_HBM_GBS, 4),
+                    "algorithmic_bytes_per_launch": round(alg_bytes * upl)},
+            "bf16_mfma_executed": {
+                "achieved": round(3 * f_y * Q * upl / t_launch / 1e12, 1), "peak": PEAK_BF16_MFMA_TF,
+                "unit": "TFLOP/s", "frac": round(3 * f_y * Q * upl / t_launch / 1e12 / PEAK_BF16_MFMA_TF, 4),
+                "note": "three split-BF16 products of the slice-basis contraction (K = compartments x y): 3*Q*F_y"},
+            "literal_transcendentals": {"achieved": round(Ly * P * (Q + 2) * upl / t_launch / 1e12, 3),
+                                        "unit": "T/s", "note": "Q exp + sin + cos per voxel and line; no peak claimed"},
+            "xstage": {"ms_per_step": round(x_ms, 4),
+                       "hbm_gbs": round(x_bytes / (x_ms * 1e-3) / 1e9, 1),
+                       "hbm_frac": round(x_bytes / (x_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                       "bf16_mfma_tflops": round(3 * 8.0 * C * lines * w.nx * w.snx / (x_ms * 1e-3) / 1e12, 2)},
         },
-        "xstage_mfma_tflops": round(3 * 8.0 * C * lines * w.nx * w.snx / (stage_ms["xstage"] * 1e-3) / 1e12, 2),
     }
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
@@ -507,12 +668,15 @@ def main():
         "plain_fp32_slices_per_s": None if plain is None else round(plain, 1),
         "batched_over_plain": None if plain is None else round(sps / plain, 2),
         "rel_l2_vs_fp64": errs,
+        "plain_fp32_rel_l2_vs_fp64": errs_plain,
         "stage_ms": stage_ms,
         "tiles": plan.tiles(),
+        "basis": plan.basis()[0],
         "grad_pass": grad,
         "nufft_comparator": nufft,
         "reconstruction": recon,
         "ge3d": ge3d,
+        "config2": c2,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

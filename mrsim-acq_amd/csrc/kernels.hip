@@ -850,19 +850,24 @@ __global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ u
 }
 
 // ----------------------------------------------------------------------------- plain FP32
-// y-stage: one thread per (x, padded line) of one unit; FP32 complex MACs over y for all coils.
-template <int CM>
+// y-stage: FP32 complex MACs over y for all coils on the VALU.  Block = 256 threads = YS y-parts x
+// (256 / YS) consecutive padded lines of one column x; the parts' partial sums are added in LDS in a
+// fixed order (deterministic).  grid = snx * ceil(LyP / (256 / YS)) blocks per unit: a 128^2 slice of
+// 64 lines fills 256 blocks.
+template <int CM, int YS>
 __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ units,
                                                     const DevSlice* __restrict__ slices,
                                                     const DevGroup* __restrict__ groups, int unit) {
+    constexpr int NL = 256 / YS;   // lines per block
+    __shared__ float2 part[YS][NL][CM];
     const DevUnit& u = units[unit];
     const DevGroup& gr = groups[u.group];
     const DevSlice& sl = slices[u.slice];
     const int LyP = gr.LyP, snx = gr.snx, sny = gr.sny;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lp = t % LyP, x = t / LyP;
-    if (x >= snx) return;
-    const LineRec L = gr.lines[lp];
+    const int nlb = (LyP + NL - 1) / NL;
+    const int x = blockIdx.x / nlb, l0 = (blockIdx.x % nlb) * NL;
+    const int li = threadIdx.x % NL, g = threadIdx.x / NL;
+    const int lp = l0 + li;
     const int C = gr.C, Q = gr.Q;
     const int snyP = (sny + 15) & ~15, snxP = (snx + 15) & ~15;
     const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
@@ -870,8 +875,10 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
     float2 acc[CM];
 #pragma unroll
     for (int c = 0; c < CM; c++) acc[c] = make_float2(0.f, 0.f);
-    if (L.ky >= 0) {
-        for (int y = 0; y < sny; y++) {
+    const int ylen = (sny + YS - 1) / YS, y0 = g * ylen, y1 = min(sny, y0 + ylen);
+    if (x < snx && lp < LyP && gr.lines[lp].ky >= 0) {
+        const LineRec L = gr.lines[lp];
+        for (int y = y0; y < y1; y++) {
             const size_t o = (size_t)x * snyP + y;
             const size_t i = (size_t)x + (size_t)snx * y;
             double cyc = __builtin_fma(L.kyn, (double)y - gr.sys_yoff, sl.base[o]);
@@ -902,31 +909,56 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
         }
     }
 #pragma unroll
-    for (int c = 0; c < CM; c++)
-        if (c < C) u.gbuf[((size_t)x * LyP + lp) * C + c] = acc[c];
+    for (int c = 0; c < CM; c++) part[g][li][c] = acc[c];
+    __syncthreads();
+    if (g == 0 && x < snx && lp < LyP) {
+        for (int c = 0; c < C; c++) {
+            float2 t = part[0][li][c];
+            for (int k = 1; k < YS; k++) {
+                t.x += part[k][li][c].x;
+                t.y += part[k][li][c].y;
+            }
+            u.gbuf[((size_t)x * LyP + lp) * C + c] = t;
+        }
+    }
 }
 
-// x-stage: one thread per (column n = line*C + coil, kx).
+// x-stage: FP32 on the VALU.  Block = 256 threads = 4 x-parts x 64 consecutive (column n = line*C +
+// coil, kx) outputs; the parts' partial sums are added in LDS in a fixed order.
 __global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ units,
                                                     const DevGroup* __restrict__ groups, int unit) {
+    constexpr int XS = 4, NO = 64;
+    __shared__ float2 part[XS][NO];
     const DevUnit& u = units[unit];
     const DevGroup& gr = groups[u.group];
     const int nx = gr.nx, C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int oi = threadIdx.x % NO, g = threadIdx.x / NO;
+    const int t = blockIdx.x * NO + oi;
     const int kx = t % nx, n = t / nx;
-    if (n >= LyP * C) return;
-    const int lp = n / C, c = n % C;
+    const bool live = n < LyP * C;
+    const int lp = live ? n / C : 0, c = n % C;
     const LineRec L = gr.lines[lp];
-    if (L.ky < 0) return;
-    const float2* __restrict__ tw = gr.t32[L.table] + (size_t)kx * snxP;
     float ar = 0.f, ai = 0.f;
-    for (int x = 0; x < snx; x++) {
-        const float2 g = u.gbuf[((size_t)x * LyP + lp) * C + c];
-        const float2 w = tw[x];
-        ar = __builtin_fmaf(g.x, w.x, __builtin_fmaf(-g.y, w.y, ar));
-        ai = __builtin_fmaf(g.x, w.y, __builtin_fmaf(g.y, w.x, ai));
+    const int xlen = (snx + XS - 1) / XS, x0 = g * xlen, x1 = min(snx, x0 + xlen);
+    if (live && L.ky >= 0) {
+        const float2* __restrict__ tw = gr.t32[L.table] + (size_t)kx * snxP;
+        for (int x = x0; x < x1; x++) {
+            const float2 gv = u.gbuf[((size_t)x * LyP + lp) * C + c];
+            const float2 w = tw[x];
+            ar = __builtin_fmaf(gv.x, w.x, __builtin_fmaf(-gv.y, w.y, ar));
+            ai = __builtin_fmaf(gv.x, w.y, __builtin_fmaf(gv.y, w.x, ai));
+        }
     }
-    u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = make_float2(ar, ai);
+    part[g][oi] = make_float2(ar, ai);
+    __syncthreads();
+    if (g == 0 && live && L.ky >= 0) {
+        float2 r = part[0][oi];
+        for (int k = 1; k < XS; k++) {
+            r.x += part[k][oi].x;
+            r.y += part[k][oi].y;
+        }
+        u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = r;
+    }
 }
 
 // ----------------------------------------------------------------------------- adjoint (VJP)
@@ -1997,16 +2029,16 @@ hipError_t set_xstage_lds_limit(size_t bytes) {
 
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st) {
-    const int ny_threads = LyP * snx;
-    dim3 gy((ny_threads + 255) / 256);
-    if (C <= 8) hipLaunchKernelGGL((ystage_plain<8>), gy, dim3(256), 0, st, u, s, g, unit);
-    else if (C <= 16) hipLaunchKernelGGL((ystage_plain<16>), gy, dim3(256), 0, st, u, s, g, unit);
-    else if (C <= 32) hipLaunchKernelGGL((ystage_plain<32>), gy, dim3(256), 0, st, u, s, g, unit);
-    else hipLaunchKernelGGL((ystage_plain<64>), gy, dim3(256), 0, st, u, s, g, unit);
+    // y parts per block: 8 (32 lines per block) up to 16 coils, 2 above (the LDS of the partial sums)
+    auto gy = [&](int ys) { return dim3(snx * ((LyP + 256 / ys - 1) / (256 / ys))); };
+    if (C <= 8) hipLaunchKernelGGL((ystage_plain<8, 8>), gy(8), dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 16) hipLaunchKernelGGL((ystage_plain<16, 8>), gy(8), dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 32) hipLaunchKernelGGL((ystage_plain<32, 2>), gy(2), dim3(256), 0, st, u, s, g, unit);
+    else hipLaunchKernelGGL((ystage_plain<64, 2>), gy(2), dim3(256), 0, st, u, s, g, unit);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nxt = LyP * C * nx;
-    hipLaunchKernelGGL(xstage_plain, dim3((nxt + 255) / 256), dim3(256), 0, st, u, g, unit);
+    hipLaunchKernelGGL(xstage_plain, dim3((nxt + 63) / 64), dim3(256), 0, st, u, g, unit);
     return hipGetLastError();
 }
 
