@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -1627,23 +1629,195 @@ extern "C" int mrs_probe_sincos(const double* cyc, float* s, float* c, size_t n,
 }
 
 // ------------------------------------------------------------------ drop-in per-slice forward
+// The reference calls build_coil_kspace_timed once per coil, concurrently from rayon workers
+// (src/kspace.rs:1248-1251, 2064-2068).  The drop-in keeps a per-device cache of contexts keyed by
+// everything that shapes a plan (geometry, compartments, coils, which optional inputs are present, the
+// acquisition and an explicit timing table): each context holds its plan, device buffers (with the
+// ring coil maps computed once), pinned staging and a non-blocking stream of its own.  A call takes an
+// idle matching context (or makes one), so concurrent callers run on separate contexts and streams;
+// a context serves one call at a time, which ends by waiting for its own stream only.  No allocation
+// happens in the steady state.
+
+// Replace a plan's slice and unit descriptors when the caller guarantees that no launch reading them
+// is in flight (the drop-in context: its previous call waited for its stream).
+static int plan_rebind_idle(mrs_plan* p, const mrs_slice_desc& sd, const mrs_unit_desc& ud, char* err, size_t errlen) {
+    DevSlice before = p->slices[0];
+    fill_slice_inputs(p->slices[0], sd);
+    p->slice_desc[0] = sd;
+    DevUnit& d = p->units[0];
+    const DevUnit ub = d;
+    d.images = ud.images;
+    d.out = reinterpret_cast<float2*>(ud.out);
+    d.seed = ud.slice_seed;
+    d.coil0 = ud.coil0;
+    p->unit_desc[0] = ud;
+    if (std::memcmp(&before, &p->slices[0], sizeof(DevSlice)) == 0 && std::memcmp(&ub, &d, sizeof(DevUnit)) == 0)
+        return MRS_OK;
+    p->planes_current = false;
+    return upload_descs(p, err, errlen);
+}
 
 namespace {
 
-struct DevBuf {
-    std::vector<void*> ptrs;
-    ~DevBuf() { for (void* p : ptrs) hipFree(p); }
-    template <class T>
-    int up(T** d, const T* h, size_t n, char* err, size_t errlen) {
-        if (dev_alloc(d, n, err, errlen)) return MRS_E_NOMEM;
-        ptrs.push_back((void*)*d);
-        if (h && n) {
-            hipError_t e = hipMemcpy(*d, h, n * sizeof(T), hipMemcpyHostToDevice);
-            if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
-        }
-        return MRS_OK;
+struct DropCtx {
+    std::string key;
+    int device = 0;
+    bool busy = true;
+    uint64_t last_use = 0;
+    size_t P = 0, Q = 0, nout = 0;
+    bool t2map = false, timap = false, has_p0 = false, has_la = false;
+    void* dmem = nullptr;      // one device allocation: images, fmap, phase0, log_amp, t2, t_inhom, maps, out
+    char* hin = nullptr;       // pinned staging of the inputs (same layout as the device inputs)
+    mrs_c32* hout = nullptr;   // pinned staging of the output
+    size_t in_bytes = 0;
+    float* d_img = nullptr;
+    mrs_c32* d_out = nullptr;
+    hipStream_t st = nullptr;
+    mrs_plan* plan = nullptr;
+    mrs_slice_desc sd{};
+    mrs_unit_desc ud{};
+    ~DropCtx() {
+        if (st) hipStreamSynchronize(st);
+        if (plan) mrs_plan_destroy(plan);
+        if (st) hipStreamDestroy(st);
+        if (dmem) hipFree(dmem);
+        if (hin) hipHostFree(hin);
+        if (hout) hipHostFree(hout);
     }
 };
+
+std::mutex g_drop_mu;
+std::vector<std::unique_ptr<DropCtx>> g_drop;
+uint64_t g_drop_clock = 0;
+constexpr size_t kDropMax = 32;   // cached contexts over all devices; idle ones are evicted beyond it
+
+template <class T>
+void put_key(std::string& k, const T& v) { k.append(reinterpret_cast<const char*>(&v), sizeof v); }
+
+std::string drop_key(int dev, const mrs_slice_input* inp, const mrs_acquisition* a, size_t coil0, size_t nout,
+                     size_t ntotal, const mrs_line_timing* timing, const double* log_amp, bool t2map, bool timap) {
+    std::string k;
+    put_key(k, dev);
+    put_key(k, inp->sim[0]); put_key(k, inp->sim[1]); put_key(k, inp->acq_matrix[0]); put_key(k, inp->acq_matrix[1]);
+    put_key(k, inp->n_compartments); put_key(k, coil0); put_key(k, nout); put_key(k, ntotal);
+    const uint8_t flags = (inp->phase0 ? 1 : 0) | (log_amp ? 2 : 0) | (t2map ? 4 : 0) | (timap ? 8 : 0) |
+                          (timing ? 16 : 0);
+    put_key(k, flags);
+    // the acquisition field by field (no padding bytes in the key)
+    put_key(k, a->t_line); put_key(k, a->t_echo); put_key(k, a->t_inhom); put_key(k, a->signal_scale);
+    put_key(k, a->reverse_phase); put_key(k, a->do_distortions); put_key(k, a->do_relaxation); put_key(k, a->pf_mode);
+    put_key(k, a->noise_variance); put_key(k, a->partial_fourier); put_key(k, a->ghost_offset);
+    put_key(k, a->eddy_strength); put_key(k, a->eddy_quad); put_key(k, a->eddy_phase); put_key(k, a->eddy_tau);
+    put_key(k, a->spike_amplitude); put_key(k, a->n_spikes); put_key(k, a->window); put_key(k, a->echo);
+    put_key(k, a->window_a); put_key(k, a->window_b); put_key(k, a->n_coils); put_key(k, a->accel);
+    put_key(k, a->acs_lines); put_key(k, a->seed);
+    if (timing) {
+        const size_t ny = timing->ny;
+        k.append(reinterpret_cast<const char*>(timing->t_ms), ny * sizeof(double));
+        k.append(reinterpret_cast<const char*>(timing->trf_ms), ny * sizeof(double));
+        k.append(reinterpret_cast<const char*>(timing->tread_ms), ny * sizeof(double));
+        k.append(reinterpret_cast<const char*>(timing->polarity), ny);
+    }
+    return k;
+}
+
+// A new context for the call's key: buffers, plan and stream (runs outside the cache lock).
+int drop_make(DropCtx& c, const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil0, size_t ncoils_total,
+              const mrs_line_timing* timing, char* err, size_t errlen) {
+    const size_t snx = inp->sim[0], sny = inp->sim[1], nx = inp->acq_matrix[0], ny = inp->acq_matrix[1];
+    const size_t P = c.P, Q = c.Q, nc = c.nout;
+    // device layout (and the pinned input staging, which mirrors its first in_bytes)
+    size_t off = 0;
+    auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~size_t(255); return o; };
+    const size_t o_img = take(Q * P * 4), o_fmap = take(P * 4), o_p0 = take(c.has_p0 ? P * 8 : 0),
+                 o_la = take(c.has_la ? P * 8 : 0), o_t2 = take(c.t2map ? Q * P * 4 : 0),
+                 o_ti = take(c.timap ? Q * P * 4 : 0);
+    c.in_bytes = off;
+    const size_t o_maps = take(nc * P * 8), o_out = take(nc * nx * ny * 8);
+    HIP_TRY(hipMalloc(&c.dmem, off));
+    HIP_TRY(hipHostMalloc((void**)&c.hin, c.in_bytes, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&c.hout, nc * nx * ny * sizeof(mrs_c32), hipHostMallocDefault));
+    HIP_TRY(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    char* d = (char*)c.dmem;
+    c.d_img = (float*)(d + o_img);
+    c.d_out = (mrs_c32*)(d + o_out);
+    std::vector<mrs_c32> maps(nc * P);
+    mrs_ring_coil_maps(coil0, nc, ncoils_total, snx, sny, nx, ny, 1.0, maps.data());
+    HIP_TRY(hipMemcpy(d + o_maps, maps.data(), maps.size() * sizeof(mrs_c32), hipMemcpyHostToDevice));
+    mrs_group_desc gd;
+    std::memset(&gd, 0, sizeof gd);
+    gd.snx = snx; gd.sny = sny; gd.nx = nx; gd.ny = ny;
+    gd.n_coils = nc;
+    gd.n_compartments = Q;
+    gd.complex_images = 0;
+    gd.acq = *acq;
+    gd.timing = timing;
+    std::memset(&c.sd, 0, sizeof c.sd);
+    c.sd.group = 0;
+    c.sd.fmap = (const float*)(d + o_fmap);
+    c.sd.phase0 = c.has_p0 ? (const double*)(d + o_p0) : nullptr;
+    c.sd.log_amp = c.has_la ? (const double*)(d + o_la) : nullptr;
+    c.sd.t2 = c.t2map ? (const float*)(d + o_t2) : nullptr;
+    c.sd.t_inhom = c.timap ? (const float*)(d + o_ti) : nullptr;
+    c.sd.coil_maps = (const mrs_c32*)(d + o_maps);
+    std::memset(&c.ud, 0, sizeof c.ud);
+    c.ud.slice = 0;
+    c.ud.images = c.d_img;
+    c.ud.out = c.d_out;
+    c.ud.coil0 = coil0;
+    return mrs_plan_create(&gd, 1, &c.sd, 1, &c.ud, 1, MRS_PREC_SPLIT_BF16, &c.plan, err, errlen);
+}
+
+// One call on an idle context of its key: stage, upload and run on the context's stream, wait for it.
+int drop_run(DropCtx& c, const mrs_slice_input* inp, const double* log_amp, mrs_c64* out, const size_t nxny,
+             char* err, size_t errlen) {
+    const size_t P = c.P, Q = c.Q;
+    char* h = c.hin;
+    size_t off = 0;
+    auto seg = [&](size_t bytes) { char* p = h + off; off += (bytes + 255) & ~size_t(255); return p; };
+    float* img = (float*)seg(Q * P * 4);
+    for (size_t q = 0; q < Q; q++) std::memcpy(img + q * P, inp->compartments[q], P * 4);
+    std::memcpy(seg(P * 4), inp->fmap, P * 4);
+    if (c.has_p0) std::memcpy(seg(P * 8), inp->phase0, P * 8);
+    if (c.has_la) std::memcpy(seg(P * 8), log_amp, P * 8);
+    mrs_slice_desc sd = c.sd;
+    for (size_t q = 0; q < Q; q++) {
+        sd.t2_uniform[q] = inp->t2[q].value;
+        sd.t_inhom_uniform[q] = inp->t_inhom ? inp->t_inhom[q].value : 0.0f;
+    }
+    if (c.t2map) {
+        float* t2 = (float*)seg(Q * P * 4);
+        for (size_t q = 0; q < Q; q++) {
+            if (inp->t2[q].map) std::memcpy(t2 + q * P, inp->t2[q].map, P * 4);
+            else std::fill(t2 + q * P, t2 + (q + 1) * P, inp->t2[q].value);
+        }
+    }
+    if (c.timap) {
+        float* ti = (float*)seg(Q * P * 4);
+        for (size_t q = 0; q < Q; q++) {
+            if (inp->t_inhom[q].map) std::memcpy(ti + q * P, inp->t_inhom[q].map, P * 4);
+            else std::fill(ti + q * P, ti + (q + 1) * P, inp->t_inhom[q].value);
+        }
+    }
+    sd.has_t_inhom = inp->t_inhom ? 1 : 0;
+    sd.z = inp->z;
+    sd.nz = inp->nz;
+    sd.has_eddy_drive = inp->has_eddy_drive;
+    for (int i = 0; i < 3; i++) sd.eddy_drive[i] = inp->eddy_drive[i];
+    mrs_unit_desc ud = c.ud;
+    ud.slice_seed = inp->slice_seed;
+    int rc = plan_rebind_idle(c.plan, sd, ud, err, errlen);
+    if (rc) return rc;
+    c.sd = sd;
+    c.ud = ud;
+    HIP_TRY(hipMemcpyAsync(c.dmem, c.hin, c.in_bytes, hipMemcpyHostToDevice, c.st));
+    rc = mrs_plan_forward(c.plan, c.st, err, errlen);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c.hout, c.d_out, c.nout * nxny * sizeof(mrs_c32), hipMemcpyDeviceToHost, c.st));
+    HIP_TRY(hipStreamSynchronize(c.st));
+    for (size_t i = 0; i < c.nout * nxny; i++) out[i] = mrs_c64{(double)c.hout[i].re, (double)c.hout[i].im};
+    return MRS_OK;
+}
 
 int drop_in(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil0, size_t ncoils_out,
             size_t ncoils_total, const mrs_line_timing* timing, const double* log_amp, mrs_c64* out, char* err,
@@ -1657,91 +1831,62 @@ int drop_in(const mrs_slice_input* inp, const mrs_acquisition* acq, size_t coil0
     const size_t Q = inp->n_compartments;
     if (Q < 1 || Q > MRS_MAX_COMP) return fail(err, errlen, MRS_E_UNSUPPORTED, "n_compartments must be 1..%d", MRS_MAX_COMP);
     if (!inp->compartments || !inp->t2 || !inp->fmap) return fail(err, errlen, MRS_E_INVALID, "missing slice inputs");
-    const size_t P = snx * sny;
-    DevBuf b;
-    // images: Q planes
-    std::vector<float> comp(Q * P);
-    for (size_t q = 0; q < Q; q++) std::copy(inp->compartments[q], inp->compartments[q] + P, comp.begin() + q * P);
-    float* d_img = nullptr;
-    if (b.up(&d_img, comp.data(), comp.size(), err, errlen)) return MRS_E_DEVICE;
-    float* d_fmap = nullptr;
-    if (b.up(&d_fmap, inp->fmap, P, err, errlen)) return MRS_E_DEVICE;
-    double* d_p0 = nullptr;
-    if (inp->phase0 && b.up(&d_p0, inp->phase0, P, err, errlen)) return MRS_E_DEVICE;
-    double* d_la = nullptr;
-    if (log_amp && b.up(&d_la, log_amp, P, err, errlen)) return MRS_E_DEVICE;
-    mrs_slice_desc sd;
-    std::memset(&sd, 0, sizeof sd);
     bool t2map = false, timap = false;
     for (size_t q = 0; q < Q; q++) {
         t2map |= inp->t2[q].map != nullptr;
         if (inp->t_inhom) timap |= inp->t_inhom[q].map != nullptr;
     }
-    std::vector<float> t2(t2map ? Q * P : 0), ti(timap ? Q * P : 0);
-    for (size_t q = 0; q < Q; q++) {
-        sd.t2_uniform[q] = inp->t2[q].value;
-        if (t2map) {
-            if (inp->t2[q].map) std::copy(inp->t2[q].map, inp->t2[q].map + P, t2.begin() + q * P);
-            else std::fill(t2.begin() + q * P, t2.begin() + (q + 1) * P, inp->t2[q].value);
-        }
-        if (inp->t_inhom) {
-            sd.t_inhom_uniform[q] = inp->t_inhom[q].value;
-            if (timap) {
-                if (inp->t_inhom[q].map) std::copy(inp->t_inhom[q].map, inp->t_inhom[q].map + P, ti.begin() + q * P);
-                else std::fill(ti.begin() + q * P, ti.begin() + (q + 1) * P, inp->t_inhom[q].value);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    const std::string key = drop_key(dev, inp, acq, coil0, ncoils_out, ncoils_total, timing, log_amp, t2map, timap);
+    DropCtx* c = nullptr;
+    std::unique_ptr<DropCtx> evicted;
+    {
+        std::lock_guard<std::mutex> lk(g_drop_mu);
+        for (auto& e : g_drop)
+            if (!e->busy && e->key == key) { c = e.get(); break; }
+        if (c) {
+            c->busy = true;
+        } else {
+            if (g_drop.size() >= kDropMax) {
+                size_t lru = g_drop.size();
+                for (size_t i = 0; i < g_drop.size(); i++)
+                    if (!g_drop[i]->busy && (lru == g_drop.size() || g_drop[i]->last_use < g_drop[lru]->last_use))
+                        lru = i;
+                if (lru < g_drop.size()) {
+                    evicted = std::move(g_drop[lru]);
+                    g_drop.erase(g_drop.begin() + lru);
+                }
             }
+            g_drop.push_back(std::make_unique<DropCtx>());
+            c = g_drop.back().get();
+            c->key = key;
+            c->device = dev;
         }
     }
-    float *d_t2 = nullptr, *d_ti = nullptr;
-    if (t2map && b.up(&d_t2, t2.data(), t2.size(), err, errlen)) return MRS_E_DEVICE;
-    if (timap && b.up(&d_ti, ti.data(), ti.size(), err, errlen)) return MRS_E_DEVICE;
-    std::vector<mrs_c32> maps(ncoils_out * P);
-    mrs_ring_coil_maps(coil0, ncoils_out, ncoils_total, snx, sny, nx, ny, 1.0, maps.data());
-    mrs_c32* d_maps = nullptr;
-    if (b.up(&d_maps, maps.data(), maps.size(), err, errlen)) return MRS_E_DEVICE;
-    mrs_c32* d_out = nullptr;
-    if (b.up(&d_out, (const mrs_c32*)nullptr, ncoils_out * nx * ny, err, errlen)) return MRS_E_DEVICE;
-
-    mrs_group_desc gd;
-    std::memset(&gd, 0, sizeof gd);
-    gd.snx = snx; gd.sny = sny; gd.nx = nx; gd.ny = ny;
-    gd.n_coils = ncoils_out;
-    gd.n_compartments = Q;
-    gd.complex_images = 0;
-    gd.acq = *acq;
-    gd.timing = timing;
-    sd.group = 0;
-    sd.fmap = d_fmap;
-    sd.phase0 = d_p0;
-    sd.t2 = d_t2;
-    sd.t_inhom = d_ti;
-    sd.has_t_inhom = inp->t_inhom ? 1 : 0;
-    sd.coil_maps = d_maps;
-    sd.log_amp = d_la;
-    sd.z = inp->z;
-    sd.nz = inp->nz;
-    sd.has_eddy_drive = inp->has_eddy_drive;
-    for (int i = 0; i < 3; i++) sd.eddy_drive[i] = inp->eddy_drive[i];
-    mrs_unit_desc ud;
-    std::memset(&ud, 0, sizeof ud);
-    ud.slice = 0;
-    ud.images = d_img;
-    ud.out = d_out;
-    ud.slice_seed = inp->slice_seed;
-    ud.coil0 = coil0;
-    mrs_plan* plan = nullptr;
-    int rc = mrs_plan_create(&gd, 1, &sd, 1, &ud, 1, MRS_PREC_SPLIT_BF16, &plan, err, errlen);
-    if (rc) return rc;
-    rc = mrs_plan_forward(plan, nullptr, err, errlen);
-    std::vector<mrs_c32> h(ncoils_out * nx * ny);
-    if (rc == MRS_OK) {
-        hipError_t e = hipMemcpy(h.data(), d_out, h.size() * sizeof(mrs_c32), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = fail(err, errlen, MRS_E_DEVICE, "hipMemcpy: %s", hipGetErrorString(e));
+    evicted.reset();   // waits for its stream, frees its memory (outside the lock)
+    int rc = MRS_OK;
+    if (!c->plan) {
+        c->P = snx * sny;
+        c->Q = Q;
+        c->nout = ncoils_out;
+        c->t2map = t2map;
+        c->timap = timap;
+        c->has_p0 = inp->phase0 != nullptr;
+        c->has_la = log_amp != nullptr;
+        rc = drop_make(*c, inp, acq, coil0, ncoils_total, timing, err, errlen);
     }
-    mrs_plan_destroy(plan);
-    if (rc) return rc;
-    for (size_t i = 0; i < h.size(); i++) out[i] = mrs_c64{(double)h[i].re, (double)h[i].im};
-    return MRS_OK;
+    if (rc == MRS_OK) rc = drop_run(*c, inp, log_amp, out, nx * ny, err, errlen);
+    std::unique_ptr<DropCtx> failed;
+    {
+        std::lock_guard<std::mutex> lk(g_drop_mu);
+        c->busy = false;
+        c->last_use = ++g_drop_clock;
+        if (rc != MRS_OK)   // a context whose call failed is not reused
+            for (size_t i = 0; i < g_drop.size(); i++)
+                if (g_drop[i].get() == c) { failed = std::move(g_drop[i]); g_drop.erase(g_drop.begin() + i); break; }
+    }
+    return rc;
 }
 
 }  // namespace

@@ -1,0 +1,96 @@
+"""The drop-in per-slice call from many host threads.
+
+The reference calls build_coil_kspace_timed once per coil from rayon workers
+(src/kspace.rs:1248-1251, 2064-2068), so the drop-in must be reentrant: 8 host threads calling
+mrs_build_coil_kspace (and mrs_build_coil_kspaces) concurrently, over several slices, must each get
+exactly the k-space the same call gives alone, and the per-call cost must be that of the forward
+(the contexts of the call's shape are reused: no allocation per call).
+"""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from mrsim_amd import abi, gpu, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    torch.cuda.set_device(0)
+    gpu.lib()
+
+
+def _slices(n):
+    out = []
+    for s in range(n):
+        st = workloads.Stream(0xD0D0 + s)
+        P = 64 * 64
+        comps = [st.unit(P).astype(np.float32) for _ in range(2)]
+        t2 = [(40 + 80 * st.unit(P)).astype(np.float32), 90.0]
+        ti = [(20 + 60 * st.unit(P)).astype(np.float32), 45.0]
+        fmap = (60 * (st.unit(P) - 0.5)).astype(np.float32)
+        b = abi.SliceInputBuilder(comps, t2, fmap, (64, 64), (32, 32), t_inhom=ti, phase0=st.unit(P) * 6.28,
+                                  z=s, nz=n, slice_seed=1000 + s)
+        out.append(b)
+    return out
+
+
+def test_concurrent_drop_in_calls_match_serial_calls():
+    acq = abi.Acquisition.default(n_coils=8, noise_variance=0.5, n_spikes=2, t_line=0.5, t_echo=30.0)
+    slices = _slices(3)
+    want = {(s, c): gpu.build_coil_kspace(b, acq, coil=c, ncoils=8) for s, b in enumerate(slices) for c in range(8)}
+    want_all = {s: gpu.build_coil_kspaces(b, acq) for s, b in enumerate(slices)}
+    got, got_all, errors = {}, {}, []
+
+    def worker(c):
+        try:
+            for rep in range(3):
+                for s, b in enumerate(slices):
+                    got[(s, c, rep)] = gpu.build_coil_kspace(b, acq, coil=c, ncoils=8)
+                    if c == rep:
+                        got_all[(s, c)] = gpu.build_coil_kspaces(b, acq)
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=worker, args=(c,)) for c in range(8)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    assert all(not t.is_alive() for t in threads)
+    for (s, c, rep), k in got.items():
+        assert np.array_equal(k, want[(s, c)]), f"slice {s} coil {c} repetition {rep}"
+    for (s, c), k in got_all.items():
+        assert np.array_equal(k, want_all[s]), f"all coils of slice {s} (thread {c})"
+    # the single-coil call equals the matching row of the all-coils call
+    for s in range(len(slices)):
+        for c in range(8):
+            assert np.array_equal(want[(s, c)], want_all[s][c])
+
+
+def test_drop_in_repeated_call_reuses_its_context():
+    # After the first call of a shape, calls of the same shape reuse its plan and buffers: far cheaper
+    # than the first, which builds them.
+    acq = abi.Acquisition.default(n_coils=8, t_line=0.5, t_echo=30.0)
+    w = workloads.make("batch_128", n_slices=1, n_volumes=1)
+    sd = w.slices[0]
+    b = abi.SliceInputBuilder([im for im in w.units[0][1]], [80.0, 45.0, 120.0], sd.fmap, (w.snx, w.sny),
+                              (w.nx, w.ny), phase0=sd.phase0)
+    t0 = time.perf_counter()
+    first = gpu.build_coil_kspaces(b, acq)
+    t_first = time.perf_counter() - t0
+    n = 10
+    t0 = time.perf_counter()
+    for _ in range(n):
+        again = gpu.build_coil_kspaces(b, acq)
+    t_again = (time.perf_counter() - t0) / n
+    print(f"drop-in 128^2 8 coils: first call {1e3 * t_first:.2f} ms, then {1e3 * t_again:.3f} ms per call", flush=True)
+    assert np.array_equal(first, again)
+    assert t_again < t_first
