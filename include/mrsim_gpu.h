@@ -356,6 +356,17 @@ int mrs_recon_destroy(mrs_recon* recon);
  * diagonal; solve_hermitian (:1457-1491).  Waits for `stream`. */
 int mrs_grappa_fit(const mrs_c32* ref_kspace, size_t n_coils, size_t nx, size_t ny, size_t accel, size_t acs_lines,
                    const uint8_t* lines, mrs_c64* weights, void* stream, char* err, size_t errlen);
+/* mrs_grappa_fit without allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as mrs_grappa_fit, but the fit's scratch (A^H A and
+ * A^H b of every offset, 16 * (accel-1) * (36 n_coils^2 + 6 n_coils^2) bytes, and the line flags) lives
+ * in one library-owned device array of 8 MiB per GPU, kept until the process ends; at most 1024 PE
+ * lines.  Larger problems are refused with MRS_E_UNSUPPORTED and a message.  Calls that overlap on one
+ * GPU share that array and race: serialise them (one stream, or one caller at a time).  Waits for
+ * `stream`. */
+int mrs_grappa_fit_noalloc(const mrs_c32* ref_kspace, size_t n_coils, size_t nx, size_t ny, size_t accel,
+                           size_t acs_lines, const uint8_t* lines, mrs_c64* weights, void* stream, char* err,
+                           size_t errlen);
 /* GrappaKernel::apply (:1605-1633): synthesize every missing PE line of n units' undersampled coil
  * k-spaces (device pointers, n_coils x ny x nx complex64) in place; `acquired` (host, ny flags) are
  * the lines the units read.  Waits for `stream`. */

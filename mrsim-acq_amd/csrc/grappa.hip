@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "internal.h"
@@ -242,6 +243,85 @@ done:
     hipFree(ahb);
     hipFree(dl);
     return rc;
+}
+
+// ----------------------------------------------------------------------------- GRAPPA fit without allocation
+// mrs_grappa_fit_noalloc keeps the fit's scratch (A^H A and A^H b per offset, then the reference scan's
+// line flags) in one file-scope device array per GPU, `scratch`.  Host code never uses the array's
+// name as an address: each device's copy is located once, on the first call with that device current,
+// by hipGetSymbolAddress, and that pointer is what the kernels receive.  Calls that overlap on one GPU
+// share its array and race; every value the kernels read is written earlier in the same call, so
+// nothing needs clearing between calls.
+static constexpr size_t kGrappaScratchBytes = 8u << 20;
+static constexpr int kGrappaMaxLines = 1024;
+static __device__ __align__(256) unsigned char scratch[kGrappaScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+struct LineFlags {
+    uint32_t w[kGrappaMaxLines / 32];
+};
+
+// The reference scan's line flags into the scratch (a kernel argument, so no host copy).
+__global__ void put_lines(uint8_t* __restrict__ dst, LineFlags f, int ny) {
+    for (int i = threadIdx.x; i < ny; i += blockDim.x) dst[i] = (uint8_t)((f.w[i >> 5] >> (i & 31)) & 1u);
+}
+
+}  // namespace
+
+extern "C" int mrs_grappa_fit_noalloc(const mrs_c32* ref_kspace, size_t n_coils, size_t nx, size_t ny, size_t accel,
+                                      size_t acs_lines, const uint8_t* lines, mrs_c64* weights, void* stream,
+                                      char* err, size_t errlen) {
+    if (!ref_kspace || !lines || !weights) return fail(err, errlen, MRS_E_INVALID, "null argument");
+    if (n_coils < 1 || n_coils > MRS_MAX_COILS) return fail(err, errlen, MRS_E_UNSUPPORTED, "1..%d coils", MRS_MAX_COILS);
+    if (accel < 2) return MRS_OK;   // no offsets to fit
+    if (nx < 1 || ny < 1) return fail(err, errlen, MRS_E_INVALID, "empty acquired matrix");
+    const int nc = (int)n_coils, nfeat = 6 * nc, nd = (int)accel - 1;
+    const size_t na = (size_t)nd * nfeat * nfeat, nb = (size_t)nd * nc * nfeat;
+    const size_t need = (na + nb) * sizeof(Cd) + ny;
+    if (ny > (size_t)kGrappaMaxLines || need > kGrappaScratchBytes)
+        return fail(err, errlen, MRS_E_UNSUPPORTED,
+                    "GRAPPA fit scratch of %zu bytes (%d coils, acceleration %zu, %zu lines) exceeds the no-alloc "
+                    "limit of %zu bytes and %d lines", need, nc, accel, ny, kGrappaScratchBytes, kGrappaMaxLines);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(err, errlen, MRS_E_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+    void* base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            e = hipGetSymbolAddress(&g_scratch_ptr[dev], HIP_SYMBOL(scratch));
+            if (e != hipSuccess) {
+                g_scratch_ptr[dev] = nullptr;
+                return fail(err, errlen, MRS_E_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+            }
+        }
+        base = g_scratch_ptr[dev];
+    }
+    Cd* aha = reinterpret_cast<Cd*>(base);
+    Cd* ahb = aha + na;
+    uint8_t* dl = reinterpret_cast<uint8_t*>(ahb + nb);
+    LineFlags f{};
+    for (size_t i = 0; i < ny; i++)
+        if (lines[i]) f.w[i >> 5] |= 1u << (i & 31);
+    const long ys = (long)(ny / 2), half = (long)(acs_lines / 2);
+    const int acs_lo = (int)std::max(ys - half, 0L), acs_hi = (int)std::min((size_t)(ys + half), ny - 1);
+    hipStream_t st = (hipStream_t)stream;
+    const int entries = nfeat * nfeat + nc * nfeat;
+    hipLaunchKernelGGL(put_lines, dim3(1), dim3(256), 0, st, dl, f, (int)ny);
+    hipLaunchKernelGGL(grappa_normal, dim3((entries + 255) / 256, nd), dim3(256), 0, st,
+                       reinterpret_cast<const float2*>(ref_kspace), nc, (int)nx, (int)ny, (int)accel, acs_lo, acs_hi,
+                       dl, aha, ahb);
+    hipLaunchKernelGGL(grappa_solve, dim3(nd), dim3(256), 0, st, aha, ahb, nc, reinterpret_cast<Cd*>(weights));
+    if ((e = hipGetLastError()) != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "launch: %s", hipGetErrorString(e));
+    if ((e = hipStreamSynchronize(st)) != hipSuccess)
+        return fail(err, errlen, MRS_E_DEVICE, "hipStreamSynchronize: %s", hipGetErrorString(e));
+    return MRS_OK;
 }
 
 extern "C" int mrs_grappa_apply(mrs_c32* const* kspace, size_t n, size_t n_coils, size_t nx, size_t ny, size_t accel,

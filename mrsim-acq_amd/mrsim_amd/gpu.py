@@ -94,6 +94,8 @@ def lib():
                                         P(abi.C64), P(abi.C64)] + E
     L.mrs_grappa_fit.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
                                  P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
+    L.mrs_grappa_fit_noalloc.argtypes = [ct.c_void_p, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
+                                 P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
     L.mrs_grappa_apply.argtypes = [P(ct.c_void_p), ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t, ct.c_size_t,
                                    P(ct.c_uint8), ct.c_void_p, ct.c_void_p] + E
     L.mrs_simulate_slice.argtypes = [P(abi.SliceInput), P(abi.Acquisition), P(ct.c_float)] + E
@@ -267,6 +269,21 @@ def grappa_fit(ref_kspace, accel: int, acs_lines: int, lines: np.ndarray, stream
                                 fl.ctypes.data_as(ct.POINTER(ct.c_uint8)), w.data_ptr(),
                                 ct.c_void_p(stream.cuda_stream), err, len(err)), err)
     return w
+
+
+def grappa_fit_noalloc(ref_kspace, accel: int, acs_lines: int, lines: np.ndarray, weights, stream=None):
+    """mrs_grappa_fit_noalloc: the fit into the caller's `weights` (accel-1, C, 6C) complex128 device
+    tensor, with the library's fixed device scratch (no allocation by the library)."""
+    import torch
+    C, ny, nx = ref_kspace.shape
+    _check_dev("ref_kspace", ref_kspace, torch.complex64, C * ny * nx)
+    _check_dev("weights", weights, torch.complex128, max(accel - 1, 1) * C * 6 * C)
+    fl = np.ascontiguousarray(lines, dtype=np.uint8)
+    stream = stream or torch.cuda.current_stream()
+    err = _errbuf()
+    _check(lib().mrs_grappa_fit_noalloc(ref_kspace.data_ptr(), C, nx, ny, accel, acs_lines,
+                                        fl.ctypes.data_as(ct.POINTER(ct.c_uint8)), weights.data_ptr(),
+                                        ct.c_void_p(stream.cuda_stream), err, len(err)), err)
 
 
 def grappa_apply(kspaces, accel: int, acquired: np.ndarray, weights, stream=None):
