@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "tables.h"
 
 namespace mrs {
 hipError_t launch_zstage(int mt, bool adjoint, const ZSet* sets, const ZBlock* blocks, int nblocks, hipStream_t st);
@@ -32,129 +33,7 @@ int fail(char* err, size_t errlen, int code, const char* fmt, ...) {
     return code;
 }
 
-// readout::grase_block, src/readout.rs:206-221: line ky = sy + ky_segments*j sits at acquisition
-// index a = j (reversed) or epi-1-j within its segment's block, t = (a - (epi-1)/2) t_line, and the
-// readout polarity alternates with a.
-int grase_block(size_t ny, size_t kseg, double t_line, bool reverse, std::vector<double>& t,
-                std::vector<int8_t>& pol, char* err, size_t errlen) {
-    if (kseg == 0 || ny % kseg != 0)
-        return fail(err, errlen, MRS_E_INVALID, "%zu phase-encode lines do not divide into %zu ky segments", ny, kseg);
-    const size_t epi = ny / kseg;
-    t.assign(ny, 0.0);
-    pol.assign(ny, 0);
-    for (size_t ky = 0; ky < ny; ky++) {
-        const size_t j = ky / kseg;
-        const size_t a = reverse ? j : epi - 1 - j;
-        t[ky] = ((double)a - ((double)epi - 1.0) / 2.0) * t_line;
-        pol[ky] = (a % 2 == 1) ? -1 : 1;
-    }
-    return MRS_OK;
-}
-
-// readout::kz_order, src/readout.rs:170-189: linear, or centric (the centre nz/2, then +1, -1, +2, ...).
-std::vector<size_t> kz_order(size_t nz, bool centric) {
-    std::vector<size_t> out;
-    if (!centric) {
-        for (size_t p = 0; p < nz; p++) out.push_back(p);
-        return out;
-    }
-    const size_t c = nz / 2;
-    out.push_back(c);
-    for (size_t d = 1; out.size() < nz; d++) {
-        if (c + d < nz) out.push_back(c + d);
-        if (d <= c && out.size() < nz) out.push_back(c - d);
-    }
-    return out;
-}
-
 }  // namespace
-
-extern "C" int mrs_grase_block(size_t ny, size_t ky_segments, double t_line_ms, int32_t reverse_phase, double* t_ms,
-                               int8_t* polarity, char* err, size_t errlen) {
-    std::vector<double> t;
-    std::vector<int8_t> pol;
-    int rc = grase_block(ny, ky_segments, t_line_ms, reverse_phase != 0, t, pol, err, errlen);
-    if (rc) return rc;
-    if (t_ms) std::copy(t.begin(), t.end(), t_ms);
-    if (polarity) std::copy(pol.begin(), pol.end(), polarity);
-    return MRS_OK;
-}
-
-// readout::ge3d_lines, src/readout.rs:428-462.
-extern "C" int mrs_ge3d_lines(size_t ny, size_t nz, size_t ky_segments, double t_line_ms, int32_t reverse_phase,
-                              size_t kz_segments, int32_t kz_order_kind, int32_t* shot, int32_t* excitation,
-                              size_t* n_shots, size_t* n_exc, size_t* e_c, char* err, size_t errlen) {
-    if (nz < 2)
-        return fail(err, errlen, MRS_E_INVALID, "a 3D gradient-echo train on %zu partition is a 2D readout", nz);
-    std::vector<double> t;
-    std::vector<int8_t> pol;
-    int rc = grase_block(ny, ky_segments, t_line_ms, reverse_phase != 0, t, pol, err, errlen);
-    if (rc) return rc;
-    if (kz_segments == 0 || nz % kz_segments != 0)
-        return fail(err, errlen, MRS_E_INVALID, "%zu partitions do not divide into %zu kz segments", nz, kz_segments);
-    const size_t nexc = nz / kz_segments, epi = ny / ky_segments;
-    const std::vector<size_t> order = kz_order(nz, kz_order_kind != 0);
-    std::vector<int32_t> sh(ny * nz, -1), ex(ny * nz, -1);
-    for (size_t sy = 0; sy < ky_segments; sy++)
-        for (size_t sz = 0; sz < kz_segments; sz++)
-            for (size_t j = 0; j < nexc; j++) {
-                const size_t p = order[sz + kz_segments * j];
-                for (size_t i = 0; i < epi; i++) {
-                    const size_t at = p * ny + sy + ky_segments * i;
-                    if (sh[at] >= 0) return fail(err, errlen, MRS_E_INVALID, "line read twice");
-                    sh[at] = (int32_t)(sy * kz_segments + sz);
-                    ex[at] = (int32_t)j;
-                }
-            }
-    if (shot) std::copy(sh.begin(), sh.end(), shot);
-    if (excitation) std::copy(ex.begin(), ex.end(), excitation);
-    if (n_shots) *n_shots = ky_segments * kz_segments;
-    if (n_exc) *n_exc = nexc;
-    if (e_c) *e_c = (size_t)(std::find(order.begin(), order.end(), nz / 2) - order.begin()) / kz_segments;
-    return MRS_OK;
-}
-
-// readout::check_ge3d_timing, src/readout.rs:465-501: each echo's block spans TE_e +- half, half =
-// max |t| + t_line/2.
-extern "C" int mrs_check_ge3d_timing(const double* tes, size_t n_echoes, double exc_spacing_ms,
-                                     double excitation_time_ms, size_t ny, size_t ky_segments, double t_line_ms,
-                                     int32_t reverse_phase, size_t n_exc, double t_exc_ms, double tr_ms, char* err,
-                                     size_t errlen) {
-    if (n_echoes == 0 || !tes)
-        return fail(err, errlen, MRS_E_INVALID, "a 3D gradient-echo train needs at least one echo time");
-    std::vector<double> t;
-    std::vector<int8_t> pol;
-    int rc = grase_block(ny, ky_segments, t_line_ms, reverse_phase != 0, t, pol, err, errlen);
-    if (rc) return rc;
-    double half = 0.0;
-    for (double v : t) half = std::max(half, std::fabs(v));
-    half += t_line_ms / 2.0;
-    const double pulse = excitation_time_ms / 2.0;
-    const size_t epi = ny / ky_segments;
-    if (tes[0] - half < pulse - 1e-9)
-        return fail(err, errlen, MRS_E_INVALID,
-                    "echo 1 at %.4f ms: its %zu-line block starts %.4f ms after the excitation, inside the excitation "
-                    "pulse (half of %.3f ms); a later echo time or a shorter block would fit",
-                    tes[0], epi, tes[0] - half, excitation_time_ms);
-    for (size_t e = 1; e < n_echoes; e++)
-        if (tes[e] - half < tes[e - 1] + half - 1e-9)
-            return fail(err, errlen, MRS_E_INVALID,
-                        "echo %zu at %.4f ms and echo %zu at %.4f ms: their blocks of %.4f ms each side overlap", e,
-                        tes[e - 1], e + 1, tes[e], half);
-    const double end = tes[n_echoes - 1] + half;
-    if (n_exc > 1 && end > exc_spacing_ms - pulse + 1e-9)
-        return fail(err, errlen, MRS_E_INVALID,
-                    "the last echo's block ends %.4f ms after its excitation, after the next excitation pulse starts "
-                    "at %.4f ms (excitation spacing %.4f ms, pulse %.3f ms): the spacing must be at least %.4f ms",
-                    end, exc_spacing_ms - pulse, exc_spacing_ms, excitation_time_ms, end + pulse);
-    const double last = t_exc_ms + ((double)n_exc - 1.0) * exc_spacing_ms + end;
-    if (last > tr_ms + 1e-9)
-        return fail(err, errlen, MRS_E_INVALID,
-                    "the train ends at %.4f ms (%zu excitations %.4f ms apart from %.4f ms), after the %.4f ms "
-                    "repetition",
-                    last, n_exc, exc_spacing_ms, t_exc_ms, tr_ms);
-    return MRS_OK;
-}
 
 // ---------------------------------------------------------------------------- partition transform
 

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "tables.h"
 
 namespace mrs {
 hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, int mode, hipStream_t st);
@@ -66,106 +67,6 @@ int fail(char* err, size_t errlen, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return fail(err, errlen, MRS_E_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
 
-// ------------------------------------------------------------------ protocol tables (host, FP64)
-
-// SingleShotEpi (src/readout.rs:21-69): tick -> (kx, ky) snake trajectory and its times.
-struct Epi {
-    size_t nx, ny;
-    double t_line, t_echo;
-    bool reverse;
-    double dt() const { return t_line / (double)nx; }
-    double half() const { return (double)(nx * ny) * dt() / 2.0; }
-    void index(size_t tick, size_t& kx, size_t& ky) const {
-        size_t x = tick % nx, y = tick / nx;
-        if (!reverse) {
-            y = ny - 1 - y;
-            if (y % 2 == 1) x = nx - x - 1;
-        } else if (y % 2 == 1) {
-            x = nx - x - 1;
-        }
-        kx = x;
-        ky = y;
-    }
-    double from_echo(size_t tick) const { return dt() * ((double)tick + 0.5) - half(); }
-    double from_prep(size_t tick) const { return (double)tick * dt() + dt() / 2.0; }
-    double from_rf(size_t tick) const { return t_echo + from_echo(tick); }
-};
-
-size_t pf_skipped_lines(size_t ny, const mrs_acquisition& a) {
-    if (a.partial_fourier < 1.0 && a.pf_mode == MRS_PF_SCANNER) {
-        size_t keep = std::min((size_t)std::round((double)ny * a.partial_fourier), ny);
-        return ny - keep;
-    }
-    return 0;
-}
-
-void timing_table(const mrs_acquisition& a, size_t nx, size_t ny, double* t, double* trf, double* tread, int8_t* pol) {
-    Epi e{nx, ny, a.t_line, a.t_echo, a.reverse_phase != 0};
-    std::fill(t, t + ny, 0.0);
-    std::fill(trf, trf + ny, 0.0);
-    std::fill(tread, tread + ny, 0.0);
-    const size_t centre = nx / 2;
-    for (size_t tick = 0; tick < nx * ny; tick++) {
-        size_t kx, ky;
-        e.index(tick, kx, ky);
-        if (kx != centre) continue;
-        t[ky] = e.from_echo(tick);
-        trf[ky] = e.from_rf(tick);
-        tread[ky] = e.from_prep(tick);
-    }
-    for (size_t k = 0; k < ny; k++) pol[k] = (k % 2 == 1) ? -1 : 1;
-    const double offset = (double)pf_skipped_lines(ny, a) * a.t_line;
-    if (offset > 0.0)
-        for (size_t k = 0; k < ny; k++) tread[k] -= offset;
-}
-
-// Acquired PE rows (sampling_mask rows, src/kspace.rs:546-588).
-std::vector<uint8_t> acquired_rows(size_t ny, const mrs_acquisition& a) {
-    std::vector<uint8_t> rows(ny, 0);
-    const size_t ys = ny / 2;
-    const size_t accel = std::max<uint64_t>(a.accel, 1);
-    const size_t keep_n = (size_t)std::round((double)ny * a.partial_fourier);
-    const bool pf = a.partial_fourier < 1.0;
-    for (size_t k = 0; k < ny; k++) {
-        bool skip = false;
-        if (pf && a.pf_mode == MRS_PF_SCANNER) {
-            skip = a.reverse_phase ? (k < ny - keep_n) : (k >= keep_n);
-        } else if (pf && a.pf_mode == MRS_PF_CONTIGUOUS) {
-            skip = a.reverse_phase ? (k >= keep_n) : (k < ny - keep_n);
-        } else if (pf) {
-            skip = a.reverse_phase ? ((double)k > std::ceil((double)ny * a.partial_fourier))
-                                   : (((double)k < std::floor((double)ny * (1.0 - a.partial_fourier))) &&
-                                      (k > 0 || ny % 2 == 1));
-        }
-        if (skip) continue;
-        if (accel <= 1 || k % accel == ys % accel) rows[k] = 1;
-    }
-    return rows;
-}
-
-
-uint16_t bf16_rne(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-float bf16_to_float(uint16_t h) {
-    uint32_t u = (uint32_t)h << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-// FP64 complex -> packed bf16 hi and residual lo, via FP32 (the device's operand precision).
-void split_c(double re, double im, uint32_t& hi, uint32_t& lo) {
-    float fr = (float)re, fi = (float)im;
-    uint16_t hr = bf16_rne(fr), hm = bf16_rne(fi);
-    float rr = (float)(re - (double)bf16_to_float(hr)), ri = (float)(im - (double)bf16_to_float(hm));
-    hi = (uint32_t)hr | ((uint32_t)hm << 16);
-    lo = (uint32_t)bf16_rne(rr) | ((uint32_t)bf16_rne(ri) << 16);
-}
-
 template <class T>
 int dev_alloc(T** p, size_t n, char* err, size_t errlen) {
     *p = nullptr;
@@ -178,20 +79,6 @@ int dev_alloc(T** p, size_t n, char* err, size_t errlen) {
 
 }  // namespace
 
-namespace mrs {
-// coil_sensitivity (src/kspace.rs:226-237): the real ring Gaussian plus 0.15, in acquired-voxel units.
-double ring_sensitivity(size_t coil, size_t n_coils, double x, double y, size_t nx, size_t ny) {
-    if (n_coils <= 1) return 1.0;
-    const double cx = (double)nx / 2.0, cy = (double)ny / 2.0;
-    const double m = (double)std::max(nx, ny);
-    const double r = 0.6 * m;
-    const double ang = kTau * (double)coil / (double)n_coils;
-    const double px = cx + r * std::cos(ang), py = cy + r * std::sin(ang);
-    const double dx = x - px, dy = y - py;
-    const double sigma = 0.9 * m;
-    return std::exp(-(dx * dx + dy * dy) / (2.0 * sigma * sigma)) + 0.15;
-}
-}  // namespace mrs
 
 // ------------------------------------------------------------------ plan
 
@@ -1537,59 +1424,6 @@ extern "C" int mrs_plan_get_tiles(const mrs_plan* p, int32_t* xt, int32_t* gp, i
     if (xt) *xt = p->xt;
     if (gp) *gp = p->gp;
     if (units_per_item) *units_per_item = p->nv;
-    return MRS_OK;
-}
-
-// ------------------------------------------------------------------ host tables (C ABI)
-
-extern "C" int mrs_acquisition_default(mrs_acquisition* a) {
-    if (!a) return MRS_E_INVALID;
-    std::memset(a, 0, sizeof *a);
-    a->t_line = 1.0;
-    a->t_echo = 90.0;
-    a->t_inhom = 50.0;
-    a->signal_scale = 100.0;
-    a->do_distortions = 1;
-    a->do_relaxation = 1;
-    a->partial_fourier = 1.0;
-    a->pf_mode = MRS_PF_FIBERFOX;
-    a->eddy_tau = 70.0;
-    a->spike_amplitude = 1.0;
-    a->window = MRS_WINDOW_NONE;
-    a->n_coils = 1;
-    a->accel = 1;
-    a->acs_lines = 24;
-    a->echo = MRS_ECHO_SPIN;
-    return MRS_OK;
-}
-
-extern "C" int mrs_line_timing_for_acquisition(const mrs_acquisition* acq, size_t nx, size_t ny, double* t_ms,
-                                               double* trf_ms, double* tread_ms, int8_t* polarity) {
-    if (!acq || !t_ms || !trf_ms || !tread_ms || !polarity || nx == 0) return MRS_E_INVALID;
-    timing_table(*acq, nx, ny, t_ms, trf_ms, tread_ms, polarity);
-    return MRS_OK;
-}
-
-extern "C" int mrs_sampling_mask(size_t nx, size_t ny, const mrs_acquisition* acq, uint8_t* mask) {
-    if (!acq || !mask) return MRS_E_INVALID;
-    std::vector<uint8_t> rows = acquired_rows(ny, *acq);
-    for (size_t k = 0; k < ny; k++)
-        for (size_t x = 0; x < nx; x++) mask[x + nx * k] = rows[k];
-    return MRS_OK;
-}
-
-extern "C" int mrs_ring_coil_maps(size_t coil0, size_t n, size_t ncoils, size_t snx, size_t sny, size_t nx, size_t ny,
-                                  double scale, mrs_c32* out) {
-    if (!out || nx == 0 || ny == 0 || snx % nx || sny % ny) return MRS_E_INVALID;
-    const size_t ox = snx / nx, oy = sny / ny;
-    const double xoff = ((double)ox - 1.0) / 2.0, yoff = ((double)oy - 1.0) / 2.0;
-    for (size_t c = 0; c < n; c++)
-        for (size_t y = 0; y < sny; y++)
-            for (size_t x = 0; x < snx; x++) {
-                const double xa = ((double)x - xoff) / (double)ox, ya = ((double)y - yoff) / (double)oy;
-                const double s = scale * ring_sensitivity(coil0 + c, ncoils, xa, ya, nx, ny);
-                out[c * snx * sny + x + snx * y] = mrs_c32{(float)s, 0.0f};
-            }
     return MRS_OK;
 }
 

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(ROOT, "build")
 LIB = os.path.join(OUT_DIR, "libmrsim_gpu.so")
-SOURCES = ["kernels.hip", "ybasis.hip", "plan.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip", "nufft.hip"]
+SOURCES = ["kernels.hip", "ybasis.hip", "plan.cpp", "tables.cpp", "ge3d.cpp", "recon.cpp", "grappa.hip", "nufft.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
