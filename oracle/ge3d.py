@@ -167,6 +167,8 @@ def compartment_kspace(snx, sny, nx, ny, nz, image, t2, t_inhom, fmap, phase0, c
     timing = within_echo_timing(ny, ky_segments, t_line_ms, acq.reverse_phase, te)
     k2 = np.zeros((nz, C, ny, nx), dtype=np.complex128)
     for z in range(nz):
+        if not np.any(image[z]):
+            continue    # the 2D forward is linear in the image: a zero partition contributes zero
         t2z = None if np.isscalar(t2) else np.asarray(t2[z], dtype=np.float32)
         tiz = None if np.isscalar(ti) else np.asarray(ti[z], dtype=np.float32)
         k2[z] = O.forward_direct(snx, sny, nx, ny, C, 1, a, np.asarray(image[z], dtype=np.float32).reshape(P),

@@ -351,6 +351,75 @@ def test_ge3d_full_size_partition_round_trip(full):
             assert r <= 1e-5
 
 
+def test_ge3d_full_size_vjp_matches_fp64_central_differences(full):
+    # Both full-size volumes (128^2 -> 64^2 and 192^2 -> 96^2, 24 partitions, 32 coils, 4 echoes): a
+    # seeded cotangent on two coils of every echo, and the fieldmap, T2' and image gradients at the
+    # brightest voxel of two partitions against FP64 central differences of the oracle's train.  The
+    # train is a sum over voxels of image[v] times a function of voxel v's own fieldmap and T2', so
+    # each derivative is that of the oracle run on an image holding voxel v alone (the oracle skips
+    # the partitions whose image is zero, which keeps this to a few seconds of CPU).
+    import torch
+    w, fw = full
+    rng = np.random.default_rng(41)
+    coils = [3, 20]
+    g, gd = [], []
+    for ov in fw.out:
+        gv, gdv = [], []
+        for o in ov:
+            x = np.zeros(o.shape, dtype=np.complex64)
+            sub = (rng.standard_normal((o.shape[0], len(coils)) + tuple(o.shape[2:])) +
+                   1j * rng.standard_normal((o.shape[0], len(coils)) + tuple(o.shape[2:]))).astype(np.complex64)
+            x[:, coils] = sub
+            gv.append(sub)
+            gdv.append(torch.from_numpy(x).cuda())
+        g.append(gv)
+        gd.append(gdv)
+    grads = fw.vjp(gd)
+    torch.cuda.synchronize()
+
+    def loss(v, gv, image, fmap=None, t_inhom=None):
+        total = 0.0
+        for e in range(len(w.echo_times_ms)):
+            k = G.ge3d_kspace(v.snx, v.sny, v.nx, v.ny, v.nz, [image], v.t2, v.t_inhom if t_inhom is None else
+                              [t_inhom], v.fmap if fmap is None else fmap, None, v.coil_maps[:, coils], w.acq,
+                              w.echo_times_ms[e], w.ky_segments, w.t_line_ms, v.weights.astype(np.float32))
+            total += float(np.real(np.vdot(gv[e], k)))
+        return total
+
+    for vi, v in enumerate(w.volumes):
+        P = v.snx * v.sny
+        gf = grads[vi]["fmap"].cpu().numpy()
+        gt = grads[vi]["t_inhom"][0].cpu().numpy()
+        gi = grads[vi]["images"][0].cpu().numpy()
+        for z in (5, 17):
+            p = int(np.argmax(np.abs(v.images[0][z])))
+            i = z * P + p
+            one = np.zeros_like(v.images[0])
+            one[z, p] = v.images[0][z, p]
+            h = 1e-3 * max(abs(float(v.fmap[z, p])), 1.0)
+            fp, fm = v.fmap.copy(), v.fmap.copy()
+            fp[z, p] += h
+            fm[z, p] -= h
+            d = (loss(v, g[vi], one, fmap=fp) - loss(v, g[vi], one, fmap=fm)) / (float(fp[z, p]) - float(fm[z, p]))
+            print(f"volume {vi} ({v.snx}->{v.nx}) partition {z} voxel {p}: dL/dfmap gpu {gf[i]:.6e} fp64 {d:.6e}",
+                  flush=True)
+            assert abs(gf[i] - d) <= 2e-3 * max(abs(d), 1e-3 * np.abs(gf).max())
+            ti0 = float(v.t_inhom[0])
+            tp = np.full((v.nz, P), ti0, dtype=np.float32)
+            tm = tp.copy()
+            tp[z, p] += 1e-3 * ti0
+            tm[z, p] -= 1e-3 * ti0
+            d = (loss(v, g[vi], one, t_inhom=tp) - loss(v, g[vi], one, t_inhom=tm)) / (float(tp[z, p]) - float(tm[z, p]))
+            print(f"  dL/dT2' gpu {gt[i]:.6e} fp64 {d:.6e}", flush=True)
+            assert abs(gt[i] - d) <= 2e-3 * max(abs(d), 1e-3 * np.abs(gt).max())
+            # the image: the loss is linear in it, so the voxel at 1 and every other at 0 is the derivative
+            unit = np.zeros_like(v.images[0])
+            unit[z, p] = 1.0
+            d = loss(v, g[vi], unit)
+            print(f"  dL/dimage gpu {gi[i]:.6e} fp64 {d:.6e}", flush=True)
+            assert abs(gi[i] - d) <= 2e-4 * max(abs(d), 1e-3 * np.abs(gi).max())
+
+
 def _oracle_pipeline(w, v, e, acq, seed, g, k3=None, kernels=None):
     """The oracle's acquired k-space -> partitions_to_slices -> reconstruct_slices: (nz, ny*nx)."""
     if k3 is None:
