@@ -396,11 +396,11 @@ def _pmc_per_dispatch(outdir, counter, match):
     return (tot / len(ids), len(ids)) if ids else (None, 0)
 
 
-def measure_traffic(tiles: str, kernel: str, timeout: float = 300.0):
-    """HBM bytes per dispatch of `kernel` in the batch forward: rocprofv3 --pmc FETCH_SIZE, then
-    WRITE_SIZE, each run on its own in a child process (before this process touches the GPU), each
-    scaled by what the same counter reads for a device copy of 256 MiB.  None when rocprofv3 is
-    missing or a pass fails."""
+def measure_traffic(tiles: str, basis: bool, kernel: str, timeout: float = 300.0):
+    """HBM bytes per dispatch of `kernel` in the batch forward run at `tiles` and `basis` (the
+    configuration the timed run used): rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, each run on its
+    own in a child process, each scaled by what the same counter reads for a device copy of 256 MiB.
+    None when rocprofv3 is missing or a pass fails."""
     import shutil
     import subprocess
     import tempfile
@@ -408,7 +408,8 @@ def measure_traffic(tiles: str, kernel: str, timeout: float = 300.0):
         return None
     me = [sys.executable, "-u", os.path.abspath(__file__)]
     child = me + ["--steps", "2", "--warmup", "1", "--no-cpu", "--no-plain", "--no-grad", "--no-ge3d",
-                  "--no-nufft", "--no-config2", "--check-units", "0", "--no-traffic", "--tiles", tiles]
+                  "--no-nufft", "--no-config2", "--check-units", "0", "--no-traffic", "--tiles", tiles,
+                  "--basis", "1" if basis else "0"]
     nbytes = 256 << 20
     copy = me + ["--calibrate-copy", str(nbytes)]
     env = dict(os.environ, TMPDIR=os.environ.get("TMPDIR", "/tmp"))
@@ -452,7 +453,8 @@ def main():
     ap.add_argument("--no-ge3d", action="store_true")
     ap.add_argument("--no-config2", action="store_true", help="skip the 256^2 16-coil partial-Fourier leg")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
-    ap.add_argument("--traffic-tiles", type=str, default="8,4,1", help="tiles of the traffic passes")
+    ap.add_argument("--basis", type=int, choices=(0, 1), default=None,
+                    help="with --tiles: run qualifying slices in the slice-basis y-stage (1) or per unit (0)")
     ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
     ap.add_argument("--ge3d-tiles", type=str, default="", help="tiles of the 3D train (skips its autotuner)")
     ap.add_argument("--stages-only", action="store_true",
@@ -475,12 +477,6 @@ def main():
         print("bench.py measures one GPU: the forward has no cross-GPU step, and multi-GPU runs are not "
               "part of this build (each GPU would take its own share of units).", file=sys.stderr)
         sys.exit(2)
-    # HBM traffic of the y-stage from counter passes in child processes, before this process
-    # initialises the GPU
-    traffic = None
-    if not args.no_traffic and not args.only_ge3d:
-        traffic = measure_traffic(args.traffic_tiles, "ystage_basis")
-        log(f"y-stage traffic per dispatch: {traffic}")
     import numpy as np
     import torch
 
@@ -496,6 +492,8 @@ def main():
     plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
     st = torch.cuda.current_stream()
     if args.tiles:
+        if args.basis is not None:
+            plan.set_basis(bool(args.basis))
         plan.set_tiles(*(int(v) for v in args.tiles.split(",")))
     else:
         log(plan.tune(trials=5).strip())
@@ -522,6 +520,14 @@ def main():
     # per-stage kernel durations with HIP events on the launch stream
     stage_ms = plan.stage_times(iters=max(3, args.steps // 2))
     log(f"stage times (ms): {stage_ms}")
+    # HBM traffic of the dominant kernel (the y-stage of the configuration just timed) from counter
+    # passes in child processes, while this process waits
+    basis_on = plan.basis()[0]
+    y_kernel = "ystage_basis" if basis_on else "ystage_mfma"
+    traffic = None
+    if not args.no_traffic:
+        traffic = measure_traffic(",".join(str(v) for v in plan.tiles()), basis_on, y_kernel)
+        log(f"y-stage traffic per dispatch: {traffic}")
 
     # parity of sampled units against the FP64 oracle
     errs = []
@@ -613,6 +619,7 @@ def main():
     traffic_bytes = None
     if traffic:
         traffic_bytes = round(traffic["fetch_bytes"] + traffic["write_bytes"])
+    kq = Q if basis_on else 1   # MFMA K per y: the compartments in the slice basis, E per unit
     x_ms = stage_ms["xstage"]
     x_bytes = nunits * C * Ly * (w.snx + w.nx) * 8
     roofline = {
@@ -622,7 +629,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TF, 4),
         "traffic": traffic_bytes,
-        "kernel": "ystage_basis",
+        "kernel": y_kernel,
         "kernel_ms_per_launch": round(y_ms / nchunks, 4),
         "launches_per_step": nchunks,
         "per_unit": f"F_y = 8*C*P*Ly = 8*{C}*{P}*{Ly:.0f} flops per slice unit, {upl:.0f} units per launch; "
@@ -634,9 +641,12 @@ def main():
                     "frac": round(alg_bytes * upl / t_launch / 1e9 / PEAK_HBM_GBS, 4),
                     "algorithmic_bytes_per_launch": round(alg_bytes * upl)},
             "bf16_mfma_executed": {
-                "achieved": round(3 * f_y * Q * upl / t_launch / 1e12, 1), "peak": PEAK_BF16_MFMA_TF,
-                "unit": "TFLOP/s", "frac": round(3 * f_y * Q * upl / t_launch / 1e12 / PEAK_BF16_MFMA_TF, 4),
-                "note": "three split-BF16 products of the slice-basis contraction (K = compartments x y): 3*Q*F_y"},
+                "achieved": round(3 * f_y * kq * upl / t_launch / 1e12, 1), "peak": PEAK_BF16_MFMA_TF,
+                "unit": "TFLOP/s", "frac": round(3 * f_y * kq * upl / t_launch / 1e12 / PEAK_BF16_MFMA_TF, 4),
+                "note": ("three split-BF16 products of the slice-basis contraction (K = compartments x y): 3*Q*F_y"
+                         if basis_on else
+                         "three split-BF16 products of the per-unit contraction (K = y, compartments summed "
+                         "into E on the VALU): 3*F_y")},
             "literal_transcendentals": {"achieved": round(Ly * P * (Q + 2) * upl / t_launch / 1e12, 3),
                                         "unit": "T/s", "note": "Q exp + sin + cos per voxel and line; no peak claimed"},
             "xstage": {"ms_per_step": round(x_ms, 4),

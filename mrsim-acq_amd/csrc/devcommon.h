@@ -9,6 +9,7 @@ namespace mrs {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+typedef __attribute__((ext_vector_type(2))) uint32_t u32x2;
 typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
 typedef __attribute__((ext_vector_type(2))) float f32x2;
 typedef __attribute__((ext_vector_type(2))) double f64x2;
@@ -46,6 +47,13 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
     const float ha = __builtin_bit_cast(float, h << 16), hb = __builtin_bit_cast(float, h & 0xffff0000u);
     hi = h;
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a - ha, b - hb}, bf16x2));
+}
+
+// One complex value of the split-precision G buffer: {hi, lo} of split2(re, im).
+__device__ __forceinline__ u32x2 split_g(float re, float im) {
+    uint32_t h, l;
+    split2(re, im, h, l);
+    return u32x2{h, l};
 }
 
 // A-operand rows of a complex contraction out = sum S * v, split into (re row, im row):

@@ -95,7 +95,10 @@ struct DevGroup {
 struct DevUnit {
     const void* images;   // Q planes of P float or complex64 (user layout x + snx*y)
     float2* out;          // C x ny x nx
-    float2* gbuf;         // y-stage result [x][LyP][C] complex f32 (plan scratch)
+    float2* gbuf;         // y-stage result [x][n], column n = line*C + coil (plan scratch):
+                          // complex f32 in the FP32 path; in the split path each complex is {hi, lo}
+                          // of the split of (re, im) (packed re | im<<16 BF16 pairs), the x-stage's
+                          // B operand as is
     int32_t slice;
     int32_t group;
     uint64_t seed;

@@ -15,6 +15,8 @@
 //               into bf16 hi + lo and multiplied as hi*hi + hi*lo + lo*hi with FP32 accumulation;
 //               E is generated in registers directly in the MFMA B-operand layout (no LDS copy).
 //   plain FP32  one thread per (x, line) summing over y on the VALU, then per (row, kx).
+#include <algorithm>
+
 #include "devcommon.h"
 
 namespace mrs {
@@ -660,31 +662,44 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                 const int lp = (g0 + g) * 16 + ln;
 #pragma unroll
                 for (int vv = 0; vv < V; vv++) {
-                    GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gbuf[vv]) + ((size_t)x * LyP + lp) * C);
+                    // G is stored split (the x-stage's B operand): {hi, lo} per complex value at
+                    // [x][line*C + coil]
+                    GAS u32x2* gg = reinterpret_cast<GAS u32x2*>(gst(gbuf[vv]));
                     if constexpr (PAIRED) {
 #pragma unroll
                         for (int cp = 0; cp < RB / 2; cp++) {
                             const int c0 = cp * 16 + 4 * lq;
+                            const int n0 = lp * C + c0;
                             const f32x4 re = acc[vv][2 * cp][g], im = acc[vv][2 * cp + 1][g];
+                            u32x2 s[4];
+#pragma unroll
+                            for (int r = 0; r < 4; r++) s[r] = split_g(re[r], im[r]);
                             if (c0 + 4 <= C && (C & 1) == 0) {
-                                // 16-byte aligned: the buffer is, and (x*LyP + lp)*C + c0 is even
-                                GAS f32x4* d4 = reinterpret_cast<GAS f32x4*>(row + c0);
-                                d4[0] = f32x4{re[0], im[0], re[1], im[1]};
-                                d4[1] = f32x4{re[2], im[2], re[3], im[3]};
+                                // 16-byte aligned: the buffer is, and x*LyP*C + n0 is even
+                                GAS u32x4* d4 = reinterpret_cast<GAS u32x4*>(gg + (size_t)x * (LyP * C) + n0);
+                                d4[0] = u32x4{s[0].x, s[0].y, s[1].x, s[1].y};
+                                d4[1] = u32x4{s[2].x, s[2].y, s[3].x, s[3].y};
                             } else {
 #pragma unroll
                                 for (int r = 0; r < 4; r++)
-                                    if (c0 + r < C) row[c0 + r] = f32x2{re[r], im[r]};
+                                    if (c0 + r < C) gg[(size_t)x * (LyP * C) + n0 + r] = s[r];
                             }
                         }
                     } else {
+                        // Lanes lq < 2 hold the real parts of coils c0..c0+3 and the lanes 32 above
+                        // them the imaginary parts: the lower lane keeps coils c0, c0+1 and the upper
+                        // one c0+2, c0+3, each taking the two parts it lacks from its partner.
+                        const bool lower = lq < 2;
 #pragma unroll
                         for (int rb = 0; rb < RB; rb++) {
-#pragma unroll
-                            for (int r = 0; r < 4; r++) {
-                                const int c = rb * 8 + 4 * (lq & 1) + r;
-                                if (c < C) reinterpret_cast<GAS float*>(row + c)[lq >> 1] = acc[vv][rb][g][r];
-                            }
+                            const f32x4 v = acc[vv][rb][g];
+                            const float r0 = __shfl_xor(lower ? v[2] : v[0], 32);
+                            const float r1 = __shfl_xor(lower ? v[3] : v[1], 32);
+                            const int c = rb * 8 + 4 * (lq & 1) + (lower ? 0 : 2);
+                            const u32x2 s0 = lower ? split_g(v[0], r0) : split_g(r0, v[2]);
+                            const u32x2 s1 = lower ? split_g(v[1], r1) : split_g(r1, v[3]);
+                            if (c < C) gg[(size_t)x * (LyP * C) + lp * C + c] = s0;
+                            if (c + 1 < C) gg[(size_t)x * (LyP * C) + lp * C + c + 1] = s1;
                         }
                     }
                 }
@@ -702,150 +717,246 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 }
 
 // ----------------------------------------------------------------------------- x-stage, MFMA
-// Block = (unit, block of 16*MT kx rows, share `part` of the column tiles).  The block stages its
-// packed twiddle rows (hi and lo; both ghost tables when the group has a ghost) in LDS once; each
-// wave then streams 16-column tiles n = line*C + coil of G through them.  The complex product is
-// carried by the B operand: for each 16-row kx tile, one MFMA chain against B_re = [G_re, -G_im]
-// gives Re K and one against B_im = [G_im, G_re] gives Im K, so the twiddles enter as stored.
+// Persistent blocks, as many as are resident at once, over a list of items (unit, block of 16*MT kx
+// rows, share `part` of the unit's column tiles).  Block b takes the contiguous item range
+// [b*nwork/grid, (b+1)*nwork/grid) of a list sorted so that neighbours share a twiddle set; it
+// stages the packed twiddle rows (hi and lo; both ghost tables when the group has a ghost) and the
+// group's line facts in LDS, and stages again only when the set changes.  Each wave streams
+// 16-column tiles n = line*C + coil of G through them.  The complex product is carried by the B
+// operand: for each 16-row kx tile, one MFMA chain against B_re = [G_re, -G_im] gives Re K and one
+// against B_im = [G_im, G_re] gives Im K, so the twiddles enter as stored.
+//
+// Every wave walks the same sequence of steps (item, tile slot j, batch k of XB 16-x chunks): an
+// item has ceil(tiles / (4 nparts)) slots for every wave, a slot past the unit's tiles loading and
+// storing nothing, so the block's waves meet at the restaging barriers together.  The loads of the
+// next step are in flight while the current one is multiplied, across item boundaries.  Every step
+// issues all of its loads, a chunk past the last x or a step past the block's last item reading
+// from an offset beyond the buffer (the load returns zeros and touches no memory): with no load
+// under a branch the compiler's wait counts stay exact.  G arrives split ({hi, lo} per complex value,
+// written so by the y-stage): the loads are the B operands.
 template <int MT>
-__global__ __launch_bounds__(256) void xstage_mfma(const DevUnit* __restrict__ units,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void xstage_mfma(const DevUnit* __restrict__ units,
                                                    const DevGroup* __restrict__ groups,
-                                                   const WorkItem* __restrict__ work, int nparts) {
+                                                   const WorkItem* __restrict__ work, int nwork, int nparts) {
     extern __shared__ uint32_t tl[];
-    const WorkItem wi = work[blockIdx.x];
-    const DevUnit& u = units[wi.unit];
-    const DevGroup& gr = groups[u.group];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int lq = lane >> 4, ln = lane & 15;
-    const int C = gr.C, LyP = gr.LyP;
-    const int snx = gr.snx, snxP = (snx + 15) & ~15;
-    const int nx = gr.nx, nxP = (nx + 7) & ~7;
-    const int stride = snxP + 4;             // u32 per LDS row (16-byte pad against bank conflicts)
     constexpr int rows = 16 * MT;
-    const int kxb = wi.tile / nparts, part = wi.tile % nparts;
-    const int kx0 = kxb * rows;
-    const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
-    // Stage the twiddle rows with SU loads of each kind in flight per thread (one latency per SU
-    // rows of work rather than per row).
-    constexpr int SU = 8;
-    const int per_tab = rows * (snxP / 4);
-    for (int tb = 0; tb < ntab; tb++) {
-        for (int i0 = threadIdx.x; i0 < per_tab; i0 += 256 * SU) {
-            u32x4 h[SU], l[SU];
+    constexpr int XB = 8;
+    constexpr uint32_t kOutside = 0x80000000u;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lq = lane >> 4, ln = lane & 15;
+    const int i_lo = (int)(((int64_t)blockIdx.x * nwork) / gridDim.x);
+    const int i_hi = (int)(((int64_t)(blockIdx.x + 1) * nwork) / gridDim.x);
+    if (i_lo >= i_hi) return;
+
+    // A step cursor and its item's facts (wave-uniform).  Past the last item the facts stay those of
+    // the last one and the step loads nothing.
+    struct Cur {
+        int i, j, k;
+        int group, kxb, nt0;
+        int C, LyP, snxP, nx, ny, ntiles, nch, nbt, nslot;
+        const float2* gbuf;
+        float2* out;
+    };
+    auto enter = [&](Cur& c, int i) {
+        c.i = i;
+        c.j = 0;
+        c.k = 0;
+        if (i >= i_hi) return;
+        const WorkItem wi = work[i];
+        const DevUnit& u = units[wi.unit];
+        const DevGroup& gr = groups[u.group];
+        c.group = u.group;
+        c.kxb = wi.tile / nparts;
+        const int part = wi.tile % nparts;
+        c.C = gr.C;
+        c.LyP = gr.LyP;
+        c.snxP = (gr.snx + 15) & ~15;
+        c.nx = gr.nx;
+        c.ny = gr.ny;
+        c.ntiles = (gr.LyP * gr.C) >> 4;
+        c.nch = c.snxP >> 4;
+        c.nbt = (c.nch + XB - 1) / XB;
+        c.nt0 = part * 4 + wv;
+        c.nslot = (c.ntiles - part * 4 + 4 * nparts - 1) / (4 * nparts);
+        c.gbuf = u.gbuf;
+        c.out = u.out;
+    };
+    auto advance = [&](Cur& c) {
+        if (c.i >= i_hi) return;
+        if (++c.k < c.nbt) return;
+        c.k = 0;
+        if (++c.j < c.nslot) return;
+        enter(c, c.i + 1);
+    };
+
+    // Staging of the twiddle rows (SU loads of each kind in flight per thread: one latency per SU
+    // rows of work rather than per row) and of the line facts: per padded line 2*ky + table
+    // (negative for padding rows), read by the table choice and the stores.
+    int s_group = -1, s_kxb = -1, s_ntab = 1;
+    auto stage = [&](const Cur& c) {
+        const DevGroup& gr = groups[c.group];
+        const int snxP = c.snxP, nxP = (c.nx + 7) & ~7, stride = snxP + 4;
+        const int ntab = gr.t_hi[1] != gr.t_hi[0] ? 2 : 1;
+        const int kx0 = c.kxb * rows;
+        constexpr int SU = 8;
+        const int per_tab = rows * (snxP / 4);
+        for (int tb = 0; tb < ntab; tb++) {
+            for (int i0 = threadIdx.x; i0 < per_tab; i0 += 256 * SU) {
+                u32x4 h[SU], l[SU];
 #pragma unroll
-            for (int k = 0; k < SU; k++) {
-                const int i = i0 + k * 256;
-                const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
-                const int kx = kx0 + r;
-                h[k] = l[k] = u32x4{0u, 0u, 0u, 0u};
-                if (i < per_tab && kx < nxP) {
-                    h[k] = *at32<u32x4>(gr.t_hi[tb], (uint32_t)(kx * snxP + x4));
-                    l[k] = *at32<u32x4>(gr.t_lo[tb], (uint32_t)(kx * snxP + x4));
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < SU; k++) {
-                const int i = i0 + k * 256;
-                if (i < per_tab) {
+                for (int k = 0; k < SU; k++) {
+                    const int i = i0 + k * 256;
                     const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
-                    *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h[k];
-                    *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l[k];
+                    const int kx = kx0 + r;
+                    h[k] = l[k] = u32x4{0u, 0u, 0u, 0u};
+                    if (i < per_tab && kx < nxP) {
+                        h[k] = *at32<u32x4>(gr.t_hi[tb], (uint32_t)(kx * snxP + x4));
+                        l[k] = *at32<u32x4>(gr.t_lo[tb], (uint32_t)(kx * snxP + x4));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < SU; k++) {
+                    const int i = i0 + k * 256;
+                    if (i < per_tab) {
+                        const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
+                        *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h[k];
+                        *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 1) * rows + r) * stride + x4) = l[k];
+                    }
                 }
             }
         }
-    }
-    __syncthreads();
-    // The wave's column tiles nt = part*4 + wave + k*nparts*4, each a sequence of 16-x chunks of G
-    // loaded in batches of XB chunks (4 complex values per lane per chunk).  Batches run as one flat
-    // sequence over the wave's tiles, double-buffered: the next batch (of this tile or the next) is in
-    // flight while the current one is split and multiplied, so each wave keeps 4*XB loads in flight.
-    constexpr int XB = 8;
-    const int ntiles = (LyP * C) >> 4;
-    const int first = part * 4 + wave, step = nparts * 4;
-    const int ntw = first < ntiles ? (ntiles - first + step - 1) / step : 0;
-    const int nch = snxP >> 4;
-    const int nbt = (nch + XB - 1) / XB;
-    const int nb = ntw * nbt;
-    const uint32_t xstep = (uint32_t)(LyP * C);
-    const f32x2* gb = reinterpret_cast<const f32x2*>(u.gbuf);
-    auto load_batch = [&](int b, f32x2 (&buf)[XB][4]) {
-        const int nt = first + (b / nbt) * step, k = b % nbt;
-        const uint32_t n = (uint32_t)(nt * 16 + ln);
+        int32_t* const lrow = reinterpret_cast<int32_t*>(tl + ntab * 2 * rows * stride);
+        for (int i = threadIdx.x; i < c.LyP; i += 256) lrow[i] = gr.lines[i].ky * 2 + (gr.lines[i].table & 1);
+        s_group = c.group;
+        s_kxb = c.kxb;
+        s_ntab = ntab;
+    };
+
+    // The step's loads: G through a buffer resource, the row offset x*(LyP*C) (uniform) in the scalar
+    // offset and the lane's (its 4-x group, column) in the vector one, so no load needs address
+    // arithmetic.
+    auto load_step = [&](const Cur& c, u32x2 (&buf)[XB][4]) {
+        const int nt = c.nt0 + c.j * 4 * nparts;
+        const bool valid = c.i < i_hi && nt < c.ntiles;
+        const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)c.gbuf, (short)0, (int)((uint32_t)c.snxP * (uint32_t)(c.LyP * c.C) * 8u), 0x00020000);
+        const uint32_t xst = (uint32_t)(c.LyP * c.C);
+        const uint32_t off = valid ? ((uint32_t)(lq * 4) * xst + (uint32_t)(nt * 16 + ln)) * 8u : kOutside;
 #pragma unroll
-        for (int c = 0; c < XB; c++) {
-            const int ch = k * XB + c;
+        for (int cc = 0; cc < XB; cc++) {
+            const int ch = c.k * XB + cc;
+            const uint32_t o = ch < c.nch ? off : kOutside;
+            const uint32_t row = (uint32_t)(min(ch, c.nch - 1) * 16);
 #pragma unroll
-            for (int jj = 0; jj < 4; jj++) {
-                if (ch < nch) {
-                    buf[c][jj] = *at32<f32x2>(gb, (uint32_t)(ch * 16 + lq * 4 + jj) * xstep + n);
-                } else {
-                    buf[c][jj] = f32x2{0.f, 0.f};
-                }
-            }
+            for (int jj = 0; jj < 4; jj++)
+                buf[cc][jj] = __builtin_bit_cast(
+                    u32x2, __builtin_amdgcn_raw_buffer_load_b64(grs, o, (row + jj) * xst * 8u, 0));
         }
     };
+
     f32x4 accr[MT], acci[MT];
-    auto compute_batch = [&](int b, const f32x2 (&buf)[XB][4]) {
-        const int nt = first + (b / nbt) * step, k = b % nbt;
-        const int n0 = nt * 16;
-        const int tb = ntab == 2 ? gr.lines[n0 / C].table : 0;
+    auto compute_step = [&](const Cur& c, const u32x2 (&buf)[XB][4]) {
+        if (c.j == 0 && c.k == 0 && (c.group != s_group || c.kxb != s_kxb)) {
+            // every wave is at this step: none still reads the rows being replaced
+            __syncthreads();
+            stage(c);
+            __syncthreads();
+        }
+        const int stride = c.snxP + 4;
+        const int nt = c.nt0 + c.j * 4 * nparts;
+        const bool valid = nt < c.ntiles;
+        const int32_t* const lrow = reinterpret_cast<const int32_t*>(tl + s_ntab * 2 * rows * stride);
+        const int tb = (s_ntab == 2 && valid) ? __builtin_amdgcn_readfirstlane(lrow[(nt * 16) / c.C] & 1) : 0;
         const uint32_t* __restrict__ th = tl + (tb * 2 + 0) * rows * stride;
         const uint32_t* __restrict__ tlo = tl + (tb * 2 + 1) * rows * stride;
-        if (k == 0) {
+        if (c.k == 0) {
 #pragma unroll
             for (int m = 0; m < MT; m++) accr[m] = acci[m] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
 #pragma unroll
-        for (int c = 0; c < XB; c++) {
-            const int ch = k * XB + c;
-            if (ch < nch) {
+        for (int cc = 0; cc < XB; cc++) {
+            const int ch = c.k * XB + cc;
+            if (ch < c.nch) {
                 const int x0 = ch * 16 + lq * 4;
                 u32x4 bhi, blo;
 #pragma unroll
                 for (int jj = 0; jj < 4; jj++) {
-                    uint32_t h, l;
-                    split2(buf[c][jj].x, buf[c][jj].y, h, l);
-                    bhi[jj] = h;
-                    blo[jj] = l;
+                    bhi[jj] = buf[cc][jj].x;
+                    blo[jj] = buf[cc][jj].y;
                 }
                 // B_re = [G_re, -G_im] (negate the high halves), B_im = [G_im, G_re] (swap halves)
                 const u32x4 brh = rows_of(bhi, 0), brl = rows_of(blo, 0);
                 const u32x4 bih = rows_of(bhi, 1), bil = rows_of(blo, 1);
+                u32x4 h[MT], l[MT];
 #pragma unroll
                 for (int m = 0; m < MT; m++) {
                     const int r = m * 16 + ln;
-                    const u32x4 h = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
-                    const u32x4 l = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
-                    accr[m] = mfma3(h, l, brh, brl, accr[m]);
-                    acci[m] = mfma3(h, l, bih, bil, acci[m]);
+                    h[m] = *reinterpret_cast<const u32x4*>(th + r * stride + x0);
+                    l[m] = *reinterpret_cast<const u32x4*>(tlo + r * stride + x0);
+                }
+                // the three split terms, each over all accumulators before the next (every
+                // accumulator still sees lo*hi, hi*lo, hi*hi in that order)
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    accr[m] = mfma_bf16(l[m], brh, accr[m]);
+                    acci[m] = mfma_bf16(l[m], bih, acci[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    accr[m] = mfma_bf16(h[m], brl, accr[m]);
+                    acci[m] = mfma_bf16(h[m], bil, acci[m]);
+                }
+#pragma unroll
+                for (int m = 0; m < MT; m++) {
+                    accr[m] = mfma_bf16(h[m], brh, accr[m]);
+                    acci[m] = mfma_bf16(h[m], bih, acci[m]);
                 }
             }
         }
-        if (k == nbt - 1) {
-            const int n = n0 + ln;
-            const int lp = n / C, c = n % C;
-            const LineRec Lrow = gr.lines[lp];
-            if (Lrow.ky >= 0) {
-                GAS f32x2* orow = reinterpret_cast<GAS f32x2*>(gst(u.out) + ((size_t)c * gr.ny + Lrow.ky) * nx);
+        if (c.k == c.nbt - 1 && valid) {
+            const int n = nt * 16 + ln;
+            const int lp = n / c.C, coil = n % c.C;
+            const int ky = lrow[lp] >> 1;
+            if (ky >= 0) {
+                const int nx = c.nx;
+                GAS f32x2* orow = reinterpret_cast<GAS f32x2*>(gst(c.out) + ((size_t)coil * c.ny + ky) * nx);
 #pragma unroll
                 for (int m = 0; m < MT; m++) {
+                    const int kx = c.kxb * rows + m * 16 + 4 * lq;
+                    if ((nx & 1) == 0 && kx + 4 <= nx) {
+                        // 16-byte aligned: the row starts at an even complex index and kx is a multiple of 4
+                        GAS f32x4* o4 = reinterpret_cast<GAS f32x4*>(orow + kx);
+                        o4[0] = f32x4{accr[m][0], acci[m][0], accr[m][1], acci[m][1]};
+                        o4[1] = f32x4{accr[m][2], acci[m][2], accr[m][3], acci[m][3]};
+                    } else {
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int kx = kx0 + m * 16 + 4 * lq + r;
-                        if (kx < nx) orow[kx] = f32x2{accr[m][r], acci[m][r]};
+                        for (int r = 0; r < 4; r++)
+                            if (kx + r < nx) orow[kx + r] = f32x2{accr[m][r], acci[m][r]};
                     }
                 }
             }
         }
     };
-    f32x2 bufa[XB][4], bufb[XB][4];
-    if (nb > 0) load_batch(0, bufa);
-    for (int b = 0; b < nb; b += 2) {
-        if (b + 1 < nb) load_batch(b + 1, bufb);
-        compute_batch(b, bufa);
-        if (b + 1 >= nb) break;
-        if (b + 2 < nb) load_batch(b + 2, bufa);
-        compute_batch(b + 1, bufb);
+
+    Cur lc, cc;
+    enter(lc, i_lo);
+    enter(cc, i_lo);
+    stage(cc);
+    __syncthreads();
+    u32x2 bufa[XB][4], bufb[XB][4];
+    load_step(lc, bufa);
+    advance(lc);
+    while (cc.i < i_hi) {
+        load_step(lc, bufb);
+        advance(lc);
+        compute_step(cc, bufa);
+        advance(cc);
+        if (cc.i >= i_hi) break;
+        load_step(lc, bufa);
+        advance(lc);
+        compute_step(cc, bufb);
+        advance(cc);
     }
 }
 
@@ -2004,15 +2115,15 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
                 : ylaunch_q<false, false>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt);
 }
 
-hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
-                              const WorkItem* w, hipStream_t st) {
+hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
+                              const DevGroup* g, const WorkItem* w, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
+    const dim3 grid(std::max(1, std::min(nwork, nblk)));
     switch (mt) {
-    case 1: hipLaunchKernelGGL((xstage_mfma<1>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
-    case 2: hipLaunchKernelGGL((xstage_mfma<2>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
-    case 3: hipLaunchKernelGGL((xstage_mfma<3>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
-    case 4: hipLaunchKernelGGL((xstage_mfma<4>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
-    case 8: hipLaunchKernelGGL((xstage_mfma<8>), dim3(nwork), dim3(256), lds, st, u, g, w, nparts); break;
+    case 1: hipLaunchKernelGGL((xstage_mfma<1>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+    case 2: hipLaunchKernelGGL((xstage_mfma<2>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+    case 3: hipLaunchKernelGGL((xstage_mfma<3>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+    case 4: hipLaunchKernelGGL((xstage_mfma<4>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -2023,8 +2134,7 @@ hipError_t set_xstage_lds_limit(size_t bytes) {
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
     if ((e = hipFuncSetAttribute((const void*)xstage_mfma<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
-    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
-    return hipFuncSetAttribute((const void*)xstage_mfma<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return hipFuncSetAttribute((const void*)xstage_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,

@@ -23,8 +23,8 @@ hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, in
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
                               hipStream_t st);
-hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, const DevUnit* u, const DevGroup* g,
-                              const WorkItem* w, hipStream_t st);
+hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
+                              const DevGroup* g, const WorkItem* w, hipStream_t st);
 hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
                                const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
@@ -152,6 +152,7 @@ struct mrs_plan {
     struct XClass {
         int mt, nparts;
         size_t lds;
+        int nblk;   // persistent blocks: as many as are resident at once (by LDS, at most 2 per CU)
         std::vector<WorkItem> items;
         WorkItem* d_items = nullptr;
         std::vector<size_t> off;
@@ -767,31 +768,34 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         // share a CU), the unit's column tiles split over `nparts` blocks
         std::vector<int> g_mt(ngroups), g_np(ngroups);
         std::vector<size_t> g_lds(ngroups);
-        std::vector<size_t> g_units(ngroups, 0);
-        for (size_t u = 0; u < nunits; u++) g_units[p->units[u].group]++;
         for (size_t gi = 0; gi < ngroups; gi++) {
             const DevGroup& dg = p->groups[gi].dev;
             const int snxP = (dg.snx + 15) & ~15, nxP = (dg.nx + 7) & ~7;
             const int ntab = a_ghost(p->groups[gi]) ? 2 : 1;
             // kx tiles of 16 rows, MT (1..4) tiles per block: the fewest kx blocks (each block of a
             // unit reads all of its G), then the fewest padded rows, within the LDS budget
-            auto lds_of = [&](int m) { return (size_t)ntab * 2 * 16 * m * (snxP + 4) * 4; };
+            // (+ 4 bytes per padded line, at most 256 lines, for the line facts)
+            auto lds_of = [&](int m) { return (size_t)ntab * 2 * 16 * m * (snxP + 4) * 4 + 1024; };
             int mt = 1;
             for (int m = 2; m <= 4; m++) {
                 if (lds_of(m) > 80 * 1024) break;
                 const int kb = (nxP + 16 * m - 1) / (16 * m), kb0 = (nxP + 16 * mt - 1) / (16 * mt);
                 if (kb < kb0 || (kb == kb0 && kb * m < kb0 * mt)) mt = m;
             }
-            const int nkxb = (nxP + 16 * mt - 1) / (16 * mt);
+            // items of one column tile per wave (the persistent blocks balance over many items)
             const int ntiles = dg.LyP * dg.C / 16;
-            const size_t want = 1024 / std::max<size_t>(1, g_units[gi] * nkxb) + 1;
-            const int np = (int)std::max<size_t>(1, std::min<size_t>(want, std::max(1, ntiles / 4)));
+            const int np = std::max(1, ntiles / 4);
             g_mt[gi] = mt;
             g_np[gi] = np;
             g_lds[gi] = lds_of(mt);
         }
         if (set_xstage_lds_limit(160 * 1024) != hipSuccess || set_yadjoint_lds_limit(160 * 1024) != hipSuccess)
             rc = fail(err, errlen, MRS_E_DEVICE, "cannot raise the x-stage and y-adjoint LDS limits");
+        int dev = 0, ncu = 0;
+        if (rc == MRS_OK && (hipGetDevice(&dev) != hipSuccess ||
+                             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                             ncu <= 0))
+            rc = fail(err, errlen, MRS_E_DEVICE, "cannot read the device's compute-unit count");
         for (size_t k = 0; k < nunits && rc == MRS_OK; k++) {
             const size_t u = order[k];
             const int gi = p->units[u].group;
@@ -802,7 +806,8 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi]);
             if (!xcls.count(xkey)) {
                 xcls[xkey] = p->xclasses.size();
-                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], {}, nullptr, {}});
+                const int per_cu = (int)std::min<size_t>(2, std::max<size_t>(1, (160 * 1024) / g_lds[gi]));
+                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], per_cu * ncu, {}, nullptr, {}});
                 xw.emplace_back(p->nchunks);
             }
             const size_t xi = xcls[xkey];
@@ -810,7 +815,15 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const int nkxb = (nxP + 16 * g_mt[gi] - 1) / (16 * g_mt[gi]);
             for (int t = 0; t < nkxb * g_np[gi]; t++) xw[xi][ch].push_back({(int32_t)u, t});
         }
-        for (size_t c = 0; c < p->xclasses.size(); c++) p->xclasses[c].items = chunked(xw[c], p->xclasses[c].off);
+        // within a chunk, items by kx block (a persistent block's range then mostly shares one
+        // twiddle set), then in unit order
+        for (size_t c = 0; c < p->xclasses.size(); c++) {
+            const int np = p->xclasses[c].nparts;
+            for (auto& lst : xw[c])
+                std::stable_sort(lst.begin(), lst.end(),
+                                 [np](const WorkItem& a, const WorkItem& b) { return a.tile / np < b.tile / np; });
+            p->xclasses[c].items = chunked(xw[c], p->xclasses[c].off);
+        }
         for (size_t c = 0; c < p->yclasses.size() && rc == MRS_OK; c++)
             rc = upload_ylists(p, p->yclasses[c], order, err, errlen);
         for (auto& xc : p->xclasses) {
@@ -1012,8 +1025,8 @@ static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t e
     for (auto& c : p->yclasses) HIP_TRY(launch_yclass(p, c, k, st));
     for (auto& xc : p->xclasses) {
         const size_t a = xc.off[k], b = xc.off[k + 1];
-        HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(b - a), p->d_units, p->d_groups, xc.d_items + a,
-                                   st));
+        HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(b - a), xc.nblk, p->d_units, p->d_groups,
+                                   xc.d_items + a, st));
     }
     return MRS_OK;
 }
@@ -1243,7 +1256,7 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
             for (auto& xc : p->xclasses)
                 for (int k = 0; k < p->nchunks; k++)
                     HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(xc.off[k + 1] - xc.off[k]),
-                                               p->d_units, p->d_groups, xc.d_items + xc.off[k], st));
+                                               xc.nblk, p->d_units, p->d_groups, xc.d_items + xc.off[k], st));
         } else {
             for (size_t u = 0; u < p->units.size(); u++) {
                 const DevGroup& dg = p->groups[p->units[u].group].dev;

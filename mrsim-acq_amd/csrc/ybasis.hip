@@ -313,16 +313,21 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
         for (int t = 0; t < 4; t++) {
             if (t >= nt) continue;
             const f32x4 re = acc[i][t][0], im = acc[i][t][1];
-            GAS f32x2* row = reinterpret_cast<GAS f32x2*>(gst(gb) + ((size_t)x * LyP + t * 16 + ln) * C);
+            // stored split, the x-stage's B operand: {hi, lo} per complex value at [x][line*C + coil]
+            u32x2 s[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) s[r] = split_g(re[r], im[r]);
+            const int n0 = (t * 16 + ln) * C + c0;
+            GAS u32x2* gg = reinterpret_cast<GAS u32x2*>(gst(gb));
             if (C == 8) {
-                // 16-byte aligned: (x*LyP + line)*8 + c0 is a multiple of 4 complex values
-                GAS f32x4* d4 = reinterpret_cast<GAS f32x4*>(row + c0);
-                d4[0] = f32x4{re[0], im[0], re[1], im[1]};
-                d4[1] = f32x4{re[2], im[2], re[3], im[3]};
+                // 32-byte aligned: x*LyP*8 + n0 is a multiple of 4 complex values
+                GAS u32x4* d4 = reinterpret_cast<GAS u32x4*>(gg + (size_t)x * (LyP * C) + n0);
+                d4[0] = u32x4{s[0].x, s[0].y, s[1].x, s[1].y};
+                d4[1] = u32x4{s[2].x, s[2].y, s[3].x, s[3].y};
             } else {
 #pragma unroll
                 for (int r = 0; r < 4; r++)
-                    if (c0 + r < C) row[c0 + r] = f32x2{re[r], im[r]};
+                    if (c0 + r < C) gg[(size_t)x * (LyP * C) + n0 + r] = s[r];
             }
         }
     }
