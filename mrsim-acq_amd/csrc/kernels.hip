@@ -961,16 +961,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
 }
 
 // ----------------------------------------------------------------------------- plain FP32
-// y-stage: FP32 complex MACs over y for all coils on the VALU.  Block = 256 threads = YS y-parts x
-// (256 / YS) consecutive padded lines of one column x; the parts' partial sums are added in LDS in a
-// fixed order (deterministic).  grid = snx * ceil(LyP / (256 / YS)) blocks per unit: a 128^2 slice of
-// 64 lines fills 256 blocks.
+// The FP32 reference path: one launch pair per slice unit, all arithmetic in FP32 on the VALU (the
+// phase formed and reduced in FP64), partial sums added in a fixed order (deterministic).
+
+// Sum of YS partial results per output: part[k][o] for k = 0..YS-1, o < nout, added in k order by
+// one thread per output (all threads of the block take outputs in turn).
+template <int YS>
+__device__ __forceinline__ float2 plain_sum(const float2* part, int nout, int o) {
+    float2 t = part[o];
+#pragma unroll 8
+    for (int k = 1; k < YS; k++) {
+        const float2 v = part[k * nout + o];
+        t.x += v.x;
+        t.y += v.y;
+    }
+    return t;
+}
+
+// y-stage: FP32 complex MACs over y for all coils.  Block = 256 threads = YS y-parts x (256 / YS)
+// consecutive padded lines of one column x; grid = snx * ceil(LyP / (256 / YS)) blocks per unit (a
+// 128^2 slice of 64 lines at YS = 32: 1024 blocks, 4 y per thread).  The y loop runs in pairs whose
+// loads are all issued before either value's arithmetic.
 template <int CM, int YS>
 __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ units,
                                                     const DevSlice* __restrict__ slices,
                                                     const DevGroup* __restrict__ groups, int unit) {
     constexpr int NL = 256 / YS;   // lines per block
-    __shared__ float2 part[YS][NL][CM];
+    __shared__ float2 part[YS * NL * CM];
     const DevUnit& u = units[unit];
     const DevGroup& gr = groups[u.group];
     const DevSlice& sl = slices[u.slice];
@@ -989,7 +1006,8 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
     const int ylen = (sny + YS - 1) / YS, y0 = g * ylen, y1 = min(sny, y0 + ylen);
     if (x < snx && lp < LyP && gr.lines[lp].ky >= 0) {
         const LineRec L = gr.lines[lp];
-        for (int y = y0; y < y1; y++) {
+        // E of voxel (x, y) at this line: the compartment sum with its decays, times cis(2 pi cyc)
+        auto e_of = [&](int y, float& er, float& ei) {
             const size_t o = (size_t)x * snyP + y;
             const size_t i = (size_t)x + (size_t)snx * y;
             double cyc = __builtin_fma(L.kyn, (double)y - gr.sys_yoff, sl.base[o]);
@@ -999,16 +1017,20 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
             sincos_turns(cyc, sn, cs);
             float wr = 0.f, wi = 0.f;
             for (int q = 0; q < Q; q++) {
-                float dq = __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, sl.nr2[q * PP + o], L.tau * sl.nri[q * PP + o]));
+                const float dq =
+                    __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, sl.nr2[q * PP + o], L.tau * sl.nri[q * PP + o]));
                 if (cplx) {
-                    float2 m = reinterpret_cast<const float2*>(u.images)[q * P + i];
+                    const float2 m = reinterpret_cast<const float2*>(u.images)[q * P + i];
                     wr = __builtin_fmaf(m.x, dq, wr);
                     wi = __builtin_fmaf(m.y, dq, wi);
                 } else {
                     wr = __builtin_fmaf(reinterpret_cast<const float*>(u.images)[q * P + i], dq, wr);
                 }
             }
-            const float er = wr * cs - wi * sn, ei = wr * sn + wi * cs;
+            er = wr * cs - wi * sn;
+            ei = wr * sn + wi * cs;
+        };
+        auto mac = [&](int y, float er, float ei) {
 #pragma unroll
             for (int c = 0; c < CM; c++) {
                 if (c < C) {
@@ -1017,29 +1039,39 @@ __global__ __launch_bounds__(256) void ystage_plain(const DevUnit* __restrict__ 
                     acc[c].y = __builtin_fmaf(s.x, ei, __builtin_fmaf(s.y, er, acc[c].y));
                 }
             }
+        };
+        int y = y0;
+        for (; y + 1 < y1; y += 2) {
+            float er0, ei0, er1, ei1;
+            e_of(y, er0, ei0);
+            e_of(y + 1, er1, ei1);
+            mac(y, er0, ei0);
+            mac(y + 1, er1, ei1);
+        }
+        if (y < y1) {
+            float er, ei;
+            e_of(y, er, ei);
+            mac(y, er, ei);
         }
     }
 #pragma unroll
-    for (int c = 0; c < CM; c++) part[g][li][c] = acc[c];
+    for (int c = 0; c < CM; c++) part[(g * NL + li) * CM + c] = acc[c];
     __syncthreads();
-    if (g == 0 && x < snx && lp < LyP) {
-        for (int c = 0; c < C; c++) {
-            float2 t = part[0][li][c];
-            for (int k = 1; k < YS; k++) {
-                t.x += part[k][li][c].x;
-                t.y += part[k][li][c].y;
-            }
-            u.gbuf[((size_t)x * LyP + lp) * C + c] = t;
-        }
+    // outputs (line li, coil c) = part index li*CM + c; the YS parts of one output lie NL*CM apart
+    for (int o = threadIdx.x; o < NL * CM; o += 256) {
+        const int l = o / CM, c = o % CM;
+        if (c < C && x < snx && l0 + l < LyP) u.gbuf[((size_t)x * LyP + l0 + l) * C + c] = plain_sum<YS>(part, NL * CM, o);
     }
 }
 
-// x-stage: FP32 on the VALU.  Block = 256 threads = 4 x-parts x 64 consecutive (column n = line*C +
-// coil, kx) outputs; the parts' partial sums are added in LDS in a fixed order.
+// x-stage: FP32 on the VALU.  Block = 256 threads = XS x-parts x (256 / XS) consecutive (column n =
+// line*C + coil, kx) outputs; grid = ceil(LyP*C*nx / (256 / XS)) blocks (a 64-line 8-coil 64^2
+// k-space at XS = 16: 2048 blocks, 8 x per thread).
+template <int XS>
 __global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ units,
                                                     const DevGroup* __restrict__ groups, int unit) {
-    constexpr int XS = 4, NO = 64;
-    __shared__ float2 part[XS][NO];
+    constexpr int NO = 256 / XS;
+    __shared__ float2 part[XS * NO];
     const DevUnit& u = units[unit];
     const DevGroup& gr = groups[u.group];
     const int nx = gr.nx, C = gr.C, LyP = gr.LyP, snx = gr.snx, snxP = (snx + 15) & ~15;
@@ -1053,23 +1085,19 @@ __global__ __launch_bounds__(256) void xstage_plain(const DevUnit* __restrict__ 
     const int xlen = (snx + XS - 1) / XS, x0 = g * xlen, x1 = min(snx, x0 + xlen);
     if (live && L.ky >= 0) {
         const float2* __restrict__ tw = gr.t32[L.table] + (size_t)kx * snxP;
+        const float2* __restrict__ gcol = u.gbuf + (size_t)lp * C + c;
+        const size_t gstep = (size_t)LyP * C;
+#pragma unroll 4
         for (int x = x0; x < x1; x++) {
-            const float2 gv = u.gbuf[((size_t)x * LyP + lp) * C + c];
+            const float2 gv = gcol[(size_t)x * gstep];
             const float2 w = tw[x];
             ar = __builtin_fmaf(gv.x, w.x, __builtin_fmaf(-gv.y, w.y, ar));
             ai = __builtin_fmaf(gv.x, w.y, __builtin_fmaf(gv.y, w.x, ai));
         }
     }
-    part[g][oi] = make_float2(ar, ai);
+    part[g * NO + oi] = make_float2(ar, ai);
     __syncthreads();
-    if (g == 0 && live && L.ky >= 0) {
-        float2 r = part[0][oi];
-        for (int k = 1; k < XS; k++) {
-            r.x += part[k][oi].x;
-            r.y += part[k][oi].y;
-        }
-        u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = r;
-    }
+    if (g == 0 && live && L.ky >= 0) u.out[((size_t)c * gr.ny + L.ky) * nx + kx] = plain_sum<XS>(part, NO, oi);
 }
 
 // ----------------------------------------------------------------------------- adjoint (VJP)
@@ -2139,16 +2167,17 @@ hipError_t set_xstage_lds_limit(size_t bytes) {
 
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st) {
-    // y parts per block: 8 (32 lines per block) up to 16 coils, 2 above (the LDS of the partial sums)
     auto gy = [&](int ys) { return dim3(snx * ((LyP + 256 / ys - 1) / (256 / ys))); };
-    if (C <= 8) hipLaunchKernelGGL((ystage_plain<8, 8>), gy(8), dim3(256), 0, st, u, s, g, unit);
-    else if (C <= 16) hipLaunchKernelGGL((ystage_plain<16, 8>), gy(8), dim3(256), 0, st, u, s, g, unit);
-    else if (C <= 32) hipLaunchKernelGGL((ystage_plain<32, 2>), gy(2), dim3(256), 0, st, u, s, g, unit);
-    else hipLaunchKernelGGL((ystage_plain<64, 2>), gy(2), dim3(256), 0, st, u, s, g, unit);
+    // y-parts per block: 32 up to 8 coils (a 128^2 slice fills 1024 blocks), 16 up to 16, 4 above
+    // (the LDS of the partial sums is 2 KiB per coil slot whatever the split)
+    if (C <= 8) hipLaunchKernelGGL((ystage_plain<8, 32>), gy(32), dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 16) hipLaunchKernelGGL((ystage_plain<16, 16>), gy(16), dim3(256), 0, st, u, s, g, unit);
+    else if (C <= 32) hipLaunchKernelGGL((ystage_plain<32, 4>), gy(4), dim3(256), 0, st, u, s, g, unit);
+    else hipLaunchKernelGGL((ystage_plain<64, 4>), gy(4), dim3(256), 0, st, u, s, g, unit);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nxt = LyP * C * nx;
-    hipLaunchKernelGGL(xstage_plain, dim3((nxt + 63) / 64), dim3(256), 0, st, u, g, unit);
+    hipLaunchKernelGGL((xstage_plain<16>), dim3((nxt + 15) / 16), dim3(256), 0, st, u, g, unit);
     return hipGetLastError();
 }
 
