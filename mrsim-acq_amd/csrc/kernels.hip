@@ -1804,10 +1804,15 @@ __global__ __launch_bounds__(256) void zstage(const ZSet* __restrict__ sets, con
 // NT = the 16-row tiles of ny (1..8, ny <= 128), which size the register arrays.
 template <int NT>
 __global__ __launch_bounds__(256) void recon_units(const ReconGeom* __restrict__ gp, const ReconItem* __restrict__ items,
-                                                   int blocks_per_unit) {
+                                                   int nitems, int blocks_per_unit) {
     extern __shared__ uint32_t rlds[];
     const ReconGeom G = *gp;
-    const int item = blockIdx.x / blocks_per_unit, t = blockIdx.x % blocks_per_unit;
+    // XCD-aware order: workgroups go to the 8 XCDs round-robin, so blocks b, b+8, b+16, ... of a
+    // group of 8*blocks_per_unit share an XCD; those are one unit's x tiles, whose re-reads of the
+    // unit's k-space then hit that XCD's L2 instead of HBM
+    const int grp8 = blockIdx.x / (8 * blocks_per_unit), in8 = blockIdx.x % (8 * blocks_per_unit);
+    const int item = grp8 * 8 + in8 % 8, t = in8 / 8;
+    if (item >= nitems) return;
     const ReconItem it = items[item];
     const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
     const int lq = lane >> 4, ln = lane & 15;
@@ -1954,16 +1959,16 @@ hipError_t set_recon_lds_limit(size_t bytes) {
 hipError_t launch_recon(const ReconGeom* g, const ReconItem* items, int nitems, int blocks_per_unit, int nty,
                         size_t lds, hipStream_t st) {
     if (nitems <= 0) return hipSuccess;
-    const dim3 grid(nitems * blocks_per_unit), block(256);
+    const dim3 grid(((nitems + 7) / 8) * 8 * blocks_per_unit), block(256);
     switch (nty) {
-    case 1: hipLaunchKernelGGL((recon_units<1>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 2: hipLaunchKernelGGL((recon_units<2>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 3: hipLaunchKernelGGL((recon_units<3>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 4: hipLaunchKernelGGL((recon_units<4>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 5: hipLaunchKernelGGL((recon_units<5>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 6: hipLaunchKernelGGL((recon_units<6>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 7: hipLaunchKernelGGL((recon_units<7>), grid, block, lds, st, g, items, blocks_per_unit); break;
-    case 8: hipLaunchKernelGGL((recon_units<8>), grid, block, lds, st, g, items, blocks_per_unit); break;
+    case 1: hipLaunchKernelGGL((recon_units<1>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 2: hipLaunchKernelGGL((recon_units<2>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 3: hipLaunchKernelGGL((recon_units<3>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 4: hipLaunchKernelGGL((recon_units<4>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 5: hipLaunchKernelGGL((recon_units<5>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 6: hipLaunchKernelGGL((recon_units<6>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 7: hipLaunchKernelGGL((recon_units<7>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
+    case 8: hipLaunchKernelGGL((recon_units<8>), grid, block, lds, st, g, items, nitems, blocks_per_unit); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
