@@ -22,7 +22,8 @@
 //
 // Work decomposition.  Block = 4 waves = one (set, column x).  A set holds up to kBasisMax units of
 // one slice, in pairs (2p, 2p + 1); wave w takes the pairs w, w + 4, w + 8 (NPF of them) and all four
-// 16-line tiles (64 acquired lines at most), so each A is formed once and used by every line tile.
+// 16-line tiles (64 acquired lines at most), so each A is formed once and used by every line tile;
+// a last round of one or two pairs is split over the waves by line tile (see split_last).
 // Per 16-y chunk:
 //   1. barrier; B (generated in registers after the previous chunk's contraction) is written to LDS
 //      in the MFMA B-operand layout (both forms, hi and lo), with the units' images of the chunk
@@ -76,6 +77,18 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
     const int nch = snyP >> 4;
     const uint32_t P = (uint32_t)(snx * sny);
     const double sys_yoff = gr.sys_yoff;
+    // Pairs per round: wave w takes pair w + 4 i in rounds i < NPF - 1.  When the last round holds
+    // one or two pairs and the lines fill four tiles, its (pair, line tile) products are split over
+    // the four waves instead (two tiles of one of two pairs, or one tile of the single pair), so no
+    // wave multiplies a pair slot past the set's units: the 10 pairs of 20 volumes then cost 10
+    // pairs' MFMA work, not 12.  Each wave forms A of its last-round pair itself.
+    const int rlast = (nu + 1) / 2 - 4 * (NPF - 1);
+    const bool split_last = nt == 4 && (rlast == 1 || rlast == 2);
+    const int last_pair = 4 * (NPF - 1) + (!split_last ? wave : (rlast == 2 ? (wave >> 1) : 0));
+    const int t_lo = !split_last ? 0 : (rlast == 2 ? 2 * (wave & 1) : wave);
+    const int t_hi = !split_last ? 4 : t_lo + (rlast == 2 ? 2 : 1);
+    auto pair_of = [&](int i) { return i < NPF - 1 ? wave + 4 * i : last_pair; };
+    auto tile_on = [&](int i, int t) { return i < NPF - 1 || (t >= t_lo && t < t_hi); };
 
     // this lane's line of the B generation (lane = line; LyP <= 64)
     double t_s = 0.0, kyn = 0.0;
@@ -234,7 +247,7 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
             u32x4 ah[NPF], al[NPF];
 #pragma unroll
             for (int i = 0; i < NPF; i++) {
-                const int ul = 2 * (wave + 4 * i) + (ln >> 3);
+                const int ul = 2 * pair_of(i) + (ln >> 3);
                 const float* mp = sm.img + ((min(ul, NU - 1) * Q + q) * 16 + 4 * lq) * CM;
                 if constexpr (CPLX) {
                     const f32x4 m01 = *reinterpret_cast<const f32x4*>(mp);
@@ -285,16 +298,19 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
                 }
 #pragma unroll
                 for (int i = 0; i < NPF; i++)
+                    if (tile_on(i, t))
 #pragma unroll
-                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(al[i], bh[f], acc[i][t][f]);
-#pragma unroll
-                for (int i = 0; i < NPF; i++)
-#pragma unroll
-                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bl[f], acc[i][t][f]);
+                        for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(al[i], bh[f], acc[i][t][f]);
 #pragma unroll
                 for (int i = 0; i < NPF; i++)
+                    if (tile_on(i, t))
 #pragma unroll
-                    for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bh[f], acc[i][t][f]);
+                        for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bl[f], acc[i][t][f]);
+#pragma unroll
+                for (int i = 0; i < NPF; i++)
+                    if (tile_on(i, t))
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bh[f], acc[i][t][f]);
             }
         }
         // B of the next chunk, while other waves still contract this one
@@ -305,13 +321,13 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
     // the set, coils 4 (lq & 1) .. +3, line 16 * tile + ln
 #pragma unroll
     for (int i = 0; i < NPF; i++) {
-        const int ul = 2 * (wave + 4 * i) + (lq >> 1);
+        const int ul = 2 * pair_of(i) + (lq >> 1);
         if (ul >= nu) continue;
         float2* gb = units[set[1 + ul]].gbuf;
         const int c0 = 4 * (lq & 1);
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            if (t >= nt) continue;
+            if (t >= nt || !tile_on(i, t)) continue;
             const f32x4 re = acc[i][t][0], im = acc[i][t][1];
             // stored split, the x-stage's B operand: {hi, lo} per complex value at [x][line*C + coil]
             u32x2 s[4];

@@ -332,6 +332,10 @@ def _basis_workload(name):
         return workloads.make("batch_128", n_slices=2, n_volumes=7)
     if name == "batch_q3_v45":     # three sets of 15 units per slice
         return workloads.make("batch_128", n_slices=1, n_volumes=45)
+    if name == "batch_q3_v18":     # 9 pairs: the last round's single pair split one line tile per wave
+        return workloads.make("batch_128", n_slices=2, n_volumes=18)
+    if name == "batch_q3_v11":     # 6 pairs, the last with one unit: the last round split two tiles per wave
+        return workloads.make("batch_128", n_slices=2, n_volumes=11)
     # small grids: 40^2 -> 20^2 (three 16-row chunks, the last one ragged; 20 lines padded to 32: the
     # upper line tiles are idle),
     # 3 coils (rows of absent coils), Q = 1, 2 or 4, real or complex images
@@ -352,8 +356,8 @@ def _basis_workload(name):
     return w
 
 
-@pytest.mark.parametrize("name", ["batch_q3_v20", "batch_q3_v7", "batch_q3_v45", "small_q1", "small_q2_cplx",
-                                  "small_q4"])
+@pytest.mark.parametrize("name", ["batch_q3_v20", "batch_q3_v7", "batch_q3_v45", "batch_q3_v18", "batch_q3_v11",
+                                  "small_q1", "small_q2_cplx", "small_q4"])
 def test_slice_basis_matches_per_unit_and_oracle(name):
     # The volumes of a slice in the slice-basis y-stage (per-slice phase and decay factors shared by
     # every unit, per-unit coil-weighted images on the MFMA) against the per-unit y-stage and the
