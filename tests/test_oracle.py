@@ -42,7 +42,7 @@ def test_oracle_matches_golden_slice_bit_for_bit(shape, name):
 
 def test_kspace_build_fixtures_agree_with_default_to_round_off():
     # The reference's two builds (twiddle table vs rustfft x-stage) differ only in round-off, so one
-    # restatement pins both (SURVEY: two elements differ by <= 2.9e-11).
+    # restatement pins both (the fixtures' two differing elements differ by <= 2.9e-11).
     worst = 0.0
     for f in sorted(os.listdir(os.path.join(GOLDEN, "default"))):
         a = _golden("default", f[:-4])

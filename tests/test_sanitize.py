@@ -1,5 +1,5 @@
-"""AddressSanitizer + UndefinedBehaviorSanitizer runs of the host code (SURVEY-style aux check of the
-CPU side): the C oracle over its entry points (`make -C oracle sanitize`, driver
+"""AddressSanitizer + UndefinedBehaviorSanitizer runs of the host code (a memory and undefined-behaviour
+check of the CPU side): the C oracle over its entry points (`make -C oracle sanitize`, driver
 oracle/sanitize_check.c) and the library's host protocol tables (tables.cpp, driver
 tests/sanitize/tables_check.cpp).  Every finding aborts the driver with a non-zero status."""
 import os
