@@ -150,6 +150,48 @@ def test_batch_128_units_match_fp64_oracle(precision):
     assert max(errs) <= TOL
 
 
+def test_full_600_unit_batch_matches_oracle_and_scales_exactly():
+    # The benchmark's whole batch (30 slices x 20 volumes): 12 units spread over slices and volumes
+    # against the FP64 oracle, both y-stage paths on every unit, and doubled images giving exactly
+    # doubled k-spaces on every unit (the forward is linear in the images and a power-of-two scale
+    # passes exactly through the BF16 split and the FP32 accumulation).
+    import torch
+    w = workloads.make("batch_128")
+    assert len(w.units) == 600
+    plan, outs, keep = workloads.to_plan(w)
+    res = {}
+    for basis in (True, False):
+        plan.set_basis(basis)
+        plan.set_tiles(8, 4, 1 if basis else 4)
+        plan.forward_graph()
+        torch.cuda.synchronize()
+        res[basis] = [o.clone() for o in outs]
+    worst = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+                for a, b in zip(res[True], res[False]))
+    print(f"600 units: max rel diff slice basis vs per-unit sets {worst:.2e}", flush=True)
+    assert worst <= TOL
+    for u in np.linspace(0, 599, 12).astype(int):
+        e = rel(res[True][u].cpu().numpy(), O.forward_workload_unit(w, int(u)))
+        print(f"  unit {u}: rel-L2 vs FP64 {e:.2e}", flush=True)
+        assert e <= TOL
+    # the units' image tensors (the slices' inputs are dicts) doubled in place: the y-stage reads
+    # them directly, so every output must come out exactly doubled
+    n = 0
+    for t in keep:
+        if isinstance(t, torch.Tensor):
+            t.mul_(2.0)
+            n += 1
+    assert n == 600
+    plan.set_basis(True)
+    plan.set_tiles(8, 4, 1)
+    plan.forward()
+    torch.cuda.synchronize()
+    bad = sum(int(not torch.equal(o, 2 * r)) for o, r in zip(outs, res[True]))
+    print(f"  doubled images: {600 - bad} of 600 units exactly doubled", flush=True)
+    assert bad == 0
+    plan.close()
+
+
 def test_slice_128_matches_fp64_oracle():
     w = workloads.make("slice_128")
     for prec in (abi.PREC_SPLIT_BF16, abi.PREC_FP32):
