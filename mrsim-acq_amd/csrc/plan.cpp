@@ -25,7 +25,7 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
                               hipStream_t st);
 hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
                               const DevGroup* g, const WorkItem* w, hipStream_t st);
-hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
+hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int last, int nwork, const DevUnit* u, const DevSlice* s,
                                const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st);
 hipError_t set_xstage_lds_limit(size_t bytes);
 hipError_t set_yadjoint_lds_limit(size_t bytes);
@@ -136,7 +136,8 @@ struct mrs_plan {
         int32_t* d_usets = nullptr;
         size_t cap_items = 0, cap_vitems = 0, cap_usets = 0;
         // slice-basis sets (kBasisStride ints each) and their (set, column) items, one list per
-        // pairs-per-wave count npf = 1..3 (index npf - 1)
+        // (pairs-per-wave count npf = 1..3, last-round tiles per wave 4 / 2 / 1): index
+        // 3 (npf - 1) + {0, 1, 2}
         std::vector<int32_t> bsets;
         int32_t* d_bsets = nullptr;
         size_t cap_bsets = 0;
@@ -146,7 +147,7 @@ struct mrs_plan {
             size_t cap = 0;
             std::vector<size_t> off;
         };
-        BList bl[3];
+        BList bl[9];
     };
     std::vector<YClass> yclasses;
     struct XClass {
@@ -878,7 +879,7 @@ static bool set_eligible(const mrs_plan* p, const mrs_plan::YClass& c, int gp) {
 static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs_plan::YClass& c, int xt, int gp,
                   std::vector<std::vector<WorkItem>>& single, std::vector<std::vector<WorkItem>>& multi,
                   std::vector<int32_t>& usets, std::vector<int32_t>& bsets,
-                  std::vector<std::vector<WorkItem>> (&bper)[3]) {
+                  std::vector<std::vector<WorkItem>> (&bper)[9]) {
     single.assign(p->nchunks, {});
     multi.assign(p->nchunks, {});
     usets.clear();
@@ -912,11 +913,15 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
                 for (size_t si = 0; si < nsets; si++) {
                     const size_t cnt = n / nsets + (si < n % nsets ? 1 : 0);
                     const int npf = (int)(((cnt + 1) / 2 + 3) / 4);   // unit pairs per wave
+                    // pairs in the last round; one or two of a 64-line group are split by line tile
+                    const int rl = (int)((cnt + 1) / 2) - 4 * (npf - 1);
+                    const int lyp = p->groups[p->units[mine[a]].group].dev.LyP;
+                    const int li = lyp == 64 && rl == 2 ? 1 : (lyp == 64 && rl == 1 ? 2 : 0);
                     const int32_t sidx = (int32_t)(bsets.size() / kBasisStride);
                     bsets.push_back((int32_t)cnt);
                     for (int v = 0; v < kBasisMax; v++) bsets.push_back((int32_t)mine[k + std::min<size_t>(v, cnt - 1)]);
                     k += cnt;
-                    for (int x = 0; x < snx; x++) bper[npf - 1][ch].push_back({sidx, x});
+                    for (int x = 0; x < snx; x++) bper[3 * (npf - 1) + li][ch].push_back({sidx, x});
                 }
                 a = b;
                 continue;
@@ -948,7 +953,7 @@ static void ylist(const mrs_plan* p, const std::vector<size_t>& order, const mrs
 // Rebuilds and uploads a class's work lists for the current tiles.
 static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<size_t>& order, char* err,
                          size_t errlen) {
-    std::vector<std::vector<WorkItem>> single, multi, bper[3];
+    std::vector<std::vector<WorkItem>> single, multi, bper[9];
     std::vector<int32_t> usets, bsets;
     ylist(p, order, c, p->xt, p->gp, single, multi, usets, bsets, bper);
     c.nv = set_eligible(p, c, p->gp) ? p->nv : 1;
@@ -956,7 +961,7 @@ static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<siz
     c.vitems = chunked(multi, c.voff);
     c.usets = usets;
     c.bsets = bsets;
-    for (int k = 0; k < 3; k++) c.bl[k].items = chunked(bper[k], c.bl[k].off);
+    for (int k = 0; k < 9; k++) c.bl[k].items = chunked(bper[k], c.bl[k].off);
     auto put = [&](auto*& dptr, size_t& cap, const auto& host) -> int {
         using T = typename std::decay_t<decltype(host)>::value_type;
         const size_t n = std::max<size_t>(host.size(), 1);
@@ -972,17 +977,18 @@ static int upload_ylists(mrs_plan* p, mrs_plan::YClass& c, const std::vector<siz
     if (!rc) rc = put(c.d_vitems, c.cap_vitems, c.vitems);
     if (!rc) rc = put(c.d_usets, c.cap_usets, c.usets);
     if (!rc) rc = put(c.d_bsets, c.cap_bsets, c.bsets);
-    for (int k = 0; k < 3 && !rc; k++) rc = put(c.bl[k].d_items, c.bl[k].cap, c.bl[k].items);
+    for (int k = 0; k < 9 && !rc; k++) rc = put(c.bl[k].d_items, c.bl[k].cap, c.bl[k].items);
     return rc;
 }
 
 static hipError_t launch_yclass(const mrs_plan* p, const mrs_plan::YClass& c, int k, hipStream_t st) {
     hipError_t e = hipSuccess;
-    for (int n = 0; n < 3 && e == hipSuccess; n++) {
+    static const int kLast[3] = {4, 2, 1};
+    for (int n = 0; n < 9 && e == hipSuccess; n++) {
         const auto& b = c.bl[n];
         if (b.off.size() > (size_t)k + 1 && b.off[k + 1] > b.off[k])
-            e = launch_ystage_basis(c.Q, c.cplx != 0, n + 1, (int)(b.off[k + 1] - b.off[k]), p->d_units, p->d_slices,
-                                    p->d_groups, b.d_items + b.off[k], c.d_bsets, st);
+            e = launch_ystage_basis(c.Q, c.cplx != 0, n / 3 + 1, kLast[n % 3], (int)(b.off[k + 1] - b.off[k]),
+                                    p->d_units, p->d_slices, p->d_groups, b.d_items + b.off[k], c.d_bsets, st);
     }
     if (e != hipSuccess) return e;
     const size_t va = c.voff[k], vb = c.voff[k + 1];

@@ -23,7 +23,7 @@
 // Work decomposition.  Block = 4 waves = one (set, column x).  A set holds up to kBasisMax units of
 // one slice, in pairs (2p, 2p + 1); wave w takes the pairs w, w + 4, w + 8 (NPF of them) and all four
 // 16-line tiles (64 acquired lines at most), so each A is formed once and used by every line tile;
-// a last round of one or two pairs is split over the waves by line tile (see split_last).
+// a last round of one or two pairs is split over the waves by line tile (template LAST).
 // Per 16-y chunk:
 //   1. barrier; B (generated in registers after the previous chunk's contraction) is written to LDS
 //      in the MFMA B-operand layout (both forms, hi and lo), with the units' images of the chunk
@@ -51,7 +51,7 @@ struct BasisSmem {
     float nri[Q][kBasisMaxY];
 };
 
-template <int Q, bool CPLX, int NPF>
+template <int Q, bool CPLX, int NPF, int LAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
                   const DevGroup* __restrict__ groups, const WorkItem* __restrict__ work,
@@ -77,18 +77,18 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
     const int nch = snyP >> 4;
     const uint32_t P = (uint32_t)(snx * sny);
     const double sys_yoff = gr.sys_yoff;
-    // Pairs per round: wave w takes pair w + 4 i in rounds i < NPF - 1.  When the last round holds
-    // one or two pairs and the lines fill four tiles, its (pair, line tile) products are split over
-    // the four waves instead (two tiles of one of two pairs, or one tile of the single pair), so no
-    // wave multiplies a pair slot past the set's units: the 10 pairs of 20 volumes then cost 10
-    // pairs' MFMA work, not 12.  Each wave forms A of its last-round pair itself.
-    const int rlast = (nu + 1) / 2 - 4 * (NPF - 1);
-    const bool split_last = nt == 4 && (rlast == 1 || rlast == 2);
-    const int last_pair = 4 * (NPF - 1) + (!split_last ? wave : (rlast == 2 ? (wave >> 1) : 0));
-    const int t_lo = !split_last ? 0 : (rlast == 2 ? 2 * (wave & 1) : wave);
-    const int t_hi = !split_last ? 4 : t_lo + (rlast == 2 ? 2 : 1);
+    // Pairs per round: wave w takes pair w + 4 i in rounds i < NPF - 1.  LAST = the line tiles of
+    // the last round per wave: 4 = the same as the others (pair w + 4 (NPF - 1)); when the last round
+    // holds two pairs (LAST = 2: wave w takes tiles 2 (w & 1), +1 of pair 4 (NPF - 1) + (w >> 1)) or
+    // one (LAST = 1: tile w of it), its (pair, line tile) products are split over the four waves, so
+    // no wave multiplies a pair slot past the set's units: the 10 pairs of 20 volumes cost 10 pairs'
+    // MFMA work, not 12.  Each wave forms A of its last-round pair itself.  The host picks LAST from
+    // the set's size, with 64 padded lines (four tiles).
+    static_assert(LAST == 4 || LAST == 2 || LAST == 1, "tiles of the last round per wave");
+    constexpr int NMAIN = LAST == 4 ? NPF : NPF - 1;   // rounds over all four tiles
+    const int last_pair = 4 * (NPF - 1) + (LAST == 4 ? wave : (LAST == 2 ? (wave >> 1) : 0));
+    const int t_lo = LAST == 4 ? 0 : (LAST == 2 ? 2 * (wave & 1) : wave);
     auto pair_of = [&](int i) { return i < NPF - 1 ? wave + 4 * i : last_pair; };
-    auto tile_on = [&](int i, int t) { return i < NPF - 1 || (t >= t_lo && t < t_hi); };
 
     // this lane's line of the B generation (lane = line; LyP <= 64)
     double t_s = 0.0, kyn = 0.0;
@@ -275,42 +275,65 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
                 }
             }
             // line tile t's B is read while tile t - 1 is multiplied; the three split products run in
-            // phases over all 2 NPF accumulators of the tile, so dependent MFMAs are 2 NPF apart
-            u32x4 bh[2], bl[2], nbh[2], nbl[2];
-#pragma unroll
-            for (int f = 0; f < 2; f++) {
-                nbh[f] = sm.b[q][0][f][0][lane];
-                nbl[f] = sm.b[q][0][f][1][lane];
-            }
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
+            // phases over all 2 NMAIN accumulators of the tile, so dependent MFMAs are 2 NMAIN apart
+            if constexpr (NMAIN > 0) {
+                u32x4 bh[2], bl[2], nbh[2], nbl[2];
 #pragma unroll
                 for (int f = 0; f < 2; f++) {
-                    bh[f] = nbh[f];
-                    bl[f] = nbl[f];
+                    nbh[f] = sm.b[q][0][f][0][lane];
+                    nbl[f] = sm.b[q][0][f][1][lane];
                 }
-                if (t + 1 < 4) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
 #pragma unroll
                     for (int f = 0; f < 2; f++) {
-                        nbh[f] = sm.b[q][t + 1][f][0][lane];
-                        nbl[f] = sm.b[q][t + 1][f][1][lane];
+                        bh[f] = nbh[f];
+                        bl[f] = nbl[f];
                     }
-                }
+                    if (t + 1 < 4) {
 #pragma unroll
-                for (int i = 0; i < NPF; i++)
-                    if (tile_on(i, t))
+                        for (int f = 0; f < 2; f++) {
+                            nbh[f] = sm.b[q][t + 1][f][0][lane];
+                            nbl[f] = sm.b[q][t + 1][f][1][lane];
+                        }
+                    }
+#pragma unroll
+                    for (int i = 0; i < NMAIN; i++)
 #pragma unroll
                         for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(al[i], bh[f], acc[i][t][f]);
 #pragma unroll
-                for (int i = 0; i < NPF; i++)
-                    if (tile_on(i, t))
+                    for (int i = 0; i < NMAIN; i++)
 #pragma unroll
                         for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bl[f], acc[i][t][f]);
 #pragma unroll
-                for (int i = 0; i < NPF; i++)
-                    if (tile_on(i, t))
+                    for (int i = 0; i < NMAIN; i++)
 #pragma unroll
                         for (int f = 0; f < 2; f++) acc[i][t][f] = mfma_bf16(ah[i], bh[f], acc[i][t][f]);
+                }
+            }
+            // the split last round: this wave's LAST tiles t_lo + k of its last-round pair
+            if constexpr (LAST < 4) {
+                constexpr int i = NPF - 1;
+                u32x4 bh[LAST][2], bl[LAST][2];
+#pragma unroll
+                for (int k = 0; k < LAST; k++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        bh[k][f] = sm.b[q][t_lo + k][f][0][lane];
+                        bl[k][f] = sm.b[q][t_lo + k][f][1][lane];
+                    }
+#pragma unroll
+                for (int k = 0; k < LAST; k++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][k][f] = mfma_bf16(al[i], bh[k][f], acc[i][k][f]);
+#pragma unroll
+                for (int k = 0; k < LAST; k++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][k][f] = mfma_bf16(ah[i], bl[k][f], acc[i][k][f]);
+#pragma unroll
+                for (int k = 0; k < LAST; k++)
+#pragma unroll
+                    for (int f = 0; f < 2; f++) acc[i][k][f] = mfma_bf16(ah[i], bh[k][f], acc[i][k][f]);
             }
         }
         // B of the next chunk, while other waves still contract this one
@@ -325,10 +348,13 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
         if (ul >= nu) continue;
         float2* gb = units[set[1 + ul]].gbuf;
         const int c0 = 4 * (lq & 1);
+        const bool split = i == NPF - 1 && LAST < 4;
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            if (t >= nt || !tile_on(i, t)) continue;
-            const f32x4 re = acc[i][t][0], im = acc[i][t][1];
+        for (int k = 0; k < 4; k++) {
+            if (split && k >= LAST) continue;
+            const int t = split ? t_lo + k : k;     // acc[i][k] holds line tile t
+            if (t >= nt) continue;
+            const f32x4 re = acc[i][k][0], im = acc[i][k][1];
             // stored split, the x-stage's B operand: {hi, lo} per complex value at [x][line*C + coil]
             u32x2 s[4];
 #pragma unroll
@@ -350,32 +376,44 @@ void ystage_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict_
 }
 
 // ----------------------------------------------------------------------------- launch
-template <int Q, bool CPLX>
-static hipError_t blaunch_npw(int npf, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
-                              const WorkItem* w, const int32_t* sets, hipStream_t st) {
-    dim3 grid(nwork), block(256);
-    switch (npf) {
-    case 1: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 1>), grid, block, 0, st, u, s, g, w, sets); break;
-    case 2: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 2>), grid, block, 0, st, u, s, g, w, sets); break;
-    case 3: hipLaunchKernelGGL((ystage_basis<Q, CPLX, 3>), grid, block, 0, st, u, s, g, w, sets); break;
+template <int Q, bool CPLX, int NPF>
+static hipError_t blaunch_last(int last, dim3 grid, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                               const WorkItem* w, const int32_t* sets, hipStream_t st) {
+    switch (last) {
+    case 4: hipLaunchKernelGGL((ystage_basis<Q, CPLX, NPF, 4>), grid, dim3(256), 0, st, u, s, g, w, sets); break;
+    case 2: hipLaunchKernelGGL((ystage_basis<Q, CPLX, NPF, 2>), grid, dim3(256), 0, st, u, s, g, w, sets); break;
+    case 1: hipLaunchKernelGGL((ystage_basis<Q, CPLX, NPF, 1>), grid, dim3(256), 0, st, u, s, g, w, sets); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-// npf = unit pairs per wave (a set holds up to 8 * npf units); Q = 1..4 compartments.
-hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int nwork, const DevUnit* u, const DevSlice* s,
+template <int Q, bool CPLX>
+static hipError_t blaunch_npw(int npf, int last, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                              const WorkItem* w, const int32_t* sets, hipStream_t st) {
+    const dim3 grid(nwork);
+    switch (npf) {
+    case 1: return blaunch_last<Q, CPLX, 1>(last, grid, u, s, g, w, sets, st);
+    case 2: return blaunch_last<Q, CPLX, 2>(last, grid, u, s, g, w, sets, st);
+    case 3: return blaunch_last<Q, CPLX, 3>(last, grid, u, s, g, w, sets, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// npf = unit pairs per wave (a set holds up to 8 * npf units); last = line tiles per wave of the
+// last round (4, or 2 / 1 when it holds two / one pairs of a 64-line group); Q = 1..4 compartments.
+hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int last, int nwork, const DevUnit* u, const DevSlice* s,
                                const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
     switch (Q) {
-    case 1: return cplx ? blaunch_npw<1, true>(npw, nwork, u, s, g, w, sets, st)
-                        : blaunch_npw<1, false>(npw, nwork, u, s, g, w, sets, st);
-    case 2: return cplx ? blaunch_npw<2, true>(npw, nwork, u, s, g, w, sets, st)
-                        : blaunch_npw<2, false>(npw, nwork, u, s, g, w, sets, st);
-    case 3: return cplx ? blaunch_npw<3, true>(npw, nwork, u, s, g, w, sets, st)
-                        : blaunch_npw<3, false>(npw, nwork, u, s, g, w, sets, st);
-    case 4: return cplx ? blaunch_npw<4, true>(npw, nwork, u, s, g, w, sets, st)
-                        : blaunch_npw<4, false>(npw, nwork, u, s, g, w, sets, st);
+    case 1: return cplx ? blaunch_npw<1, true>(npw, last, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<1, false>(npw, last, nwork, u, s, g, w, sets, st);
+    case 2: return cplx ? blaunch_npw<2, true>(npw, last, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<2, false>(npw, last, nwork, u, s, g, w, sets, st);
+    case 3: return cplx ? blaunch_npw<3, true>(npw, last, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<3, false>(npw, last, nwork, u, s, g, w, sets, st);
+    case 4: return cplx ? blaunch_npw<4, true>(npw, last, nwork, u, s, g, w, sets, st)
+                        : blaunch_npw<4, false>(npw, last, nwork, u, s, g, w, sets, st);
     default: return hipErrorInvalidValue;
     }
 }
