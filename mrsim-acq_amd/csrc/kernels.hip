@@ -24,15 +24,16 @@ namespace mrs {
 // ----------------------------------------------------------------------------- slice prep
 // Static per-slice planes (src/kspace.rs:678-725, 908-925).  Block = one 32 x 32 voxel tile of one
 // slice: user planes (x fastest) are read coalesced along x, staged in LDS, and written in the
-// kernels' [x][y] order coalesced along y.  grid = (tiles_x * tiles_y, slices).
+// kernels' [x][y] order coalesced along y.  grid = (tiles_x * tiles_y, listed slices): only slices
+// that own a plane set are listed (the others read their owner's planes).
 __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices, const DevGroup* __restrict__ groups,
-                                                   int nslices, int mode) {
+                                                   const int32_t* __restrict__ list, int nlist, int mode) {
     __shared__ double sd[2][32][33];
     __shared__ float sf[1 + 2 * MRS_MAX_COMP][32][33];
     __shared__ float2 sc[32][33];
     __shared__ float samp[32][33];
-    const int s = blockIdx.y;
-    if (s >= nslices) return;
+    if ((int)blockIdx.y >= nlist) return;
+    const int s = list[blockIdx.y];
     DevSlice& sl = slices[s];
     const DevGroup& gr = groups[sl.group];
     const int snx = gr.snx, sny = gr.sny;
@@ -2074,9 +2075,12 @@ __global__ void probe_sincos(const double* __restrict__ cyc, float* __restrict__
 namespace mrs {
 
 // `maxtiles`: the largest number of 32 x 32 tiles of any slice's padded grid.
-hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxtiles, int mode, hipStream_t st) {
-    dim3 grid(maxtiles, nslices);
-    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, nslices, mode);
+// list = the slices to prepare (those that own a plane set: the others read an owner's planes)
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, const int32_t* list, int nlist, int maxtiles, int mode,
+                       hipStream_t st) {
+    if (nlist <= 0) return hipSuccess;
+    dim3 grid(maxtiles, nlist);
+    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
     return hipGetLastError();
 }
 

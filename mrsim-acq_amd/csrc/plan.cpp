@@ -19,7 +19,8 @@
 #include "tables.h"
 
 namespace mrs {
-hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, int nslices, int maxvox, int mode, hipStream_t st);
+hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, const int32_t* list, int nlist, int maxvox, int mode,
+                       hipStream_t st);
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
                               hipStream_t st);
@@ -197,6 +198,16 @@ struct mrs_plan {
     // per-slice coil-plane allocations (s_hi, s_lo, s32); slices with the same maps share one set
     std::vector<std::array<void*, 3>> s_alloc;
     std::vector<size_t> s_owner;   // slice whose coil planes each slice reads
+    std::vector<size_t> p_owner;   // slice whose static planes each slice reads
+    // The slices the prep runs on (owners of a coil-plane or static-plane set), all of them
+    // (prep_all) and per unit chunk (the owners a chunk's units read that no earlier chunk's list
+    // holds, prep_off[k]..prep_off[k+1] of prep_chunks); prep_wait[k][j]: chunk k reads planes that
+    // chunk j < k prepares.
+    std::vector<int32_t> prep_all, prep_chunks;
+    std::vector<size_t> prep_off;
+    std::vector<std::vector<uint8_t>> prep_wait;
+    int32_t* d_prep = nullptr;     // prep_all, then prep_chunks
+    std::vector<hipEvent_t> ev_prepk;
     // per-slice allocations of the static planes (base, rate, poly, nr2, nri, rot, rq); slices whose
     // prep would write identical ones (the echoes of one partition) share one set
     std::vector<std::array<void*, 7>> p_alloc;
@@ -222,9 +233,11 @@ static void free_plan_memory(mrs_plan* p) {
     if (p->graph) hipGraphDestroy(p->graph);
     for (hipStream_t s : p->streams) hipStreamDestroy(s);
     for (hipEvent_t e : p->ev_chunk) hipEventDestroy(e);
+    for (hipEvent_t e : p->ev_prepk) hipEventDestroy(e);
     if (p->ev_prep) hipEventDestroy(p->ev_prep);
     p->streams.clear();
     p->ev_chunk.clear();
+    p->ev_prepk.clear();
     p->ev_prep = nullptr;
     for (void* a : p->allocations) hipFree(a);
     p->allocations.clear();
@@ -563,6 +576,8 @@ static void share_static_planes(mrs_plan* p) {
         const size_t o = it == owner.end() ? s : it->second;
         if (it == owner.end()) owner.emplace(std::move(key), s);
         sl.p_owner = o == s;
+        p->p_owner.resize(p->slices.size());
+        p->p_owner[s] = o;
         const auto& a = p->p_alloc[o];
         sl.base = (double*)a[0];
         sl.rate = (float*)a[1];
@@ -574,6 +589,43 @@ static void share_static_planes(mrs_plan* p) {
     }
 }
 
+// The prep lists of the current plane ownership and unit chunks (see mrs_plan::prep_all).
+static void build_prep_lists(mrs_plan* p) {
+    const size_t ns = p->slices.size();
+    p->prep_all.clear();
+    for (size_t s = 0; s < ns; s++)
+        if (p->slices[s].s_owner || p->slices[s].p_owner) p->prep_all.push_back((int32_t)s);
+    p->prep_chunks.clear();
+    p->prep_off.assign(1, 0);
+    p->prep_wait.assign(p->nchunks, std::vector<uint8_t>(p->nchunks, 0));
+    if (p->nchunks <= 1 || p->unit_chunk.size() != p->units.size()) return;
+    std::vector<int> first(ns, -1);   // the chunk whose list prepares each owner
+    for (int k = 0; k < p->nchunks; k++) {
+        for (size_t u = 0; u < p->units.size(); u++) {
+            if (p->unit_chunk[u] != k) continue;
+            const size_t s = p->units[u].slice;
+            for (size_t o : {p->s_owner[s], p->p_owner[s]}) {
+                if (first[o] < 0) {
+                    first[o] = k;
+                    p->prep_chunks.push_back((int32_t)o);
+                } else if (first[o] < k) {
+                    p->prep_wait[k][first[o]] = 1;
+                }
+            }
+        }
+        p->prep_off.push_back(p->prep_chunks.size());
+    }
+}
+
+static hipError_t upload_prep(mrs_plan* p) {
+    build_prep_lists(p);
+    hipError_t e = hipMemcpy(p->d_prep, p->prep_all.data(), p->prep_all.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !p->prep_chunks.empty())
+        e = hipMemcpy(p->d_prep + p->prep_all.size(), p->prep_chunks.data(), p->prep_chunks.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice);
+    return e;
+}
+
 static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     share_coil_planes(p);
     share_static_planes(p);
@@ -581,6 +633,7 @@ static int upload_descs(mrs_plan* p, char* err, size_t errlen) {
     for (DevSlice& sl : p->slices) sl.prep_extra = 0;
     for (size_t s = 0; s < p->slices.size(); s++)
         if (p->basis && p->basis_slice[s]) p->slices[p->s_owner[s]].prep_extra |= PREP_S32;
+    HIP_TRY(upload_prep(p));
     HIP_TRY(hipMemcpy(p->d_slices, p->slices.data(), p->slices.size() * sizeof(DevSlice), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_units, p->units.data(), p->units.size() * sizeof(DevUnit), hipMemcpyHostToDevice));
     return MRS_OK;
@@ -740,6 +793,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
     if (rc == MRS_OK) {
         rc = plan_alloc(p, &p->d_groups, ngroups, err, errlen);
         if (!rc) rc = plan_alloc(p, &p->d_slices, nslices, err, errlen);
+        if (!rc) rc = plan_alloc(p, &p->d_prep, 2 * nslices, err, errlen);
         if (!rc) rc = plan_alloc(p, &p->d_units, nunits, err, errlen);
         if (!rc) rc = plan_alloc(p, &p->d_rows, 256 * ngroups, err, errlen);
     }
@@ -838,14 +892,18 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
     if (rc == MRS_OK && p->nchunks > 1) {
         p->streams.resize(p->nchunks);
         p->ev_chunk.resize(p->nchunks);
+        p->ev_prepk.resize(p->nchunks);
         for (int k = 0; k < p->nchunks && rc == MRS_OK; k++) {
             if (hipStreamCreateWithFlags(&p->streams[k], hipStreamNonBlocking) != hipSuccess ||
-                hipEventCreateWithFlags(&p->ev_chunk[k], hipEventDisableTiming) != hipSuccess)
+                hipEventCreateWithFlags(&p->ev_chunk[k], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&p->ev_prepk[k], hipEventDisableTiming) != hipSuccess)
                 rc = fail(err, errlen, MRS_E_DEVICE, "stream/event creation failed");
         }
         if (rc == MRS_OK && hipEventCreateWithFlags(&p->ev_prep, hipEventDisableTiming) != hipSuccess)
             rc = fail(err, errlen, MRS_E_DEVICE, "event creation failed");
     }
+    // the per-chunk prep lists need the unit chunks, assigned after the first descriptor upload
+    if (rc == MRS_OK && upload_prep(p) != hipSuccess) rc = fail(err, errlen, MRS_E_DEVICE, "upload of prep lists failed");
     if (rc) { free_plan_memory(p); delete p; return rc; }
     *out = p;
     return MRS_OK;
@@ -1038,15 +1096,30 @@ static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t e
 }
 
 static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen) {
-    HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, fwd_prep_mode(p), st));
+    const bool chunked = p->precision == MRS_PREC_SPLIT_BF16 && p->nchunks > 1;
+    if (!chunked)
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
+                            fwd_prep_mode(p), st));
     if (p->precision == MRS_PREC_SPLIT_BF16) {
         if (p->nchunks == 1) {
             int rc = enqueue_chunk(p, 0, st, err, errlen);
             if (rc) return rc;
         } else {
+            // Each chunk's stream prepares the planes its units read first (the owners no earlier
+            // chunk prepares), then waits for the earlier chunks' preps it reads from, then runs its
+            // stages: the second chunk's prep overlaps the first chunk's y-stage.
             HIP_TRY(hipEventRecord(p->ev_prep, st));
+            const int32_t* dl = p->d_prep + p->prep_all.size();
             for (int k = 0; k < p->nchunks; k++) {
                 HIP_TRY(hipStreamWaitEvent(p->streams[k], p->ev_prep, 0));
+                HIP_TRY(launch_prep(p->d_slices, p->d_groups, dl + p->prep_off[k],
+                                    (int)(p->prep_off[k + 1] - p->prep_off[k]), p->maxvox, fwd_prep_mode(p),
+                                    p->streams[k]));
+                HIP_TRY(hipEventRecord(p->ev_prepk[k], p->streams[k]));
+            }
+            for (int k = 0; k < p->nchunks; k++) {
+                for (int j = 0; j < k; j++)
+                    if (p->prep_wait[k][j]) HIP_TRY(hipStreamWaitEvent(p->streams[k], p->ev_prepk[j], 0));
                 int rc = enqueue_chunk(p, k, p->streams[k], err, errlen);
                 if (rc) return rc;
                 HIP_TRY(hipEventRecord(p->ev_chunk[k], p->streams[k]));
@@ -1215,7 +1288,8 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
     const bool mfma = p->precision == MRS_PREC_SPLIT_BF16;
     static_assert(PREP_SPLIT != PREP_S32, "plane forms");
     if (!(reuse && p->planes_current && fwd_prep_mode(p) == (mfma ? PREP_SPLIT : PREP_S32)))
-        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, mfma ? PREP_SPLIT : PREP_S32,
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
+                            mfma ? PREP_SPLIT : PREP_S32,
                             st));
     for (const auto& run : p->vjp_runs) {
         const DevGroup& dg = p->groups[p->units[p->vjp_order[run.first]].group].dev;
@@ -1252,7 +1326,8 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
     int rc = MRS_OK;
     for (int it = 0; it < iters && rc == MRS_OK; it++) {
         HIP_TRY(hipEventRecord(ev[0], st));
-        HIP_TRY(launch_prep(p->d_slices, p->d_groups, (int)p->slices.size(), p->maxvox, fwd_prep_mode(p), st));
+        HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
+                            fwd_prep_mode(p), st));
         HIP_TRY(hipEventRecord(ev[1], st));
         if (p->precision == MRS_PREC_SPLIT_BF16) {
             // every chunk's y-stage, then every chunk's x-stage, serially on `stream`
