@@ -241,6 +241,39 @@ def nufft_pass(iters: int):
             "note": "FP64 NUFFT (tol 1e-13, width 14) vs split-BF16 exact; drop-ins include host transfers"}
 
 
+def grad_batch_pass(plan, w, outs, iters: int):
+    """The headline batch (600 units) forward and forward + adjoint: gradients for every unit's images,
+    fieldmap and T2' from a seeded complex cotangent, the adjoint reusing the forward's slice planes
+    (mrs_plan_vjp_after_forward).  Both timed as whole-batch passes."""
+    import torch
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    gout = [torch.randn(o.shape, dtype=o.dtype, device="cuda", generator=gen) for o in outs]
+    P, Q = w.snx * w.sny, w.n_comp
+    gi = [torch.empty((Q, P), dtype=torch.float32, device="cuda") for _ in outs]
+    gf = [torch.empty(P, dtype=torch.float32, device="cuda") for _ in outs]
+    gt = [torch.empty((Q, P), dtype=torch.float32, device="cuda") for _ in outs]
+    arrays = plan.pointer_arrays(gout, gi, gf, gt)
+    for _ in range(2):
+        plan.forward_graph()
+        plan.vjp_arrays(arrays, after_forward=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        plan.forward_graph()
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / iters
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        plan.forward_graph()
+        plan.vjp_arrays(arrays, after_forward=True)
+    torch.cuda.synchronize()
+    t_both = (time.perf_counter() - t0) / iters
+    return {"workload": f"the forward's batch: {len(outs)} units 128^2->64^2, 8 coils, 3 compartments, real images; "
+                        "gradients for images, fieldmap and T2' per unit",
+            "forward_ms": round(1e3 * t_fwd, 4), "forward_adjoint_ms": round(1e3 * t_both, 4),
+            "ratio": round(t_both / t_fwd, 3)}
+
+
 def grad_pass(iters: int):
     """One slice, 128^2 -> 64^2, 8 coils, 3 compartments, complex64 images: forward time, forward +
     adjoint (gradients for images, fieldmap and T2') time, and a timed 50-iteration Adam fit of the
@@ -588,6 +621,9 @@ def main():
     log(f"reconstruction: {recon}")
 
     grad = None if args.no_grad else grad_pass(max(5, args.steps))
+    grad_batch = None if args.no_grad else grad_batch_pass(plan, w, outs, max(5, args.steps // 2))
+    if grad_batch:
+        log(f"batched gradient pass: {grad_batch}")
     nufft = None if args.no_nufft else nufft_pass(max(5, args.steps))
     if nufft:
         log(f"NUFFT comparator: {nufft}")
@@ -688,6 +724,7 @@ def main():
         "tiles": plan.tiles(),
         "basis": plan.basis()[0],
         "grad_pass": grad,
+        "grad_pass_batch": grad_batch,
         "nufft_comparator": nufft,
         "reconstruction": recon,
         "ge3d": ge3d,
