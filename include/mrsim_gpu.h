@@ -256,6 +256,15 @@ int mrs_plan_get_tiles(const mrs_plan* plan, int32_t* x_tile, int32_t* line_tile
 int mrs_plan_set_basis(mrs_plan* plan, int32_t on, char* err, size_t errlen);
 int mrs_plan_get_basis(const mrs_plan* plan, int32_t* on, int32_t* eligible_slices);
 
+/* Slice-basis y-adjoint of mrs_plan_vjp on (1, the default) or off (0).  With it on, the units of a
+ * slice that qualifies for the slice-basis y-stage (same conditions) take their y-adjoint in one
+ * kernel per (set of up to 20 units, column) that generates the slice's per-line phase and decay
+ * factors once for all of them and contracts every unit's x-adjoint result against them over the
+ * lines on MFMA.  Split-BF16 plans only; other units keep the per-unit y-adjoint.  No counterpart in
+ * the reference (which has no adjoint): a planner control of this build's vector-Jacobian product. */
+int mrs_plan_set_vjp_basis(mrs_plan* plan, int32_t on, char* err, size_t errlen);
+int mrs_plan_get_vjp_basis(const mrs_plan* plan, int32_t* on);
+
 /* ---- 3D gradient-echo train (stack of EPI blocks over nz partitions) ----------------------- */
 
 /* Within-echo timing of a GRASE/3D-GE block on ny lines: t_ms[ky] from the echo centre and the

@@ -2390,9 +2390,10 @@ hipError_t set_yadjoint_lds_limit(size_t bytes) {
 // Adjoint of units [unit0, unit0 + n) sharing one group shape.  mfma: the x-adjoint and the
 // y-adjoint's coil contraction as split-BF16 MFMA (the y-adjoint needs the split coil planes), else
 // FP32 on the VALU (the y-adjoint reads the FP32 planes).
+// yadj: also launch the per-unit y-adjoint (off where the slice-basis y-adjoint takes the units).
 hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy, int snx, int sny, int nx, int LyP,
-                            int ntab, bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
-                            hipStream_t st) {
+                            int ntab, bool mfma, bool yadj, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                            const VjpUnit* v, hipStream_t st) {
     // MFMA x-adjoint tiling: 16*MT x rows per block with their twiddle rows within 64 KiB of LDS, the
     // unit's column tiles split over enough parts for about 1024 blocks per launch
     const int nkP = (nx + 15) & ~15;
@@ -2419,6 +2420,7 @@ hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        if (!yadj) continue;
         if (mfma) {
             dim3 grid(snx, m);
             if (eddy)

@@ -31,8 +31,11 @@ hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int last, int nwork, c
 hipError_t set_xstage_lds_limit(size_t bytes);
 hipError_t set_yadjoint_lds_limit(size_t bytes);
 hipError_t launch_vjp_units(int unit0, int n, int Q, int C, bool cplx, bool eddy, int snx, int sny, int nx, int LyP,
-                            int ntab, bool mfma, const DevUnit* u, const DevSlice* s, const DevGroup* g, const VjpUnit* v,
-                            hipStream_t st);
+                            int ntab, bool mfma, bool yadj, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                            const VjpUnit* v, hipStream_t st);
+hipError_t launch_yadjoint_basis(int Q, bool cplx, int npf, int nwork, const DevUnit* u, const DevSlice* s,
+                                 const DevGroup* g, const VjpUnit* v, const WorkItem* w, const int32_t* sets,
+                                 hipStream_t st);
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,
                              const DevGroup* g, hipStream_t st);
 hipError_t launch_zero_rows(int unit0, int nunits, const DevUnit* u, const DevGroup* g, hipStream_t st);
@@ -190,7 +193,21 @@ struct mrs_plan {
     // column x meet on one XCD and share the coil planes in L2.  The adjoint's descriptor slots hold
     // the units in this order; vjp_runs are its launches.
     std::vector<size_t> vjp_order;
-    std::vector<std::pair<size_t, size_t>> vjp_runs;
+    struct VjpRun {
+        size_t first, second;
+        bool basis;            // the run's y-adjoint is the slice-basis kernel over its sets
+        size_t boff[4];        // its (set, column) items per pairs-per-wave count npf: [boff[npf-1], boff[npf])
+    };
+    std::vector<VjpRun> vjp_runs;
+    // slice-basis y-adjoint (on by default): the units of a slice that qualifies for the forward's slice
+    // basis share one adjoint kernel; sets of up to kBasisMax units in vjp-order indices
+    int vjp_basis = 1;
+    bool vjp_layout_ok = false;
+    std::vector<int32_t> vjp_bsets;
+    std::vector<WorkItem> vjp_bitems;
+    int32_t* d_vjp_bsets = nullptr;
+    WorkItem* d_vjp_bitems = nullptr;
+    size_t cap_vjp_bsets = 0, cap_vjp_bitems = 0;
     // the adjoint's descriptors (units in vjp order, then their VjpUnit records) and what the current
     // ring slot holds
     DescRing vjp_ring;
@@ -1197,6 +1214,97 @@ extern "C" int mrs_plan_forward_graph(mrs_plan* p, void* stream, char* err, size
     return MRS_OK;
 }
 
+// The adjoint's launch layout.  Units sort by launch shape (with whether the unit runs in the
+// slice-basis y-adjoint), then by coil-plane set and slice, so that the units reading one coil-plane
+// set sit side by side (their blocks for a column x meet on one XCD and share the planes in L2) and a
+// slice's volumes are contiguous.  Units of slices that qualify for the slice basis (split-BF16 plans,
+// the switch on) go to sets of up to kBasisMax units of one slice, near-equal in size, whose
+// (set, column) items run in the slice-basis y-adjoint after the run's x-adjoint.
+static int vjp_layout(mrs_plan* p, char* err, size_t errlen) {
+    const size_t n = p->units.size();
+    std::vector<uint8_t> ub(n, 0);
+    if (p->vjp_basis && p->precision == MRS_PREC_SPLIT_BF16)
+        for (size_t u = 0; u < n; u++) ub[u] = p->basis_slice[p->units[u].slice];
+    auto shape = [&](size_t u) {
+        const DevGroup& dg = p->groups[p->units[u].group].dev;
+        return std::make_tuple(dg.Q, dg.C, dg.cplx, dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP,
+                               dg.t_hi[1] != dg.t_hi[0], (int)ub[u]);
+    };
+    p->vjp_order.resize(n);
+    for (size_t u = 0; u < n; u++) p->vjp_order[u] = u;
+    std::stable_sort(p->vjp_order.begin(), p->vjp_order.end(), [&](size_t a, size_t b) {
+        const auto ka = shape(a), kb = shape(b);
+        if (ka != kb) return ka < kb;
+        const size_t oa = p->s_owner[p->units[a].slice], ob = p->s_owner[p->units[b].slice];
+        if (oa != ob) return oa < ob;
+        return p->units[a].slice < p->units[b].slice;
+    });
+    p->vjp_runs.clear();
+    p->vjp_bsets.clear();
+    p->vjp_bitems.clear();
+    for (size_t k = 0; k < n;) {
+        size_t e = k + 1;
+        while (e < n && shape(p->vjp_order[e]) == shape(p->vjp_order[k])) e++;
+        mrs_plan::VjpRun run{k, e, ub[p->vjp_order[k]] != 0, {}};
+        if (run.basis) {
+            std::vector<WorkItem> per[3];
+            const DevGroup& dg = p->groups[p->units[p->vjp_order[k]].group].dev;
+            const int snx = dg.snx;
+            // four compartments: sets of up to 16 units (two pairs per wave fit the registers)
+            const size_t smax = dg.Q == 4 ? 16 : kBasisMax;
+            for (size_t a = k; a < e;) {
+                size_t b = a;
+                while (b < e && p->units[p->vjp_order[b]].slice == p->units[p->vjp_order[a]].slice) b++;
+                const size_t cnt_all = b - a, nsets = (cnt_all + smax - 1) / smax;
+                size_t m = a;
+                for (size_t si = 0; si < nsets; si++) {
+                    const size_t cnt = cnt_all / nsets + (si < cnt_all % nsets ? 1 : 0);
+                    const int npf = (int)(((cnt + 1) / 2 + 3) / 4);   // unit pairs per wave
+                    const int32_t sidx = (int32_t)(p->vjp_bsets.size() / kBasisStride);
+                    p->vjp_bsets.push_back((int32_t)cnt);
+                    for (int v = 0; v < kBasisMax; v++)
+                        p->vjp_bsets.push_back((int32_t)(m + std::min<size_t>(v, cnt - 1)));
+                    m += cnt;
+                    for (int x = 0; x < snx; x++) per[npf - 1].push_back({sidx, x});
+                }
+                a = b;
+            }
+            // each XCD takes a contiguous run of a set's columns: the blocks writing one cache line of
+            // an output row (consecutive x) share that XCD's L2, as do their image reads
+            run.boff[0] = p->vjp_bitems.size();
+            for (int f = 0; f < 3; f++) {
+                const std::vector<WorkItem> o = xcd_order(per[f]);
+                p->vjp_bitems.insert(p->vjp_bitems.end(), o.begin(), o.end());
+                run.boff[f + 1] = p->vjp_bitems.size();
+            }
+        }
+        p->vjp_runs.push_back(run);
+        k = e;
+    }
+    // device copies; launches of an earlier layout may still read the old arrays
+    if (p->vjp_bsets.size() > p->cap_vjp_bsets || p->vjp_bitems.size() > p->cap_vjp_bitems) HIP_TRY(hipDeviceSynchronize());
+    if (p->vjp_bsets.size() > p->cap_vjp_bsets) {
+        int rc = plan_alloc(p, &p->d_vjp_bsets, p->vjp_bsets.size(), err, errlen);
+        if (rc) return rc;
+        p->cap_vjp_bsets = p->vjp_bsets.size();
+    }
+    if (p->vjp_bitems.size() > p->cap_vjp_bitems) {
+        int rc = plan_alloc(p, &p->d_vjp_bitems, p->vjp_bitems.size(), err, errlen);
+        if (rc) return rc;
+        p->cap_vjp_bitems = p->vjp_bitems.size();
+    }
+    if (!p->vjp_bsets.empty()) {
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(p->d_vjp_bsets, p->vjp_bsets.data(), p->vjp_bsets.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(p->d_vjp_bitems, p->vjp_bitems.data(), p->vjp_bitems.size() * sizeof(WorkItem),
+                          hipMemcpyHostToDevice));
+    }
+    p->vjp_layout_ok = true;
+    p->vjp_fresh = true;   // the descriptor order may have changed
+    return MRS_OK;
+}
+
 // reuse: skip the slice prep when a forward built the planes from the inputs as bound now (the
 // forward and the adjoint read the same plane forms: split coil planes for split-BF16 plans, FP32
 // ones otherwise)
@@ -1225,26 +1333,11 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
             if (rc) { p->vjp_units.clear(); return rc; }
         }
         p->vjp_fresh = true;
-        // launch shape, then coil-plane set, then unit
-        auto shape = [&](size_t u) {
-            const DevGroup& dg = p->groups[p->units[u].group].dev;
-            return std::make_tuple(dg.Q, dg.C, dg.cplx, dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP,
-                                   dg.t_hi[1] != dg.t_hi[0]);
-        };
-        p->vjp_order.resize(n);
-        for (size_t u = 0; u < n; u++) p->vjp_order[u] = u;
-        std::stable_sort(p->vjp_order.begin(), p->vjp_order.end(), [&](size_t a, size_t b) {
-            const auto ka = shape(a), kb = shape(b);
-            if (ka != kb) return ka < kb;
-            return p->s_owner[p->units[a].slice] < p->s_owner[p->units[b].slice];
-        });
-        p->vjp_runs.clear();
-        for (size_t k = 0; k < n;) {
-            size_t e = k + 1;
-            while (e < n && shape(p->vjp_order[e]) == shape(p->vjp_order[k])) e++;
-            p->vjp_runs.push_back({k, e});
-            k = e;
-        }
+        p->vjp_layout_ok = false;
+    }
+    if (!p->vjp_layout_ok) {
+        int rc = vjp_layout(p, err, errlen);
+        if (rc) return rc;
     }
     // The adjoint reads a unit's images, slice and group (a new output pointer, which each forward of
     // the autograd wrapper binds, changes nothing it reads) and the unit's cotangent and gradient
@@ -1295,7 +1388,12 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
         const DevGroup& dg = p->groups[p->units[p->vjp_order[run.first]].group].dev;
         HIP_TRY(launch_vjp_units((int)run.first, (int)(run.second - run.first), dg.Q, dg.C, dg.cplx != 0,
                                  dg.eddy_strength != 0.0, dg.snx, dg.sny, dg.nx, dg.LyP, dg.t_hi[1] != dg.t_hi[0] ? 2 : 1,
-                                 mfma, d_units_vjp, p->d_slices, p->d_groups, d_vjp_units, st));
+                                 mfma, !run.basis, d_units_vjp, p->d_slices, p->d_groups, d_vjp_units, st));
+        if (run.basis)
+            for (int f = 0; f < 3; f++)
+                HIP_TRY(launch_yadjoint_basis(dg.Q, dg.cplx != 0, f + 1, (int)(run.boff[f + 1] - run.boff[f]), d_units_vjp,
+                                              p->d_slices, p->d_groups, d_vjp_units, p->d_vjp_bitems + run.boff[f],
+                                              p->d_vjp_bsets, st));
     }
     HIP_TRY(hipEventRecord(R.ev[R.cur], st));
     R.armed[R.cur] = true;
@@ -1510,6 +1608,22 @@ extern "C" int mrs_plan_get_basis(const mrs_plan* p, int32_t* on, int32_t* slice
         for (uint8_t b : p->basis_slice) n += b ? 1 : 0;
         *slices = n;
     }
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_set_vjp_basis(mrs_plan* p, int32_t on, char* err, size_t errlen) {
+    if (!p) return fail(err, errlen, MRS_E_INVALID, "null plan");
+    if (on != 0 && on != 1) return fail(err, errlen, MRS_E_INVALID, "slice-basis adjoint: 0 or 1");
+    if (on != p->vjp_basis) {
+        p->vjp_basis = on;
+        p->vjp_layout_ok = false;
+    }
+    return MRS_OK;
+}
+
+extern "C" int mrs_plan_get_vjp_basis(const mrs_plan* p, int32_t* on) {
+    if (!p || !on) return MRS_E_INVALID;
+    *on = p->vjp_basis;
     return MRS_OK;
 }
 

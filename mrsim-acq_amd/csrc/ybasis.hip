@@ -418,4 +418,377 @@ hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int last, int nwork, c
     }
 }
 
+
+// ============================================================================= adjoint
+// The y-adjoint of the volumes of a slice, evaluated together (the per-unit kernels are yadjoint and
+// yadjoint_mfma in kernels.hip; the reference has no adjoint: this is the transpose of the y-sum of
+// build_coil_kspace_timed, src/kspace.rs:940-975).
+//
+// With the x-adjoint's result Gb_{u,c}(x, l) of unit u, coil c, line l, and the unit-independent
+//   W_q(y, l) = D_q(y, l) conj(P(y, l)),  D_q = 2^{trf_l nr2_q + tau_l nri_q},  P = cis(2 pi cyc),
+// every sum the adjoint needs is a coil contraction of one product per line weight w^k:
+//   R^k_{u,c}(q, y) = sum_l Gb_{u,c}(l) w^k_l W_q(y, l)      (w^0 = 1, w^1 = tau_l, w^2 = tb0_l)
+//   S^k_{u,q}(y)    = sum_c conj(S_c(y)) R^k_{u,c}(q, y)
+// S^0 is the image gradient, Re(conj(m_q) S^1)/T2'^2 the T2' gradient and 2 pi Im(sum_q conj(m_q) S^2)
+// the fieldmap gradient.  R^k is a product of Gb [(unit, coil), line] and W^k [line, y] on
+// v_mfma_f32_16x16x32_bf16 (split BF16, three products, FP32 accumulation); W^k is generated once
+// per (slice, column, chunk of 16 y) for all the set's units, so the phase and the decays of a voxel
+// and line are evaluated once per slice instead of once per unit.  The coil contraction runs on the
+// VALU with the split coil planes (S = hi + lo).
+//
+// Tile layout.  A 16-row tile holds two units x 8 coils (row = 8 * unit of the pair + coil); K =
+// (line, re/im) with 16 lines per step; N = 16 y of the chunk.  The complex product rides on B:
+//   B_re[(l, re), y] = Re W^k,  B_re[(l, im), y] = -Im W^k   ->  Re R
+//   B_im[(l, re), y] = Im W^k,  B_im[(l, im), y] =  Re W^k   ->  Im R
+// Work.  Block = 4 waves = one (set, column x).  Wave w holds A (the split Gb) of the pairs
+// w, w + 4, w + 8 (NPF of them) for all lines in registers; per chunk every thread generates W of one
+// y and four lines, and for each needed kind writes W^k to LDS (both forms, hi and lo), after which
+// the waves contract their pairs against it and fold the result into per-lane sums of the lane's
+// unit, y and four coils.  A lane's accumulator rows are four coils of one unit; the lane 16 apart
+// holds the other four, and one exchange completes the coil sum.
+
+template <int Q>
+struct AdjBasisSmem {
+    u32x4 w[Q][4][2][2][64];   // [q][line step][form re/im][hi/lo][lane]: W^k of the chunk, B layout
+    double base[kBasisMaxY];   // the column's static planes, y = 0 .. snyP-1
+    float rate[kBasisMaxY];
+    float nr2[Q][kBasisMaxY];
+    float nri[Q][kBasisMaxY];
+    LineRec lines[64];
+};
+
+template <int Q, bool CPLX, int NPF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restrict__ slices,
+                    const DevGroup* __restrict__ groups, const VjpUnit* __restrict__ vj,
+                    const WorkItem* __restrict__ work, const int32_t* __restrict__ sets) {
+    constexpr int CM = CPLX ? 2 : 1;   // parts kept per sum: real images need one of each
+    __shared__ __attribute__((aligned(16))) AdjBasisSmem<Q> sm;
+
+    const WorkItem wi = work[blockIdx.x];
+    const int32_t* set = sets + (size_t)wi.unit * kBasisStride;
+    const int nu = set[0];
+    const int x = wi.tile;
+    const DevUnit& u0 = units[set[1]];
+    const DevGroup& gr = groups[u0.group];
+    const DevSlice& sl = slices[u0.slice];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lq = lane >> 4, ln = lane & 15;
+    const int snx = gr.snx, sny = gr.sny;
+    const int snyP = (sny + 15) & ~15, snxP = (snx + 15) & ~15;
+    const int C = gr.C, LyP = gr.LyP, nks = LyP >> 4;
+    const int nch = snyP >> 4;
+    const uint32_t P = (uint32_t)(snx * sny);
+    const int np = (nu + 1) >> 1;
+    // this wave's last-round pair exists (the earlier rounds always do: the host gives a set of n
+    // units the variant with NPF = ceil(ceil(n / 2) / 4))
+    const bool last_ok = wave + 4 * (NPF - 1) < np;
+    // the two units of each of this wave's pairs (wave-uniform: their descriptors come through scalar
+    // loads, and each lane selects its unit's, lq >> 1); pairs past the set alias the last unit
+    int uidx_a[NPF], uidx_b[NPF];
+#pragma unroll
+    for (int i = 0; i < NPF; i++) {
+        const int p = wave + 4 * i;
+        uidx_a[i] = set[1 + min(2 * p, nu - 1)];
+        uidx_b[i] = set[1 + min(2 * p + 1, nu - 1)];
+    }
+
+    // the sums the set's units ask for (the same in every thread: block-uniform branches)
+    bool need1 = false, need2 = false;
+    for (int k = 0; k < nu; k++) {
+        const VjpUnit& v = vj[set[1 + k]];
+        need1 = need1 || v.gti != nullptr;
+        need2 = need2 || v.gfmap != nullptr;
+    }
+    need1 = need1 && gr.do_relax;
+    need2 = need2 && gr.do_dist;
+
+    // the column's static planes and the line table into LDS, once per block
+    {
+        const uint32_t o = (uint32_t)(x * snyP), PPl = (uint32_t)(snxP * snyP);
+        for (int i = tid; i < snyP; i += 256) {
+            sm.base[i] = gld(sl.base)[o + i];
+            sm.rate[i] = gld(sl.rate)[o + i];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                sm.nr2[q][i] = gld(sl.nr2)[q * PPl + o + i];
+                sm.nri[q][i] = gld(sl.nri)[q * PPl + o + i];
+            }
+        }
+        for (int i = tid; i < LyP; i += 256) sm.lines[i] = gr.lines[i];
+    }
+
+    // A: the split Gb of this wave's pairs, lines 16 ks + 4 lq .. +3 of row ln (unit ln >> 3 of the
+    // pair, coil ln & 7); zero for rows past the set's units or C and for steps past LyP
+    u32x4 ah[NPF][4], al[NPF][4];
+#pragma unroll
+    for (int i = 0; i < NPF; i++) {
+        const int p = wave + 4 * i;
+        const int ul = 2 * p + (ln >> 3), c = ln & 7;
+        const bool ok = p < np && ul < nu && c < C;
+        const GAS float2* gb = gld(vj[set[1 + (ok ? ul : 0)]].gbar) + ((size_t)x * LyP) * C + (ok ? c : 0);
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int l = ks * 16 + 4 * lq + j;
+                float2 g = make_float2(0.f, 0.f);
+                if (ok && ks < nks) {
+                    const GAS float2& v = gb[(size_t)l * C];
+                    g.x = v.x;
+                    g.y = v.y;
+                }
+                uint32_t hh, ll;
+                split2(g.x, g.y, hh, ll);
+                ah[i][ks][j] = hh;
+                al[i][ks][j] = ll;
+            }
+    }
+    __syncthreads();
+
+    // W generation: this thread's y = 16 ch + yl and lines 4 g4 .. 4 g4 + 3 (one B-operand slot)
+    const int yl = tid & 15, g4 = tid >> 4;
+    const int wks = g4 >> 2, wslot = (g4 & 3) * 16 + yl;
+    const bool wact = 4 * g4 < LyP;
+    const double sys_yoff = gr.sys_yoff;
+    // coil planes of this lane's four coils 4 (lq & 1) .. +3 (split, [x][c][y], c padded to 8)
+    const int c0 = 4 * (lq & 1);
+
+    for (int ch = 0; ch < nch; ch++) {
+        // D cos, D sin of the thread's (y, lines): W = D (cos - i sin)
+        float pr[Q][4], ps[Q][4];
+        {
+            const int y = ch * 16 + yl;
+            const double bb = sm.base[y];
+            const double rr = (double)sm.rate[y];
+            const double yy = (double)y - sys_yoff;
+            float a2[Q], ai2[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                a2[q] = sm.nr2[q][y];
+                ai2[q] = sm.nri[q][y];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const LineRec& L = sm.lines[wact ? 4 * g4 + j : 0];
+                double cyc = __builtin_fma(L.kyn, yy, bb);
+                cyc = __builtin_fma(rr, L.t_s, cyc);
+                float sn, cs;
+                sincos_turns(cyc, sn, cs);
+                const bool ok = wact && L.ky >= 0;
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    float d = __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, a2[q], L.tau * ai2[q]));
+                    d = ok ? d : 0.0f;
+                    pr[q][j] = d * cs;
+                    ps[q][j] = d * sn;
+                }
+            }
+        }
+        // conj(S_c) of the lane's coils at y = 16 ch + ln (zero rows past sny and coils past C)
+        float sr[4], si[4];
+        {
+            const int y = ch * 16 + ln;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const size_t so = ((size_t)x * 8 + c0 + r) * snyP + y;
+                const uint32_t h = gld(sl.s_hi)[so], lo = gld(sl.s_lo)[so];
+                sr[r] = bf16_float(h & 0xffffu) + bf16_float(lo & 0xffffu);
+                si[r] = __builtin_bit_cast(float, h & 0xffff0000u) + __builtin_bit_cast(float, lo & 0xffff0000u);
+            }
+        }
+        // the output role of this lane: unit 2 p + (lq >> 1) of pair p at y = 16 ch + ln, written by the
+        // lanes with (lq & 1) == 0 once the coil sum is complete; its images and T2' (ms) at the voxel
+        const int y = ch * 16 + ln;
+        const bool wr_lane = (lq & 1) == 0 && y < sny;
+        const uint32_t iv = (uint32_t)x + (uint32_t)snx * (uint32_t)min(y, sny - 1);
+        float m_r[NPF][Q], m_i[NPF][Q], fm[NPF][CM];
+#pragma unroll
+        for (int i = 0; i < NPF; i++) {
+            const int p = wave + 4 * i, ul = 2 * p + (lq >> 1);
+            const bool ok = p < np && ul < nu;
+            const GAS float* img = gld((lq >> 1) ? (const float*)units[uidx_b[i]].images : (const float*)units[uidx_a[i]].images);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if constexpr (CPLX) {
+                    m_r[i][q] = ok ? img[2 * (q * P + iv)] : 0.0f;
+                    m_i[i][q] = ok ? img[2 * (q * P + iv) + 1] : 0.0f;
+                } else {
+                    m_r[i][q] = ok ? img[q * P + iv] : 0.0f;
+                    m_i[i][q] = 0.0f;
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < CM; m++) fm[i][m] = 0.0f;
+        }
+        float ti_ms[Q];   // T2' (ms) of each compartment at the voxel, for the T2' gradient
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            ti_ms[q] = sl.t_inhom ? gld(sl.t_inhom)[q * P + iv] : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
+
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            if ((k == 1 && !need1) || (k == 2 && !need2)) continue;
+            __syncthreads();   // every wave is done reading the previous W^k
+            if (wact) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    u32x4 hre, lre, him, lim;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const LineRec& L = sm.lines[4 * g4 + j];
+                        const float s = k == 0 ? 1.0f : (k == 1 ? L.tau : L.tb0);
+                        uint32_t hh, ll;
+                        split2(pr[q][j] * s, ps[q][j] * s, hh, ll);   // B_re: (Re W, -Im W) = (D cos, D sin)
+                        hre[j] = hh;
+                        lre[j] = ll;
+                        him[j] = __builtin_amdgcn_alignbit(hh, hh, 16) ^ 0x8000u;   // B_im: (Im W, Re W)
+                        lim[j] = __builtin_amdgcn_alignbit(ll, ll, 16) ^ 0x8000u;
+                    }
+                    sm.w[q][wks][0][0][wslot] = hre;
+                    sm.w[q][wks][0][1][wslot] = lre;
+                    sm.w[q][wks][1][0][wslot] = him;
+                    sm.w[q][wks][1][1][wslot] = lim;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                f32x4 acc[NPF][2];
+#pragma unroll
+                for (int i = 0; i < NPF; i++) acc[i][0] = acc[i][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) {
+                    if (ks >= nks) break;   // uniform; the unrolled steps keep A in registers
+                    u32x4 bh[2], bl[2];
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        bh[f] = sm.w[q][ks][f][0][lane];
+                        bl[f] = sm.w[q][ks][f][1][lane];
+                    }
+                    if (last_ok) {
+#pragma unroll
+                        for (int i = 0; i < NPF; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
+#pragma unroll
+                        for (int i = 0; i < NPF; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
+#pragma unroll
+                        for (int i = 0; i < NPF; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
+                    } else {
+#pragma unroll
+                        for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
+#pragma unroll
+                        for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
+#pragma unroll
+                        for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
+                    }
+                }
+                // coil contraction: conj(S) R = (sr re + si im) + i (sr im - si re) over the lane's four
+                // coils, then the other four from the lane 16 apart
+#pragma unroll
+                for (int i = 0; i < NPF; i++) {
+                    const int p = wave + 4 * i;
+                    if (p >= np) continue;   // wave-uniform
+                    float re_s = 0.f, im_s = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float rr = acc[i][0][r], ri = acc[i][1][r];
+                        if (CPLX || k != 2) re_s = __builtin_fmaf(sr[r], rr, __builtin_fmaf(si[r], ri, re_s));
+                        if (CPLX || k == 2) im_s = __builtin_fmaf(sr[r], ri, __builtin_fmaf(-si[r], rr, im_s));
+                    }
+                    if (CPLX || k != 2) re_s += __shfl_xor(re_s, 16);
+                    if (CPLX || k == 2) im_s += __shfl_xor(im_s, 16);
+                    const int ul = 2 * p + (lq >> 1);
+                    const bool wr = wr_lane && ul < nu;
+                    const VjpUnit& va = vj[uidx_a[i]];
+                    const VjpUnit& vb = vj[uidx_b[i]];
+                    if (k == 0) {
+                        GAS float* gimg = gst((float*)((lq >> 1) ? vb.gimg : va.gimg));
+                        if (wr && gimg) {
+                            if constexpr (CPLX) {
+                                gimg[2 * (q * P + iv)] = re_s;
+                                gimg[2 * (q * P + iv) + 1] = im_s;
+                            } else {
+                                gimg[q * P + iv] = re_s;
+                            }
+                        }
+                    } else if (k == 1) {
+                        GAS float* gti = gst((lq >> 1) ? vb.gti : va.gti);
+                        if (wr && gti) {
+                            // Re(conj(m) b) = m_r b_r + m_i b_i
+                            const float re = CPLX ? m_r[i][q] * re_s + m_i[i][q] * im_s : m_r[i][q] * re_s;
+                            gti[q * P + iv] = re / (ti_ms[q] * ti_ms[q]);
+                        }
+                    } else {
+                        // Im(conj(m) f) = m_r f_i - m_i f_r
+                        fm[i][0] = __builtin_fmaf(m_r[i][q], im_s, fm[i][0]);
+                        if constexpr (CPLX) fm[i][1] = __builtin_fmaf(-m_i[i][q], re_s, fm[i][1]);
+                    }
+                }
+            }
+        }
+        // the fieldmap gradient, and zeros where a unit asks for a sum its group does not have
+        if (wr_lane) {
+#pragma unroll
+            for (int i = 0; i < NPF; i++) {
+                const int p = wave + 4 * i, ul = 2 * p + (lq >> 1);
+                if (p >= np || ul >= nu) continue;
+                const VjpUnit& vv = vj[(lq >> 1) ? uidx_b[i] : uidx_a[i]];
+                if (vv.gfmap) {
+                    const float f = CPLX ? fm[i][0] + fm[i][CM - 1] : fm[i][0];
+                    gst(vv.gfmap)[iv] = need2 ? (float)(kTau * (double)f) : 0.0f;
+                }
+                if (vv.gti && !need1) {
+#pragma unroll
+                    for (int q = 0; q < Q; q++) gst(vv.gti)[q * P + iv] = 0.0f;
+                }
+            }
+        }
+    }
+}
+
+template <int Q, bool CPLX>
+static hipError_t alaunch_npf(int npf, int nwork, const DevUnit* u, const DevSlice* s, const DevGroup* g,
+                              const VjpUnit* v, const WorkItem* w, const int32_t* sets, hipStream_t st) {
+    const dim3 grid(nwork);
+    switch (npf) {
+    case 1: hipLaunchKernelGGL((yadjoint_basis<Q, CPLX, 1>), grid, dim3(256), 0, st, u, s, g, v, w, sets); break;
+    case 2: hipLaunchKernelGGL((yadjoint_basis<Q, CPLX, 2>), grid, dim3(256), 0, st, u, s, g, v, w, sets); break;
+    case 3: hipLaunchKernelGGL((yadjoint_basis<Q, CPLX, 3>), grid, dim3(256), 0, st, u, s, g, v, w, sets); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// npf = unit pairs per wave (a set holds up to 8 * npf units); Q = 1..4 compartments.  `units` and
+// `v` are the adjoint's descriptor arrays; the sets list indices into them.
+hipError_t launch_yadjoint_basis(int Q, bool cplx, int npf, int nwork, const DevUnit* u, const DevSlice* s,
+                                 const DevGroup* g, const VjpUnit* v, const WorkItem* w, const int32_t* sets,
+                                 hipStream_t st) {
+    if (nwork <= 0) return hipSuccess;
+    switch (Q) {
+    case 1: return cplx ? alaunch_npf<1, true>(npf, nwork, u, s, g, v, w, sets, st)
+                        : alaunch_npf<1, false>(npf, nwork, u, s, g, v, w, sets, st);
+    case 2: return cplx ? alaunch_npf<2, true>(npf, nwork, u, s, g, v, w, sets, st)
+                        : alaunch_npf<2, false>(npf, nwork, u, s, g, v, w, sets, st);
+    case 3: return cplx ? alaunch_npf<3, true>(npf, nwork, u, s, g, v, w, sets, st)
+                        : alaunch_npf<3, false>(npf, nwork, u, s, g, v, w, sets, st);
+    case 4: return cplx ? alaunch_npf<4, true>(npf, nwork, u, s, g, v, w, sets, st)
+                        : alaunch_npf<4, false>(npf, nwork, u, s, g, v, w, sets, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace mrs

@@ -68,6 +68,8 @@ def lib():
     L.mrs_plan_get_tiles.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32), P(ct.c_int32)]
     L.mrs_plan_set_basis.argtypes = [ct.c_void_p, ct.c_int32] + E
     L.mrs_plan_get_basis.argtypes = [ct.c_void_p, P(ct.c_int32), P(ct.c_int32)]
+    L.mrs_plan_set_vjp_basis.argtypes = [ct.c_void_p, ct.c_int32] + E
+    L.mrs_plan_get_vjp_basis.argtypes = [ct.c_void_p, P(ct.c_int32)]
     L.mrs_version.restype = ct.c_char_p
     L.mrs_sizeof.argtypes = [ct.c_char_p]
     L.mrs_sizeof.restype = ct.c_size_t
@@ -724,6 +726,16 @@ class Plan:
         a, b = ct.c_int32(), ct.c_int32()
         lib().mrs_plan_get_basis(self._h, ct.byref(a), ct.byref(b))
         return bool(a.value), b.value
+
+    def set_vjp_basis(self, on: bool):
+        """Run the y-adjoint of qualifying slices' volumes in the slice-basis kernel (True) or per unit."""
+        err = _errbuf()
+        _check(lib().mrs_plan_set_vjp_basis(self._h, 1 if on else 0, err, len(err)), err)
+
+    def vjp_basis(self) -> bool:
+        a = ct.c_int32()
+        lib().mrs_plan_get_vjp_basis(self._h, ct.byref(a))
+        return bool(a.value)
 
     def set_zstage(self, sets: Sequence["ZSet"]):
         """Attach partition transforms that run after the 2D forward (and inside its graph)."""
