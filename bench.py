@@ -489,6 +489,10 @@ def main():
     ap.add_argument("--basis", type=int, choices=(0, 1), default=None,
                     help="with --tiles: run qualifying slices in the slice-basis y-stage (1) or per unit (0)")
     ap.add_argument("--only-ge3d", action="store_true", help="run only the 3D gradient-echo train (profiling)")
+    ap.add_argument("--only-grad-batch", action="store_true",
+                    help="run only the batch forward + adjoint (profiling); --vjp-basis picks the y-adjoint")
+    ap.add_argument("--vjp-basis", type=int, choices=(0, 1), default=1,
+                    help="slice-basis y-adjoint (1) or the per-unit one (0) in --only-grad-batch")
     ap.add_argument("--ge3d-tiles", type=str, default="", help="tiles of the 3D train (skips its autotuner)")
     ap.add_argument("--stages-only", action="store_true",
                     help="only the per-stage serial launches (profiling: one dispatch per chunk at a time)")
@@ -531,6 +535,12 @@ def main():
     else:
         log(plan.tune(trials=5).strip())
     nunits = len(w.units)
+    if args.only_grad_batch:
+        plan.set_vjp_basis(bool(args.vjp_basis))
+        print(json.dumps({"grad_pass_batch": grad_batch_pass(plan, w, outs, args.steps),
+                          "vjp_basis": plan.vjp_basis(), "tiles": plan.tiles()}), flush=True)
+        plan.close()
+        return
     if args.stages_only:
         st_ms = plan.stage_times(iters=args.steps)
         print(json.dumps({"stage_ms": st_ms, "tiles": plan.tiles(), "describe": plan.describe()}), flush=True)
