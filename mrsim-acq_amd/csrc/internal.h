@@ -90,6 +90,14 @@ struct DevGroup {
     int32_t n_spikes;
     double noise_sigma;
     double spike_amplitude;
+    // How the slice-basis y-adjoint forms its T2' and fieldmap sums from the line weights tau_l and
+    // tb0_l: 0, one product each; 1, tau_l = adj_alpha * tb0_l on every acquired line, so the T2' sum
+    // is adj_alpha times the fieldmap sum; 2, tau_l = adj_alpha * |tb0_l|, with the lines in the
+    // adjoint's K order (adj_perm) by the sign of tb0 (tb0 >= 0 first, each side padded to 16 lines),
+    // so one tb0-weighted product split at K step adj_ksplit gives both sums.
+    int32_t adj_mode, adj_nks, adj_ksplit;
+    float adj_alpha;
+    int8_t adj_perm[64];       // K position -> line (-1: none); modes 0 and 1: the identity up to LyP
 };
 
 struct DevUnit {

@@ -300,6 +300,59 @@ static std::vector<WorkItem> xcd_order(const std::vector<WorkItem>& w) {
 
 static bool a_ghost(const GroupHost& g) { return g.desc.acq.ghost_offset != 0.0; }
 
+// The relation of the adjoint's line weights (LineRec::tau for T2', tb0 for the fieldmap) over the
+// acquired lines, for the slice-basis y-adjoint: a gradient echo has tau = trf = TE + t and tb0 =
+// t + TE/1000, so tau = 1000 tb0 (mode 1); a spin echo has tau = 1000 |t| and tb0 = t (mode 2, with
+// the lines reordered by the sign of tb0 so each side fills whole 16-line K steps).  Anything else
+// (a caller's timing table) keeps three products (mode 0).  The ratio must hold to float rounding.
+static void adjoint_weights(const std::vector<LineRec>& lines, DevGroup& dg) {
+    dg.adj_mode = 0;
+    dg.adj_alpha = 0.0f;
+    dg.adj_nks = (int)(lines.size() / 16);
+    dg.adj_ksplit = dg.adj_nks;
+    for (int k = 0; k < 64; k++) dg.adj_perm[k] = (int8_t)(k < (int)lines.size() ? k : -1);
+    if (lines.size() > 64) return;
+    double amax = 0.0, tmax = 0.0;
+    int kref = -1;
+    for (size_t l = 0; l < lines.size(); l++) {
+        if (lines[l].ky < 0) continue;
+        tmax = std::max(tmax, (double)std::fabs(lines[l].tau));
+        if (std::fabs(lines[l].tb0) > amax) { amax = std::fabs(lines[l].tb0); kref = (int)l; }
+    }
+    if (kref < 0) return;
+    const double tol = 1e-5 * std::max(tmax, 1e-30);
+    auto holds = [&](double alpha, bool by_sign) {
+        for (const LineRec& L : lines) {
+            if (L.ky < 0) continue;
+            const double w = by_sign ? std::fabs((double)L.tb0) : (double)L.tb0;
+            if (std::fabs((double)L.tau - alpha * w) > tol) return false;
+        }
+        return true;
+    };
+    const double a1 = (double)lines[kref].tau / (double)lines[kref].tb0;
+    if (holds(a1, false)) {
+        dg.adj_mode = 1;
+        dg.adj_alpha = (float)a1;
+        return;
+    }
+    const double a2 = (double)lines[kref].tau / std::fabs((double)lines[kref].tb0);
+    if (!holds(a2, true)) return;
+    std::vector<int> pos, neg;
+    for (size_t l = 0; l < lines.size(); l++) {
+        if (lines[l].ky < 0) continue;
+        (lines[l].tb0 >= 0.0f ? pos : neg).push_back((int)l);
+    }
+    const int np16 = ((int)pos.size() + 15) / 16, nn16 = ((int)neg.size() + 15) / 16;
+    if (16 * (np16 + nn16) > 64) return;
+    for (int k = 0; k < 64; k++) dg.adj_perm[k] = -1;
+    for (size_t i = 0; i < pos.size(); i++) dg.adj_perm[i] = (int8_t)pos[i];
+    for (size_t i = 0; i < neg.size(); i++) dg.adj_perm[16 * np16 + i] = (int8_t)neg[i];
+    dg.adj_mode = 2;
+    dg.adj_alpha = (float)a2;
+    dg.adj_nks = np16 + nn16;
+    dg.adj_ksplit = np16;
+}
+
 // Four lines form a chain when their (t, kyn, d, trf, tau) advance by equal steps, so the
 // y-stage can evaluate the first directly and advance the rest by per-voxel rotors.
 static bool is_chain(const LineRec* c) {
@@ -516,6 +569,7 @@ static int build_group(mrs_plan* p, GroupHost& g, char* err, size_t errlen) {
     dg.t_echo = a.t_echo; dg.t_inhom = a.t_inhom; dg.signal_scale = a.signal_scale;
     dg.eddy_strength = a.eddy_strength; dg.eddy_quad = a.eddy_quad; dg.eddy_phase = a.eddy_phase;
     dg.lines = g.d_lines;
+    adjoint_weights(g.lines, dg);
     for (int tb = 0; tb < 2; tb++) {
         // without a ghost both polarities read table 0, and the x-stage stages one table
         const int src = a.ghost_offset != 0.0 ? tb : 0;
@@ -1490,6 +1544,11 @@ extern "C" int mrs_plan_describe(const mrs_plan* p, char* out, size_t outlen) {
                  "group %zu: %dx%d -> %dx%d, %d coils (%d row blocks), %d compartments, %d lines (%d padded), "
                  "%d chain passes, shared steps %s\n",
                  g, d.snx, d.sny, d.nx, d.ny, d.C, d.RB, d.Q, d.Ly, d.LyP, d.rotor_passes, d.rot_shared ? "yes" : "no");
+        r += line;
+        static const char* kAdj[3] = {"separate T2' and fieldmap products", "T2' = alpha x fieldmap",
+                                      "fieldmap product split by the sign of tb0"};
+        snprintf(line, sizeof line, "group %zu adjoint weights: mode %d (%s), alpha %g, %d K steps, split at %d\n", g,
+                 d.adj_mode, kAdj[d.adj_mode], (double)d.adj_alpha, d.adj_nks, d.adj_ksplit);
         r += line;
     }
     std::strncpy(out, r.c_str(), outlen - 1);

@@ -455,6 +455,7 @@ struct AdjBasisSmem {
     float nr2[Q][kBasisMaxY];
     float nri[Q][kBasisMaxY];
     LineRec lines[64];
+    int32_t perm[64];          // the adjoint's K order: K position -> line (-1: none)
 };
 
 template <int Q, bool CPLX, int NPF>
@@ -477,10 +478,14 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
     const int lq = lane >> 4, ln = lane & 15;
     const int snx = gr.snx, sny = gr.sny;
     const int snyP = (sny + 15) & ~15, snxP = (snx + 15) & ~15;
-    const int C = gr.C, LyP = gr.LyP, nks = LyP >> 4;
+    const int C = gr.C, LyP = gr.LyP;
     const int nch = snyP >> 4;
     const uint32_t P = (uint32_t)(snx * sny);
     const int np = (nu + 1) >> 1;
+    // K steps of 16 lines in the adjoint's line order (the group's table: the identity but in mode 2,
+    // which orders them by the sign of tb0 and splits the tb0-weighted product at ksplit)
+    const int mode = gr.adj_mode, nks = gr.adj_nks, ksplit = gr.adj_ksplit;
+    const float alpha = gr.adj_alpha;
     // this wave's last-round pair exists (the earlier rounds always do: the host gives a set of n
     // units the variant with NPF = ceil(ceil(n / 2) / 4))
     const bool last_ok = wave + 4 * (NPF - 1) < np;
@@ -503,6 +508,9 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
     }
     need1 = need1 && gr.do_relax;
     need2 = need2 && gr.do_dist;
+    // the products to form: the image sum, then (mode 0) the tau- and the tb0-weighted ones, or
+    // (modes 1, 2) one tb0-weighted product that gives both the T2' and the fieldmap sums
+    const bool want1 = mode == 0 && need1, want2 = mode == 0 ? need2 : (need1 || need2);
 
     // the column's static planes and the line table into LDS, once per block
     {
@@ -517,7 +525,9 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
             }
         }
         for (int i = tid; i < LyP; i += 256) sm.lines[i] = gr.lines[i];
+        for (int i = tid; i < 64; i += 256) sm.perm[i] = gr.adj_perm[i];
     }
+    __syncthreads();
 
     // A: the split Gb of this wave's pairs, lines 16 ks + 4 lq .. +3 of row ln (unit ln >> 3 of the
     // pair, coil ln & 7); zero for rows past the set's units or C and for steps past LyP
@@ -532,9 +542,9 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
         for (int ks = 0; ks < 4; ks++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const int l = ks * 16 + 4 * lq + j;
+                const int l = sm.perm[ks * 16 + 4 * lq + j];
                 float2 g = make_float2(0.f, 0.f);
-                if (ok && ks < nks) {
+                if (ok && ks < nks && l >= 0) {
                     const GAS float2& v = gb[(size_t)l * C];
                     g.x = v.x;
                     g.y = v.y;
@@ -545,12 +555,15 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
                 al[i][ks][j] = ll;
             }
     }
-    __syncthreads();
 
     // W generation: this thread's y = 16 ch + yl and lines 4 g4 .. 4 g4 + 3 (one B-operand slot)
     const int yl = tid & 15, g4 = tid >> 4;
     const int wks = g4 >> 2, wslot = (g4 & 3) * 16 + yl;
-    const bool wact = 4 * g4 < LyP;
+    const bool wact = 4 * g4 < 16 * nks;
+    // the thread's four lines in the adjoint's K order (-1: none), read once
+    int lkk[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) lkk[j] = wact ? sm.perm[4 * g4 + j] : -1;
     const double sys_yoff = gr.sys_yoff;
     // coil planes of this lane's four coils 4 (lq & 1) .. +3 (split, [x][c][y], c padded to 8)
     const int c0 = 4 * (lq & 1);
@@ -571,12 +584,13 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
             }
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                const LineRec& L = sm.lines[wact ? 4 * g4 + j : 0];
+                const int lk = lkk[j];
+                const LineRec& L = sm.lines[lk >= 0 ? lk : 0];
                 double cyc = __builtin_fma(L.kyn, yy, bb);
                 cyc = __builtin_fma(rr, L.t_s, cyc);
                 float sn, cs;
                 sincos_turns(cyc, sn, cs);
-                const bool ok = wact && L.ky >= 0;
+                const bool ok = lk >= 0 && L.ky >= 0;
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
                     float d = __builtin_amdgcn_exp2f(__builtin_fmaf(L.trf, a2[q], L.tau * ai2[q]));
@@ -627,9 +641,71 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
         for (int q = 0; q < Q; q++)
             ti_ms[q] = sl.t_inhom ? gld(sl.t_inhom)[q * P + iv] : (sl.has_t_inhom ? sl.ti_uniform[q] : (float)gr.t_inhom);
 
+        // Products over the K steps [k0, k1) of compartment q for this wave's pairs, then the coil sums
+        // of the lane's four coils, completed by the lane 16 apart: ore = sum_c Re(conj(S_c) R_c),
+        // oim = sum_c Im(conj(S_c) R_c), conj(S) R = (sr re + si im) + i (sr im - si re).
+        auto contract = [&](int q, int k0, int k1, bool want_re, bool want_im, float (&ore)[NPF], float (&oim)[NPF]) {
+            f32x4 acc[NPF][2];
+#pragma unroll
+            for (int i = 0; i < NPF; i++) acc[i][0] = acc[i][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                if (ks < k0 || ks >= k1) continue;   // uniform; the unrolled steps keep A in registers
+                u32x4 bh[2], bl[2];
+#pragma unroll
+                for (int f = 0; f < 2; f++) {
+                    bh[f] = sm.w[q][ks][f][0][lane];
+                    bl[f] = sm.w[q][ks][f][1][lane];
+                }
+                if (last_ok) {
+#pragma unroll
+                    for (int i = 0; i < NPF; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
+#pragma unroll
+                    for (int i = 0; i < NPF; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
+#pragma unroll
+                    for (int i = 0; i < NPF; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
+#pragma unroll
+                    for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
+#pragma unroll
+                    for (int i = 0; i < NPF - 1; i++)
+#pragma unroll
+                        for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NPF; i++) {
+                float re_s = 0.f, im_s = 0.f;
+                if (wave + 4 * i < np) {   // wave-uniform
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float rr = acc[i][0][r], ri = acc[i][1][r];
+                        if (want_re) re_s = __builtin_fmaf(sr[r], rr, __builtin_fmaf(si[r], ri, re_s));
+                        if (want_im) im_s = __builtin_fmaf(sr[r], ri, __builtin_fmaf(-si[r], rr, im_s));
+                    }
+                    if (want_re) re_s += __shfl_xor(re_s, 16);
+                    if (want_im) im_s += __shfl_xor(im_s, 16);
+                }
+                ore[i] = re_s;
+                oim[i] = im_s;
+            }
+        };
+
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            if ((k == 1 && !need1) || (k == 2 && !need2)) continue;
+            if ((k == 1 && !want1) || (k == 2 && !want2)) continue;
             __syncthreads();   // every wave is done reading the previous W^k
             if (wact) {
 #pragma unroll
@@ -637,7 +713,8 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
                     u32x4 hre, lre, him, lim;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        const LineRec& L = sm.lines[4 * g4 + j];
+                        const int lk = lkk[j];
+                        const LineRec& L = sm.lines[lk >= 0 ? lk : 0];   // W is zero where lk < 0
                         const float s = k == 0 ? 1.0f : (k == 1 ? L.tau : L.tb0);
                         uint32_t hh, ll;
                         split2(pr[q][j] * s, ps[q][j] * s, hh, ll);   // B_re: (Re W, -Im W) = (D cos, D sin)
@@ -655,86 +732,70 @@ void yadjoint_basis(const DevUnit* __restrict__ units, const DevSlice* __restric
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < Q; q++) {
-                f32x4 acc[NPF][2];
+                float re_a[NPF], im_a[NPF], re_b[NPF], im_b[NPF];
+                // the tb0-weighted product in two parts only when the T2' sum comes from it
+                const bool b_from_f = k == 2 && mode != 0 && need1;
+                if (k == 0 || k == 1)
+                    contract(q, 0, nks, true, CPLX, re_a, im_a);
+                else if (!b_from_f)
+                    contract(q, 0, nks, CPLX, true, re_a, im_a);
+                else
+                    contract(q, 0, ksplit, true, true, re_a, im_a);
+                if (b_from_f && mode == 2)
+                    contract(q, ksplit, nks, true, true, re_b, im_b);
+                else
 #pragma unroll
-                for (int i = 0; i < NPF; i++) acc[i][0] = acc[i][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int ks = 0; ks < 4; ks++) {
-                    if (ks >= nks) break;   // uniform; the unrolled steps keep A in registers
-                    u32x4 bh[2], bl[2];
-#pragma unroll
-                    for (int f = 0; f < 2; f++) {
-                        bh[f] = sm.w[q][ks][f][0][lane];
-                        bl[f] = sm.w[q][ks][f][1][lane];
-                    }
-                    if (last_ok) {
-#pragma unroll
-                        for (int i = 0; i < NPF; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
-#pragma unroll
-                        for (int i = 0; i < NPF; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
-#pragma unroll
-                        for (int i = 0; i < NPF; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
-                    } else {
-#pragma unroll
-                        for (int i = 0; i < NPF - 1; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(al[i][ks], bh[f], acc[i][f]);
-#pragma unroll
-                        for (int i = 0; i < NPF - 1; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bl[f], acc[i][f]);
-#pragma unroll
-                        for (int i = 0; i < NPF - 1; i++)
-#pragma unroll
-                            for (int f = 0; f < 2; f++) acc[i][f] = mfma_bf16(ah[i][ks], bh[f], acc[i][f]);
-                    }
-                }
-                // coil contraction: conj(S) R = (sr re + si im) + i (sr im - si re) over the lane's four
-                // coils, then the other four from the lane 16 apart
+                    for (int i = 0; i < NPF; i++) re_b[i] = im_b[i] = 0.0f;
 #pragma unroll
                 for (int i = 0; i < NPF; i++) {
                     const int p = wave + 4 * i;
                     if (p >= np) continue;   // wave-uniform
-                    float re_s = 0.f, im_s = 0.f;
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const float rr = acc[i][0][r], ri = acc[i][1][r];
-                        if (CPLX || k != 2) re_s = __builtin_fmaf(sr[r], rr, __builtin_fmaf(si[r], ri, re_s));
-                        if (CPLX || k == 2) im_s = __builtin_fmaf(sr[r], ri, __builtin_fmaf(-si[r], rr, im_s));
-                    }
-                    if (CPLX || k != 2) re_s += __shfl_xor(re_s, 16);
-                    if (CPLX || k == 2) im_s += __shfl_xor(im_s, 16);
                     const int ul = 2 * p + (lq >> 1);
                     const bool wr = wr_lane && ul < nu;
                     const VjpUnit& va = vj[uidx_a[i]];
                     const VjpUnit& vb = vj[uidx_b[i]];
+                    // the T2' sum b and the fieldmap sum f of this kind, when it gives them
+                    float br = 0.f, bi = 0.f, fr = 0.f, fi = 0.f;
+                    bool has_b = false, has_f = false;
+                    if (k == 1) {
+                        br = re_a[i];
+                        bi = im_a[i];
+                        has_b = true;
+                    } else if (k == 2) {
+                        fr = re_a[i] + re_b[i];
+                        fi = im_a[i] + im_b[i];
+                        has_f = true;
+                        if (b_from_f) {
+                            // mode 1: tau = alpha tb0 (no second part); mode 2: tau = alpha tb0 on the
+                            // lines with tb0 >= 0 (part a) and -alpha tb0 on the others (part b)
+                            br = alpha * (re_a[i] - re_b[i]);
+                            bi = alpha * (im_a[i] - im_b[i]);
+                            has_b = true;
+                        }
+                    }
                     if (k == 0) {
                         GAS float* gimg = gst((float*)((lq >> 1) ? vb.gimg : va.gimg));
                         if (wr && gimg) {
                             if constexpr (CPLX) {
-                                gimg[2 * (q * P + iv)] = re_s;
-                                gimg[2 * (q * P + iv) + 1] = im_s;
+                                gimg[2 * (q * P + iv)] = re_a[i];
+                                gimg[2 * (q * P + iv) + 1] = im_a[i];
                             } else {
-                                gimg[q * P + iv] = re_s;
+                                gimg[q * P + iv] = re_a[i];
                             }
                         }
-                    } else if (k == 1) {
+                    }
+                    if (has_b) {
                         GAS float* gti = gst((lq >> 1) ? vb.gti : va.gti);
                         if (wr && gti) {
                             // Re(conj(m) b) = m_r b_r + m_i b_i
-                            const float re = CPLX ? m_r[i][q] * re_s + m_i[i][q] * im_s : m_r[i][q] * re_s;
+                            const float re = CPLX ? m_r[i][q] * br + m_i[i][q] * bi : m_r[i][q] * br;
                             gti[q * P + iv] = re / (ti_ms[q] * ti_ms[q]);
                         }
-                    } else {
+                    }
+                    if (has_f) {
                         // Im(conj(m) f) = m_r f_i - m_i f_r
-                        fm[i][0] = __builtin_fmaf(m_r[i][q], im_s, fm[i][0]);
-                        if constexpr (CPLX) fm[i][1] = __builtin_fmaf(-m_i[i][q], re_s, fm[i][1]);
+                        fm[i][0] = __builtin_fmaf(m_r[i][q], fi, fm[i][0]);
+                        if constexpr (CPLX) fm[i][1] = __builtin_fmaf(-m_i[i][q], fr, fm[i][1]);
                     }
                 }
             }

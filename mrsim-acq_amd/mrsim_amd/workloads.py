@@ -120,6 +120,7 @@ class Workload:
     units: list = field(default_factory=list)    # (slice index, images (Q,P) float32 or complex64)
     complex_images: bool = False
     recon_window: int = abi.WINDOW_NONE
+    timing: object = None   # a caller's line-timing table (t_ms, trf_ms, tread_ms, polarity), or None
 
 
 def _slice(st: Stream, snx, sny, Q, C, coil_sigma, coil_radius):
@@ -202,7 +203,8 @@ def to_plan(w: Workload, precision=abi.PREC_SPLIT_BF16, device="cuda"):
     import torch
 
     from .gpu import Group, Plan, Slice, Unit
-    g = Group(w.snx, w.sny, w.nx, w.ny, w.n_coils, w.n_comp, w.acq, complex_images=w.complex_images)
+    g = Group(w.snx, w.sny, w.nx, w.ny, w.n_coils, w.n_comp, w.acq, complex_images=w.complex_images,
+              timing=w.timing)
     keep, slices, units, outs = [], [], [], []
     for sd in w.slices:
         t = dict(fmap=torch.from_numpy(sd.fmap).to(device),

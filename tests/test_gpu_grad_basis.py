@@ -4,7 +4,8 @@ them on MFMA (mrs_plan_set_vjp_basis).  The adjoint of the forward in src/kspace
 reference has none; the per-unit adjoint kernels are the comparison).
 
 Checks, on workloads with 20, 7 and 45 volumes per slice (128^2 -> 64^2, 8 coils, 3 compartments) and
-small grids (40^2 -> 20^2, 3 coils, Q = 1, 2 complex, 4):
+small grids (40^2 -> 20^2, 3 coils, Q = 1, 2 complex, 4; gradient echoes with the default timing and
+with a caller's timing table), covering the three ways the adjoint forms its T2' and fieldmap sums:
   - every unit's image, fieldmap and T2' gradients against the per-unit y-adjoint (both split-BF16
     products of the same FP32 sums);
   - every output element written (the buffers start as NaN);
@@ -41,11 +42,20 @@ def _workload(name):
         return workloads.make("batch_128", n_slices=2, n_volumes=7)
     if name == "batch_q3_v45":       # three sets of 15 units
         return workloads.make("batch_128", n_slices=1, n_volumes=45)
-    Q = {"small_q1": 1, "small_q2_cplx": 2, "small_q4": 4}[name]
+    Q = {"small_q1": 1, "small_q2_cplx": 2, "small_q4": 4, "small_ge": 2, "small_ge_timing": 1}[name]
     cplx = name.endswith("cplx")
     C = 3
     acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=C)
+    if name.startswith("small_ge"):
+        acq.echo = abi.ECHO_GRADIENT
     w = Workload(name, 40, 40, 20, 20, C, Q, acq, complex_images=cplx)
+    if name == "small_ge_timing":
+        # a caller's timing table whose RF times are not TE + t: the T2' weights are no multiple of
+        # the fieldmap ones, so the adjoint keeps separate products
+        from mrsim_amd import gpu
+        t, trf, tread, pol = gpu.line_timing(acq, 20, 20)
+        trf = trf + 0.3 * np.sin(np.arange(20))
+        w.timing = (t, trf, tread, pol)
     for s in range(2):
         st = Stream(0x5EED0B00 + 31 * s + Q)
         sd = _slice(st, 40, 40, Q, C, 16.0, 24.0)
@@ -90,10 +100,25 @@ def _oracle_k(w, u, images=None, fmap=None, t_inhom=None):
     return O.forward_direct(w.snx, w.sny, w.nx, w.ny, w.n_coils, w.n_comp, w.acq,
                             img if images is None else images, sd.fmap if fmap is None else fmap, sd.coil_maps,
                             phase0=None if w.complex_images else sd.phase0, t2=sd.t2,
-                            t_inhom=sd.t_inhom if t_inhom is None else t_inhom, complex_images=w.complex_images)
+                            t_inhom=sd.t_inhom if t_inhom is None else t_inhom, complex_images=w.complex_images,
+                            timing=w.timing)
 
 
-CASES = ["batch_q3_v20", "batch_q3_v7", "batch_q3_v45", "small_q1", "small_q2_cplx", "small_q4"]
+# The line-weight mode the plan picks for each workload: spin echoes split the fieldmap product by
+# the sign of the line time (2), gradient echoes scale it (1), a caller's timing table with other RF
+# times keeps separate products (0).
+MODES = {"batch_q3_v20": 2, "batch_q3_v7": 2, "batch_q3_v45": 2, "small_q1": 2, "small_q2_cplx": 2, "small_q4": 2,
+         "small_ge": 1, "small_ge_timing": 0}
+CASES = list(MODES)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_adjoint_weight_mode(name):
+    w = _workload(name)
+    plan, outs, keep = workloads.to_plan(w)
+    desc = plan.describe()
+    plan.close()
+    assert f"adjoint weights: mode {MODES[name]} " in desc, desc
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -125,7 +150,7 @@ def test_slice_basis_adjoint_matches_per_unit_and_fp64(name):
         assert abs(lhs - rhs) <= 1e-5 * scale
 
 
-@pytest.mark.parametrize("name", ["small_q1", "small_q2_cplx", "small_q4"])
+@pytest.mark.parametrize("name", ["small_q1", "small_q2_cplx", "small_q4", "small_ge", "small_ge_timing"])
 def test_slice_basis_adjoint_matches_fp64_central_differences(name):
     # L_u = Re <g_u, A_u(m, f, T2')>.  The fieldmap and T2' derivatives at a few voxels of a unit against
     # FP64 central differences of the oracle (step 1e-3 relative), and the image gradient against the
