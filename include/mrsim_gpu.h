@@ -235,7 +235,9 @@ int mrs_plan_stats(const mrs_plan* plan, double* terms, double* lines, double* u
 int mrs_plan_describe(const mrs_plan* plan, char* out, size_t outlen);
 int mrs_plan_destroy(mrs_plan* plan);
 
-/* Kernel tile parameters chosen by the autotuner (see mrs_plan_tune). */
+/* Kernel tile parameters chosen by the autotuner: every candidate's graph replay timed on `stream`,
+ * the six fastest again round-robin, the lowest median kept (the plan's graph is rebuilt per
+ * candidate).  `report` lists the times. */
 int mrs_plan_tune(mrs_plan* plan, void* stream, int32_t trials, char* report, size_t reportlen);
 
 /* Set / read the y-stage tiles directly: x columns per block (1..64), 16-line tiles per pass

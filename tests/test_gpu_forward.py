@@ -242,6 +242,11 @@ def test_graph_replay_tuning_and_determinism():
         _ = (ref[0].abs() * 2).sum().item()  # eager work between replays
     report = plan.tune(trials=2)
     print(report, flush=True)
+    # the pick is the finalist (graph replays, round-robin) with the lowest median, and the plan keeps it
+    best = [ln for ln in report.splitlines() if ln.startswith("best ")][-1]
+    kv = dict(t.split("=") for t in best.split()[1:5])
+    assert f"finalist basis={kv['basis']} xt={kv['xt']} gp={kv['gp']} nv={kv['nv']} graph median" in report, report
+    assert plan.tiles() == (int(kv["xt"]), int(kv["gp"]), int(kv["nv"]))
     plan.forward()
     torch.cuda.synchronize()
     # the x tile only redistributes work (bitwise equal); line tiles and unit sets change the
