@@ -1632,7 +1632,8 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
                 rc = mrs_plan_forward_graph(p, st, err, errlen);  // capture, warm-up
                 if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
                 HIP_TRY(hipEventRecord(e0, st));
-                for (int t = 0; t < trials; t++) mrs_plan_forward_graph(p, st, err, errlen);
+                for (int t = 0; t < trials && !rc; t++) rc = mrs_plan_forward_graph(p, st, err, errlen);
+                if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
                 HIP_TRY(hipEventRecord(e1, st));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1657,7 +1658,8 @@ extern "C" int mrs_plan_tune(mrs_plan* p, void* stream, int32_t trials, char* re
                 for (int w = 0; w < 2 && !rc; w++) rc = mrs_plan_forward_graph(p, st, err, errlen);
                 if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
                 HIP_TRY(hipEventRecord(e0, st));
-                for (int t = 0; t < reps; t++) mrs_plan_forward_graph(p, st, err, errlen);
+                for (int t = 0; t < reps && !rc; t++) rc = mrs_plan_forward_graph(p, st, err, errlen);
+                if (rc) { hipEventDestroy(e0); hipEventDestroy(e1); return rc; }
                 HIP_TRY(hipEventRecord(e1, st));
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0.f;
