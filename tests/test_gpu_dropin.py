@@ -93,4 +93,6 @@ def test_drop_in_repeated_call_reuses_its_context():
     t_again = (time.perf_counter() - t0) / n
     print(f"drop-in 128^2 8 coils: first call {1e3 * t_first:.2f} ms, then {1e3 * t_again:.3f} ms per call", flush=True)
     assert np.array_equal(first, again)
-    assert t_again < t_first
+    # an earlier test of this process may have built the shape's context already: then every call
+    # here is a reuse, and all are fast (a call that builds one takes several milliseconds)
+    assert t_again < t_first or t_again < 1e-3, (t_first, t_again)
