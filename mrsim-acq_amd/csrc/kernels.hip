@@ -26,12 +26,21 @@ namespace mrs {
 // slice: user planes (x fastest) are read coalesced along x, staged in LDS, and written in the
 // kernels' [x][y] order coalesced along y.  grid = (tiles_x * tiles_y, listed slices): only slices
 // that own a plane set are listed (the others read their owner's planes).
+// QM: the most compartments of any slice in the launch (sizes the per-compartment tiles, so that the
+// LDS of a block stays within a third of a CU's for up to three compartments).  The static tiles
+// (base, poly) and the coil tiles of a pair of coils share their LDS: the coil phase starts after the
+// static planes are written.
+template <int QM>
 __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices, const DevGroup* __restrict__ groups,
                                                    const int32_t* __restrict__ list, int nlist, int mode) {
-    __shared__ double sd[2][32][33];
-    __shared__ float sf[1 + 2 * MRS_MAX_COMP][32][33];
-    __shared__ float2 sc[32][33];
+    __shared__ union {
+        double sd[2][32][33];
+        float2 sc[2][32][33];
+    } lds_u;
+    __shared__ float sf[1 + 2 * QM][32][33];
     __shared__ float samp[32][33];
+    auto& sd = lds_u.sd;
+    auto& sc = lds_u.sc;
     if ((int)blockIdx.y >= nlist) return;
     const int s = list[blockIdx.y];
     DevSlice& sl = slices[s];
@@ -41,7 +50,7 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
     const int tiles_x = (snxP + 31) / 32, tiles_y = (snyP + 31) / 32;
     if ((int)blockIdx.x >= tiles_x * tiles_y) return;
     const int x0 = (blockIdx.x % tiles_x) * 32, y0 = (blockIdx.x / tiles_x) * 32;
-    const int C = gr.C, Q = gr.Q;
+    const int C = gr.C, Q = min(gr.Q, QM);   // the launch's QM covers every slice's Q
     const bool p_own = sl.p_owner != 0;   // read here, where its latency hides behind the read phase
     const size_t PP = (size_t)snxP * snyP, P = (size_t)snx * sny;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // read phase: x fastest
@@ -135,31 +144,39 @@ __global__ __launch_bounds__(256) void prep_slices(DevSlice* __restrict__ slices
                     m[cc][k] = *at32<f32x2>(sl.coil_maps, (uint32_t)c * (uint32_t)P + (uint32_t)(x + snx * y));
             }
         }
+        // two coils per transpose through LDS (one pair of barriers)
 #pragma unroll
-        for (int cc = 0; cc < CG; cc++) {
+        for (int cc = 0; cc < CG; cc += 2) {
             const int c = c0 + cc;
             if (c >= Cp) break;
-            __syncthreads();
+            __syncthreads();   // the previous pair's (or the static planes') reads of this LDS are done
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int yl = ty + 8 * k;
-                const float a = samp[tx][yl];
-                sc[tx][yl] = make_float2(m[cc][k].x * a, m[cc][k].y * a);
-            }
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int yl = ty + 8 * k;
+                    const float a = samp[tx][yl];
+                    sc[h][tx][yl] = make_float2(m[cc + h][k].x * a, m[cc + h][k].y * a);
+                }
             __syncthreads();
             for (int k = 0; k < 4; k++) {
                 const int xl = wx + 8 * k, x = x0 + xl, y = y0 + wy;
                 if (x >= snxP || y >= snyP) continue;
-                const float2 v = sc[xl][wy];
-                if (w_split) {
-                    uint32_t hi, lo;
-                    split2(v.x, v.y, hi, lo);
-                    const size_t so = ((size_t)x * Cp + c) * snyP + y;
-                    gst(sl.s_hi)[so] = hi;
-                    gst(sl.s_lo)[so] = lo;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int ch = c + h;
+                    if (ch >= Cp) break;
+                    const float2 v = sc[h][xl][wy];
+                    if (w_split) {
+                        uint32_t hi, lo;
+                        split2(v.x, v.y, hi, lo);
+                        const size_t so = ((size_t)x * Cp + ch) * snyP + y;
+                        gst(sl.s_hi)[so] = hi;
+                        gst(sl.s_lo)[so] = lo;
+                    }
+                    if (w_s32 && ch < C)
+                        reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)ch * snxP + x) * snyP + y] = f32x2{v.x, v.y};
                 }
-                if (w_s32 && c < C)
-                    reinterpret_cast<GAS f32x2*>(gst(sl.s32))[((size_t)c * snxP + x) * snyP + y] = f32x2{v.x, v.y};
             }
         }
     }
@@ -2076,11 +2093,19 @@ namespace mrs {
 
 // `maxtiles`: the largest number of 32 x 32 tiles of any slice's padded grid.
 // list = the slices to prepare (those that own a plane set: the others read an owner's planes)
+// maxq: the most compartments of any slice in the plan.
 hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, const int32_t* list, int nlist, int maxtiles, int mode,
-                       hipStream_t st) {
+                       int maxq, hipStream_t st) {
     if (nlist <= 0) return hipSuccess;
     dim3 grid(maxtiles, nlist);
-    hipLaunchKernelGGL(prep_slices, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
+    if (maxq <= 1)
+        hipLaunchKernelGGL(prep_slices<1>, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
+    else if (maxq == 2)
+        hipLaunchKernelGGL(prep_slices<2>, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
+    else if (maxq == 3)
+        hipLaunchKernelGGL(prep_slices<3>, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
+    else
+        hipLaunchKernelGGL(prep_slices<MRS_MAX_COMP>, grid, dim3(256), 0, st, slices, groups, list, nlist, mode);
     return hipGetLastError();
 }
 

@@ -20,7 +20,7 @@
 
 namespace mrs {
 hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, const int32_t* list, int nlist, int maxvox, int mode,
-                       hipStream_t st);
+                       int maxq, hipStream_t st);
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
                               hipStream_t st);
@@ -177,6 +177,7 @@ struct mrs_plan {
     std::vector<uint8_t> basis_slice;   // the slice's units can run in the slice-basis y-stage
     int maxvox = 0;
     int maxC = 1;
+    int maxQ = 1;   // the most compartments of any group (sizes the prep's per-compartment tiles)
     bool any_zero_rows = false, any_epilogue = false;
     double terms = 0.0, nlines = 0.0;
     hipGraph_t graph = nullptr;
@@ -788,6 +789,7 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         gh.desc.timing = groups[g].timing;
         rc = build_group(p, gh, err, errlen);
         gh.desc.timing = nullptr;
+        if (rc == MRS_OK) p->maxQ = std::max(p->maxQ, gh.dev.Q);
     }
     if (rc) { free_plan_memory(p); delete p; return rc; }
     p->slices.resize(nslices);
@@ -1170,7 +1172,7 @@ static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen
     const bool chunked = p->precision == MRS_PREC_SPLIT_BF16 && p->nchunks > 1;
     if (!chunked)
         HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
-                            fwd_prep_mode(p), st));
+                            fwd_prep_mode(p), p->maxQ, st));
     if (p->precision == MRS_PREC_SPLIT_BF16) {
         if (p->nchunks == 1) {
             int rc = enqueue_chunk(p, 0, st, err, errlen);
@@ -1184,7 +1186,7 @@ static int enqueue_forward(mrs_plan* p, hipStream_t st, char* err, size_t errlen
             for (int k = 0; k < p->nchunks; k++) {
                 HIP_TRY(hipStreamWaitEvent(p->streams[k], p->ev_prep, 0));
                 HIP_TRY(launch_prep(p->d_slices, p->d_groups, dl + p->prep_off[k],
-                                    (int)(p->prep_off[k + 1] - p->prep_off[k]), p->maxvox, fwd_prep_mode(p),
+                                    (int)(p->prep_off[k + 1] - p->prep_off[k]), p->maxvox, fwd_prep_mode(p), p->maxQ,
                                     p->streams[k]));
                 HIP_TRY(hipEventRecord(p->ev_prepk[k], p->streams[k]));
             }
@@ -1436,8 +1438,7 @@ static int plan_vjp(mrs_plan* p, const mrs_c32* const* grad_out, void* const* gr
     static_assert(PREP_SPLIT != PREP_S32, "plane forms");
     if (!(reuse && p->planes_current && fwd_prep_mode(p) == (mfma ? PREP_SPLIT : PREP_S32)))
         HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
-                            mfma ? PREP_SPLIT : PREP_S32,
-                            st));
+                            mfma ? PREP_SPLIT : PREP_S32, p->maxQ, st));
     for (const auto& run : p->vjp_runs) {
         const DevGroup& dg = p->groups[p->units[p->vjp_order[run.first]].group].dev;
         HIP_TRY(launch_vjp_units((int)run.first, (int)(run.second - run.first), dg.Q, dg.C, dg.cplx != 0,
@@ -1479,7 +1480,7 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
     for (int it = 0; it < iters && rc == MRS_OK; it++) {
         HIP_TRY(hipEventRecord(ev[0], st));
         HIP_TRY(launch_prep(p->d_slices, p->d_groups, p->d_prep, (int)p->prep_all.size(), p->maxvox,
-                            fwd_prep_mode(p), st));
+                            fwd_prep_mode(p), p->maxQ, st));
         HIP_TRY(hipEventRecord(ev[1], st));
         if (p->precision == MRS_PREC_SPLIT_BF16) {
             // every chunk's y-stage, then every chunk's x-stage, serially on `stream`
