@@ -752,13 +752,17 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
 // from an offset beyond the buffer (the load returns zeros and touches no memory): with no load
 // under a branch the compiler's wait counts stay exact.  G arrives split ({hi, lo} per complex value,
 // written so by the y-stage): the loads are the B operands.
-template <int MT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void xstage_mfma(const DevUnit* __restrict__ units,
+// NW waves per block: 4, or 8 where the twiddle rows of more kx (up to 96) fill most of a CU's LDS
+// (one block per CU, two waves per SIMD) so that fewer kx blocks re-read each unit's G.  XB: the
+// 16-x chunks per step (fewer above MT = 4, where the accumulators and twiddle operands grow).
+template <int MT, int NW = 4>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void xstage_mfma(const DevUnit* __restrict__ units,
                                                    const DevGroup* __restrict__ groups,
                                                    const WorkItem* __restrict__ work, int nwork, int nparts) {
     extern __shared__ uint32_t tl[];
     constexpr int rows = 16 * MT;
-    constexpr int XB = 8;
+    constexpr int XB = MT > 4 ? 4 : 8;
+    constexpr int NT = 64 * NW;   // threads per block
     constexpr uint32_t kOutside = 0x80000000u;
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -795,8 +799,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
         c.ntiles = (gr.LyP * gr.C) >> 4;
         c.nch = c.snxP >> 4;
         c.nbt = (c.nch + XB - 1) / XB;
-        c.nt0 = part * 4 + wv;
-        c.nslot = (c.ntiles - part * 4 + 4 * nparts - 1) / (4 * nparts);
+        c.nt0 = part * NW + wv;
+        c.nslot = (c.ntiles - part * NW + NW * nparts - 1) / (NW * nparts);
         c.gbuf = u.gbuf;
         c.out = u.out;
     };
@@ -820,11 +824,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
         constexpr int SU = 8;
         const int per_tab = rows * (snxP / 4);
         for (int tb = 0; tb < ntab; tb++) {
-            for (int i0 = threadIdx.x; i0 < per_tab; i0 += 256 * SU) {
+            for (int i0 = threadIdx.x; i0 < per_tab; i0 += NT * SU) {
                 u32x4 h[SU], l[SU];
 #pragma unroll
                 for (int k = 0; k < SU; k++) {
-                    const int i = i0 + k * 256;
+                    const int i = i0 + k * NT;
                     const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
                     const int kx = kx0 + r;
                     h[k] = l[k] = u32x4{0u, 0u, 0u, 0u};
@@ -835,7 +839,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
                 }
 #pragma unroll
                 for (int k = 0; k < SU; k++) {
-                    const int i = i0 + k * 256;
+                    const int i = i0 + k * NT;
                     if (i < per_tab) {
                         const int r = i / (snxP / 4), x4 = (i % (snxP / 4)) * 4;
                         *reinterpret_cast<u32x4*>(tl + ((tb * 2 + 0) * rows + r) * stride + x4) = h[k];
@@ -845,7 +849,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
             }
         }
         int32_t* const lrow = reinterpret_cast<int32_t*>(tl + ntab * 2 * rows * stride);
-        for (int i = threadIdx.x; i < c.LyP; i += 256) lrow[i] = gr.lines[i].ky * 2 + (gr.lines[i].table & 1);
+        for (int i = threadIdx.x; i < c.LyP; i += NT) lrow[i] = gr.lines[i].ky * 2 + (gr.lines[i].table & 1);
         s_group = c.group;
         s_kxb = c.kxb;
         s_ntab = ntab;
@@ -855,7 +859,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
     // offset and the lane's (its 4-x group, column) in the vector one, so no load needs address
     // arithmetic.
     auto load_step = [&](const Cur& c, u32x2 (&buf)[XB][4]) {
-        const int nt = c.nt0 + c.j * 4 * nparts;
+        const int nt = c.nt0 + c.j * NW * nparts;
         const bool valid = c.i < i_hi && nt < c.ntiles;
         const __amdgpu_buffer_rsrc_t grs = __builtin_amdgcn_make_buffer_rsrc(
             (void*)c.gbuf, (short)0, (int)((uint32_t)c.snxP * (uint32_t)(c.LyP * c.C) * 8u), 0x00020000);
@@ -882,7 +886,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
             __syncthreads();
         }
         const int stride = c.snxP + 4;
-        const int nt = c.nt0 + c.j * 4 * nparts;
+        const int nt = c.nt0 + c.j * NW * nparts;
         const bool valid = nt < c.ntiles;
         const int32_t* const lrow = reinterpret_cast<const int32_t*>(tl + s_ntab * 2 * rows * stride);
         const int tb = (s_ntab == 2 && valid) ? __builtin_amdgcn_readfirstlane(lrow[(nt * 16) / c.C] & 1) : 0;
@@ -2177,26 +2181,42 @@ hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool edd
                 : ylaunch_q<false, false>(Q, RB, GP, nv, grid, st, u, s, g, w, usets, xt);
 }
 
-hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
+// nw = 4: mt = 1..4; nw = 8: mt = 3..6.
+hipError_t launch_xstage_mfma(int mt, int nw, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
                               const DevGroup* g, const WorkItem* w, hipStream_t st) {
     if (nwork <= 0) return hipSuccess;
     const dim3 grid(std::max(1, std::min(nwork, nblk)));
-    switch (mt) {
-    case 1: hipLaunchKernelGGL((xstage_mfma<1>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
-    case 2: hipLaunchKernelGGL((xstage_mfma<2>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
-    case 3: hipLaunchKernelGGL((xstage_mfma<3>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
-    case 4: hipLaunchKernelGGL((xstage_mfma<4>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
-    default: return hipErrorInvalidValue;
+    if (nw == 4) {
+        switch (mt) {
+        case 1: hipLaunchKernelGGL((xstage_mfma<1>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+        case 2: hipLaunchKernelGGL((xstage_mfma<2>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+        case 3: hipLaunchKernelGGL((xstage_mfma<3>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+        case 4: hipLaunchKernelGGL((xstage_mfma<4>), grid, dim3(256), lds, st, u, g, w, nwork, nparts); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else if (nw == 8) {
+        switch (mt) {
+        case 3: hipLaunchKernelGGL((xstage_mfma<3, 8>), grid, dim3(512), lds, st, u, g, w, nwork, nparts); break;
+        case 4: hipLaunchKernelGGL((xstage_mfma<4, 8>), grid, dim3(512), lds, st, u, g, w, nwork, nparts); break;
+        case 5: hipLaunchKernelGGL((xstage_mfma<5, 8>), grid, dim3(512), lds, st, u, g, w, nwork, nparts); break;
+        case 6: hipLaunchKernelGGL((xstage_mfma<6, 8>), grid, dim3(512), lds, st, u, g, w, nwork, nparts); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t set_xstage_lds_limit(size_t bytes) {
-    hipError_t e;
-    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
-    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
-    if ((e = hipFuncSetAttribute((const void*)xstage_mfma<3>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes))) return e;
-    return hipFuncSetAttribute((const void*)xstage_mfma<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fns[] = {(const void*)xstage_mfma<1>,    (const void*)xstage_mfma<2>,    (const void*)xstage_mfma<3>,
+                         (const void*)xstage_mfma<4>,    (const void*)xstage_mfma<3, 8>, (const void*)xstage_mfma<4, 8>,
+                         (const void*)xstage_mfma<5, 8>, (const void*)xstage_mfma<6, 8>};
+    for (const void* f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_plain_unit(int unit, int C, int LyP, int snx, int nx, const DevUnit* u, const DevSlice* s,

@@ -24,7 +24,7 @@ hipError_t launch_prep(DevSlice* slices, const DevGroup* groups, const int32_t* 
 hipError_t launch_ystage_mfma(int Q, int RB, int GP, int nv, bool cplx, bool eddy, int nwork, int xt, const DevUnit* u,
                               const DevSlice* s, const DevGroup* g, const WorkItem* w, const int32_t* usets,
                               hipStream_t st);
-hipError_t launch_xstage_mfma(int mt, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
+hipError_t launch_xstage_mfma(int mt, int nw, int nparts, size_t lds, int nwork, int nblk, const DevUnit* u,
                               const DevGroup* g, const WorkItem* w, hipStream_t st);
 hipError_t launch_ystage_basis(int Q, bool cplx, int npw, int last, int nwork, const DevUnit* u, const DevSlice* s,
                                const DevGroup* g, const WorkItem* w, const int32_t* sets, hipStream_t st);
@@ -155,7 +155,7 @@ struct mrs_plan {
     };
     std::vector<YClass> yclasses;
     struct XClass {
-        int mt, nparts;
+        int mt, nw, nparts;   // kx tiles and waves per block, column-tile parts per unit
         size_t lds;
         int nblk;   // persistent blocks: as many as are resident at once (by LDS, at most 2 per CU)
         std::vector<WorkItem> items;
@@ -891,10 +891,11 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
         p->unit_chunk.assign(nunits, 0);
         for (size_t k = 0; k < nunits; k++) p->unit_chunk[order[k]] = (int)(k * p->nchunks / nunits);
         std::vector<std::vector<std::vector<WorkItem>>> xw;
-        std::map<std::tuple<int, int, size_t>, size_t> xcls;
-        // x-stage tiling per group: 8*MT kx rows of twiddles staged in LDS (<= 80 KiB so two blocks
-        // share a CU), the unit's column tiles split over `nparts` blocks
-        std::vector<int> g_mt(ngroups), g_np(ngroups);
+        std::map<std::tuple<int, int, size_t, int>, size_t> xcls;
+        // x-stage tiling per group: 16*MT kx rows of twiddles staged in LDS (<= 80 KiB so two blocks of
+        // four waves share a CU, or, where that needs more kx blocks than one block of eight waves with
+        // up to 156 KiB, that block alone on a CU), the unit's column tiles split over `nparts` blocks
+        std::vector<int> g_mt(ngroups), g_np(ngroups), g_nw(ngroups);
         std::vector<size_t> g_lds(ngroups);
         for (size_t gi = 0; gi < ngroups; gi++) {
             const DevGroup& dg = p->groups[gi].dev;
@@ -910,10 +911,24 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
                 const int kb = (nxP + 16 * m - 1) / (16 * m), kb0 = (nxP + 16 * mt - 1) / (16 * mt);
                 if (kb < kb0 || (kb == kb0 && kb * m < kb0 * mt)) mt = m;
             }
+            // each kx block reads all of the unit's G: where four waves need several, eight waves with
+            // more twiddle rows per block (one block per CU) may need fewer
+            int nw = 4;
+            {
+                const int kb4 = (nxP + 16 * mt - 1) / (16 * mt);
+                int bm = mt;
+                for (int m = 3; m <= 6 && kb4 > 1; m++) {
+                    if (lds_of(m) > 156 * 1024) break;
+                    const int kb = (nxP + 16 * m - 1) / (16 * m), kbb = (nxP + 16 * bm - 1) / (16 * bm);
+                    if (kb < kbb || (kb == kbb && nw == 8 && kb * m < kbb * bm)) { bm = m; nw = 8; }
+                }
+                mt = bm;
+            }
             // items of one column tile per wave (the persistent blocks balance over many items)
             const int ntiles = dg.LyP * dg.C / 16;
-            const int np = std::max(1, ntiles / 4);
+            const int np = std::max(1, ntiles / nw);
             g_mt[gi] = mt;
+            g_nw[gi] = nw;
             g_np[gi] = np;
             g_lds[gi] = lds_of(mt);
         }
@@ -931,11 +946,11 @@ extern "C" int mrs_plan_create(const mrs_group_desc* groups, size_t ngroups, con
             const size_t c = cls[std::make_tuple(dg.Q, dg.RB, dg.cplx, dg.eddy_strength != 0.0 ? 1 : 0)];
             const int ch = p->unit_chunk[u];
             (void)c;
-            const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi]);
+            const auto xkey = std::make_tuple(g_mt[gi], g_np[gi], g_lds[gi], g_nw[gi]);
             if (!xcls.count(xkey)) {
                 xcls[xkey] = p->xclasses.size();
                 const int per_cu = (int)std::min<size_t>(2, std::max<size_t>(1, (160 * 1024) / g_lds[gi]));
-                p->xclasses.push_back({g_mt[gi], g_np[gi], g_lds[gi], per_cu * ncu, {}, nullptr, {}});
+                p->xclasses.push_back({g_mt[gi], g_nw[gi], g_np[gi], g_lds[gi], per_cu * ncu, {}, nullptr, {}});
                 xw.emplace_back(p->nchunks);
             }
             const size_t xi = xcls[xkey];
@@ -1162,7 +1177,7 @@ static int enqueue_chunk(mrs_plan* p, int k, hipStream_t st, char* err, size_t e
     for (auto& c : p->yclasses) HIP_TRY(launch_yclass(p, c, k, st));
     for (auto& xc : p->xclasses) {
         const size_t a = xc.off[k], b = xc.off[k + 1];
-        HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(b - a), xc.nblk, p->d_units, p->d_groups,
+        HIP_TRY(launch_xstage_mfma(xc.mt, xc.nw, xc.nparts, xc.lds, (int)(b - a), xc.nblk, p->d_units, p->d_groups,
                                    xc.d_items + a, st));
     }
     return MRS_OK;
@@ -1489,7 +1504,7 @@ extern "C" int mrs_plan_stage_times(mrs_plan* p, void* stream, int32_t iters, do
             HIP_TRY(hipEventRecord(ev[2], st));
             for (auto& xc : p->xclasses)
                 for (int k = 0; k < p->nchunks; k++)
-                    HIP_TRY(launch_xstage_mfma(xc.mt, xc.nparts, xc.lds, (int)(xc.off[k + 1] - xc.off[k]),
+                    HIP_TRY(launch_xstage_mfma(xc.mt, xc.nw, xc.nparts, xc.lds, (int)(xc.off[k + 1] - xc.off[k]),
                                                xc.nblk, p->d_units, p->d_groups, xc.d_items + xc.off[k], st));
         } else {
             for (size_t u = 0; u < p->units.size(); u++) {
