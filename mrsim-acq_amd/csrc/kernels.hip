@@ -506,8 +506,8 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                     float zr[4], zi[4], sr[4], si[4];  // ECHAIN: E of the current line and its step
 #pragma unroll
                     for (int g = 0; g < GP; g++) {
-                        // Every tile runs the same MFMA block (zero B past the last tile), so the
-                        // accumulators stay in place whichever way B was generated.
+                        // Every tile runs the same MFMA block whichever way B was generated, so the
+                        // accumulators stay in place.
                         u32x4 bh[V], bl[V];
 #pragma unroll
                         for (int vv = 0; vv < V; vv++) bh[vv] = bl[vv] = u32x4{0u, 0u, 0u, 0u};
@@ -585,11 +585,15 @@ void ystage_mfma(const DevUnit* __restrict__ units, const DevSlice* __restrict__
                                 }
                             }
                         }
+                        // tiles past the last line (the short last pass of a group whose line tiles
+                        // are no multiple of GP) skip their MFMAs: uniform, and never stored
+                        if (g0 + g < G) {
 #pragma unroll
-                        for (int vv = 0; vv < V; vv++)
+                            for (int vv = 0; vv < V; vv++)
 #pragma unroll
-                            for (int rb = 0; rb < RB; rb++)
-                                acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
+                                for (int rb = 0; rb < RB; rb++)
+                                    acc[vv][rb][g] = mfma3(ahi[rb], alo[rb], bh[vv], bl[vv], acc[vv][rb][g]);
+                        }
                     }
                 } else {
                     if (chain) {
