@@ -291,6 +291,9 @@ __device__ __forceinline__ void read_images(Vox4<Q, CPLX, EDDY, V>& v, const uin
 template <int Q, int RB, int GP, bool CPLX, bool EDDY, int V>
 constexpr int ystage_waves() {
     if constexpr (V > 1) return (V == 2 && (CPLX ? 2 * Q : Q) <= 3) ? 3 : 2;
+    // 32 coils of one compartment (the 3D train): 168 VGPRs at three waves, one register spilled
+    // outside the stage loop, 1-3% faster than two waves
+    if (Q == 1 && RB == 4 && GP == 4 && !CPLX && !EDDY) return 3;
     return (Q <= 3 && RB == 1 && !CPLX && !EDDY) ? 4 : ((RB * GP >= 8 || RB >= 4) ? 2 : 3);
 }
 
