@@ -1664,22 +1664,30 @@ template <int MT, bool ADJ>
 __global__ __launch_bounds__(256) void zstage(const ZSet* __restrict__ sets, const ZBlock* __restrict__ blocks) {
     constexpr int NP = MT * 16;
     __shared__ uint32_t zh[NP][NP + 4], zl[NP][NP + 4];
+    __shared__ uint32_t eh[NP], el[NP];   // split e^{+-i 2 pi m / nz}, m < nz
     const ZBlock blk = blocks[blockIdx.x];
     const ZSet S = sets[blk.set];
     const int nz = S.nz, h = nz / 2;
+    // the table takes nz distinct values (the angle of entry (r, k) is 2 pi m / nz exactly, with
+    // m = (r - h)(k - h) mod nz): evaluate those once, then fill the table from them
+    if ((int)threadIdx.x < nz) {
+        double sv, cv;
+        sincos(kTau * (double)threadIdx.x / (double)nz, &sv, &cv);
+        uint32_t hi, lo;
+        split2((float)cv, ADJ ? -(float)sv : (float)sv, hi, lo);
+        eh[threadIdx.x] = hi;
+        el[threadIdx.x] = lo;
+    }
+    __syncthreads();
     for (int i = threadIdx.x; i < NP * NP; i += 256) {
         const int r = i / NP, k = i % NP;
-        float re = 0.0f, im = 0.0f;
+        uint32_t hi = 0u, lo = 0u;
         if (r < nz && k < nz) {
-            int m = ((r - h) * (k - h)) % nz;   // the angle is 2 pi m / nz exactly
+            int m = ((r - h) * (k - h)) % nz;
             if (m < 0) m += nz;
-            double sv, cv;
-            sincos(kTau * (double)m / (double)nz, &sv, &cv);
-            re = (float)cv;
-            im = ADJ ? -(float)sv : (float)sv;
+            hi = eh[m];
+            lo = el[m];
         }
-        uint32_t hi, lo;
-        split2(re, im, hi, lo);
         zh[r][k] = hi;
         zl[r][k] = lo;
     }
