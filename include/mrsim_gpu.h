@@ -147,7 +147,9 @@ int mrs_build_coil_kspace(const mrs_slice_input* inp, const mrs_acquisition* acq
 int mrs_build_coil_kspaces(const mrs_slice_input* inp, const mrs_acquisition* acq, const mrs_line_timing* timing,
                            const double* log_amp, mrs_c64* out, char* err, size_t errlen);
 
-/* ---------------------------------------------------------------- batched plan (device buffers) */
+/* ---------------------------------------------------------------- batched plan (device buffers)
+ * A plan is used by one host thread at a time (its calls read and update the plan's host state);
+ * distinct plans may be used from different threads.  Work is enqueued on the caller's stream. */
 
 /* Arithmetic of the contractions.  Phases are formed and range-reduced in FP64 in both. */
 enum {
