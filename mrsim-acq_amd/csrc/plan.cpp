@@ -1554,6 +1554,14 @@ extern "C" int mrs_plan_describe(const mrs_plan* p, char* out, size_t outlen) {
     for (const DevSlice& s : p->slices) owners += s.s_owner ? 1 : 0;
     snprintf(line, sizeof line, "coil planes: %zu distinct sets\n", owners);
     r += line;
+    for (size_t c = 0; c < p->xclasses.size(); c++) {
+        const auto& xc = p->xclasses[c];
+        snprintf(line, sizeof line,
+                 "x-stage class %zu: %d kx tiles and %d waves per block, %d column-tile parts per unit, %zu bytes of "
+                 "LDS, %d persistent blocks, %zu items\n",
+                 c, xc.mt, xc.nw, xc.nparts, xc.lds, xc.nblk, xc.items.size());
+        r += line;
+    }
     for (size_t g = 0; g < p->groups.size(); g++) {
         const DevGroup& d = p->groups[g].dev;
         snprintf(line, sizeof line,

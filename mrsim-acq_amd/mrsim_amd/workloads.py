@@ -176,8 +176,9 @@ def make(name: str, n_slices=None, n_volumes=None, seed_base=0x5EED0000) -> Work
             w.units.append((s, sd.mag[None, :].astype(np.float32)))
         return w
     if name in ("tile_96", "tile_192"):
-        # x-stage tiles of three 16-row kx tiles: 48 kx in one block (96^2 -> 48^2, 8 coils,
-        # 2 compartments) and 96 kx in two (192^2 -> 96^2, 4 coils)
+        # x-stage blocks of 48 and 96 kx: 48 kx in one block of three 16-row kx tiles (96^2 -> 48^2,
+        # 8 coils, 2 compartments) and 96 kx in one block of six tiles and eight waves (192^2 -> 96^2,
+        # 4 coils)
         sn, n, C, Q = (96, 48, 8, 2) if name == "tile_96" else (192, 96, 4, 1)
         acq = abi.Acquisition.default(t_line=0.5, t_echo=30.0, signal_scale=1.0, n_coils=C)
         w = Workload(name, sn, sn, n, n, C, Q, acq)

@@ -210,10 +210,30 @@ def test_pf_256_matches_fp64_oracle():
 
 @pytest.mark.parametrize("name", ["tile_96", "tile_192"])
 def test_three_row_tile_x_stage_matches_fp64_oracle(name):
-    # 48 kx in one block of three 16-row tiles, and 96 kx in two such blocks
+    # 48 kx in one block of three 16-row tiles; 96 kx in one block of six tiles and eight waves
     w = workloads.make(name)
     errs = _plan_vs_oracle(w, abi.PREC_SPLIT_BF16, [0])
     print(f"{name} rel-L2 vs FP64: {errs}", flush=True)
+    assert max(errs) <= TOL
+
+
+@pytest.mark.parametrize("name,ghost,mt,nw", [("tile_192", 0.0, 6, 8), ("tile_192", 0.015, 3, 8),
+                                              ("slice_128", 0.015, 4, 8), ("pf_256", 0.015, 1, 4)])
+def test_x_stage_block_shapes_match_fp64_oracle(name, ghost, mt, nw):
+    # The planner's x-stage blocks hold 16*MT kx rows of twiddles (both ghost tables where the group
+    # has a ghost): four waves within 80 KB of LDS, or eight waves within 156 KB where that needs fewer
+    # kx blocks.  Covered: 96 kx in one eight-wave block; with ghost tables 96 kx in two eight-wave
+    # blocks of 48 rows (restaging between kx blocks), 64 kx in one, and 128 kx of a 256-column grid
+    # in eight four-wave blocks of 16 rows.  Each against the FP64 oracle.
+    w = workloads.make(name, n_slices=1) if name == "pf_256" else workloads.make(name)
+    if ghost:
+        w.acq = w.acq.copy(ghost_offset=ghost)
+    plan, outs, keep = workloads.to_plan(w, abi.PREC_SPLIT_BF16)
+    desc = plan.describe()
+    plan.close()
+    assert f"x-stage class 0: {mt} kx tiles and {nw} waves per block" in desc, desc
+    errs = _plan_vs_oracle(w, abi.PREC_SPLIT_BF16, [0])
+    print(f"{name} ghost {ghost}: MT {mt}, {nw} waves, rel-L2 vs FP64: {errs}", flush=True)
     assert max(errs) <= TOL
 
 
